@@ -1,0 +1,38 @@
+"""Writes workloads.json beside this file: the census rows and completion bounds of the two
+divisor families the reference logged, so instances can be rebuilt where the reference is absent.
+
+Sources (read as data): gpu/divisor_family_lambdaJ.json (118 rows, bound U = N of
+results_k64_best_global.json), gpu/mstar_divisor_family.json (111 rows, bound U = N0 of
+results_k64_best_global_prev_148_539839443357.json), and the positive-control incumbents
+results_k64*.json (factor lists and moduli).
+Run once in a checkout that has the reference:  python extract_workloads.py /path/to/reference
+"""
+import json
+import os
+import sys
+
+
+def main(ref: str) -> None:
+    out = {"families": {}, "incumbents": {}}
+    N = json.load(open(os.path.join(ref, "results_k64_best_global.json")))
+    N0 = json.load(open(os.path.join(ref, "results_k64_best_global_prev_148_539839443357.json")))
+    lam = json.load(open(os.path.join(ref, "gpu", "divisor_family_lambdaJ.json")))
+    m0 = json.load(open(os.path.join(ref, "gpu", "mstar_divisor_family.json")))
+    keep = ("D", "s", "cap", "pool_capped", "rmax")
+    out["families"]["lambdaN"] = {
+        "U": str(N["n"]), "anchor": str(N["modulus"]), "census": "gpu/divisor_family_lambdaJ.json",
+        "rows": [{k: row[k] for k in keep} for row in lam["survivors"]]}
+    out["families"]["M0"] = {
+        "U": str(N0["n"]), "anchor": str(N0["modulus"]), "census": "gpu/mstar_divisor_family.json",
+        "rows": [{k: row[k] for k in keep} for row in (m0["survivors"] if isinstance(m0, dict) else m0)]}
+    for f in ("results_k64.json", "results_k64_best_global.json",
+              "results_k64_best_global_prev_148_539839443357.json", "results_k64_best_global_prev_148_N.json"):
+        d = json.load(open(os.path.join(ref, f)))
+        out["incumbents"][f] = {"n": str(d["n"]), "modulus": str(d["modulus"]),
+                                "factors": [int(p) for p in d["factors"]]}
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "workloads.json"), "w") as fh:
+        json.dump(out, fh, indent=0)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,96 @@
+"""Whole-instance solves on the device.
+
+solve(inst)  -- one instance per call ("plain"): D side generated with the Dmin filter and
+               sorted, I side expanded and streamed through sort + join in chunks, every
+               equal-residue pair returned. The device half of
+               gpu/mitm_gpu_mstar_driver.py:209-243 (DFILTER=1, IGEN=gpu) in one C call.
+decode(inst, pairs) / recheck(inst, pairs) -- host decode and exact re-check of the hits
+               (gpu/mitm_gpu_mstar_driver.py:224-237).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import lib as L
+from . import verify
+
+
+@dataclass
+class SolveResult:
+    pairs_d: np.ndarray          # D ids (uint64)
+    pairs_i: np.ndarray          # I ids (uint64)
+    nD: int
+    nI: int
+    stats: dict = field(default_factory=dict)
+
+    @property
+    def keys_joined(self) -> int:
+        return self.nD + self.nI
+
+    def pair_array(self) -> np.ndarray:
+        """Pairs as sorted (D id, I id) rows: the canonical form compared for bit-exactness."""
+        a = np.stack([self.pairs_d, self.pairs_i], axis=1) if len(self.pairs_d) else np.zeros((0, 2), np.uint64)
+        if len(a):
+            a = a[np.lexsort((a[:, 1], a[:, 0]))]
+        return a
+
+
+def id_modes(inst):
+    i_mode = L.CF_ID_COLEX if inst.rank_ids else L.CF_ID_PACK_ASC
+    d_mode = L.CF_ID_MASK if inst.r > 10 else L.CF_ID_PACK_DESC
+    return i_mode, d_mode
+
+
+def make_cinstance(inst, keep: list) -> L.cf_instance:
+    B = L.u64_array(inst.B0)
+    INS = L.u64_array(inst.INS)
+    keep += [B, INS]
+    i_mode, d_mode = id_modes(inst)
+    ilo, ihi = L.split128(inst.Icap)
+    dlo, dhi = L.split128(inst.Dmin)
+    return L.cf_instance(M=inst.M, r=inst.r, nB=len(B), B=B.ctypes.data_as(ctypes.c_void_p), nI=len(INS),
+                         hmax=int(inst.hmax), INS=INS.ctypes.data_as(ctypes.c_void_p), P0modM=inst.P0 % inst.M,
+                         icap_lo=ilo, icap_hi=ihi, dmin_lo=dlo, dmin_hi=dhi, split_idx=int(inst.split_idx),
+                         i_id_mode=i_mode, d_id_mode=d_mode, pad=0)
+
+
+def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0) -> SolveResult:
+    ctx = ctx or L.default_context()
+    keep: List = []
+    ci = make_cinstance(inst, keep)
+    od, oi = ctypes.c_void_p(), ctypes.c_void_p()
+    n = L.c_u64()
+    st = L.cf_stats()
+    L.check(L.load().cf_solve(ctx.handle, ctypes.byref(ci), max_chunk, ctypes.byref(od), ctypes.byref(oi),
+                              ctypes.byref(n), ctypes.byref(st)), "cf_solve")
+    pd = np.empty(n.value, np.uint64)
+    pi = np.empty(n.value, np.uint64)
+    if n.value:
+        ctx.d2h(pd, od.value)
+        ctx.d2h(pi, oi.value)
+        ctx.free(od.value)
+        ctx.free(oi.value)
+    return SolveResult(pd, pi, int(st.nD), int(st.nI), st.as_dict())
+
+
+def decoders(inst):
+    """(D id -> index tuple, I id -> index tuple) for the instance's id encodings."""
+    if inst.r > 10:
+        d_dec = verify.decode_bitmask
+    else:
+        d_dec = lambda v: verify.decode_packed(v, inst.r, 6)  # noqa: E731
+    if inst.rank_ids:
+        i_dec = lambda v: verify.unrank_colex(v, inst.r, len(inst.INS))  # noqa: E731
+    else:
+        i_dec = lambda v: verify.decode_packed(v, inst.r, 8)  # noqa: E731
+    return d_dec, i_dec
+
+
+def recheck(inst, res: SolveResult) -> List[dict]:
+    """Exact host re-check of every pair (residue identity, integrality, Korselt)."""
+    d_dec, i_dec = decoders(inst)
+    return verify.recheck_pairs(inst, res.pairs_d, res.pairs_i, d_dec, i_dec)

@@ -1,0 +1,161 @@
+// cf_internal.h -- shared declarations of libcfjoin (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "modarith.h"
+
+// ---- errors -----------------------------------------------------------------------------------
+void cf_set_error(const std::string &msg);
+struct CfError {
+    int code;
+    std::string msg;
+};
+#define CF_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw CfError{-3, std::string(#call) + ": " + hipGetErrorString(e_)};           \
+    } while (0)
+#define CF_REQUIRE(cond, msg)                        \
+    do {                                             \
+        if (!(cond)) throw CfError{-1, (msg)};       \
+    } while (0)
+
+// ---- context: one device, one stream, a caching allocator ---------------------------------
+struct cf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::multimap<size_t, void *> free_blocks;    // cached blocks by size
+    std::map<void *, size_t> live;                // live allocations and their block size
+    size_t cached_bytes = 0;
+    std::mutex mu;
+    void *pinned = nullptr;                       // small pinned staging buffer for scalars
+    size_t pinned_bytes = 0;
+};
+
+void *ctx_alloc(cf_ctx *ctx, size_t bytes);
+void ctx_free(cf_ctx *ctx, void *p);
+void ctx_release(cf_ctx *ctx);
+
+// RAII device buffer from the context pool.
+template <typename T>
+struct DBuf {
+    cf_ctx *ctx = nullptr;
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(cf_ctx *c, size_t count) : ctx(c), n(count) { p = (T *)ctx_alloc(c, count * sizeof(T)); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        reset();
+        ctx = o.ctx; p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p) ctx_free(ctx, p);
+        p = nullptr;
+        n = 0;
+    }
+    T *release() { T *q = p; p = nullptr; n = 0; return q; }
+};
+
+template <typename T>
+void h2d(cf_ctx *ctx, T *dst, const T *src, size_t n) {
+    if (n) CF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+}
+template <typename T>
+void d2h(cf_ctx *ctx, T *dst, const T *src, size_t n) {
+    if (n) CF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+}
+u64 d2h_u64(cf_ctx *ctx, const u64 *src);     // one scalar, synchronous
+
+// ---- enumeration tree ---------------------------------------------------------------------
+// The r-subsets of one side are the leaves of a tree whose depth-d nodes are the d-prefixes
+// that still admit a valid completion. The bounds make the admissible children of every node
+// a contiguous index range that starts at the node's `nxt`, so nodes are expanded level by
+// level with a count -> scan -> write pass and no per-thread DFS.
+enum : int { SIDE_I = 0, SIDE_D = 1 };
+enum : int {
+    ID_COLEX = 0,      // sum_j C(i_j, j + 1) over ascending positions
+    ID_PACK_ASC = 1,   // index of position d at bits [bits*d, bits*d + bits)
+    ID_PACK_DESC = 2,  // descending choice order; position d lands at field r-1-d (6-bit D ids)
+    ID_MASK = 3,       // 64-bit deletion mask
+};
+
+struct __align__(16) Node {
+    u64 A_lo, A_hi;  // exact product of the chosen primes
+    u64 xm, x2;      // residue state (odd part, 2-adic part), includes the side multiplier
+    u64 id;          // partial record id
+    u32 pos;         // bits 0..15: first candidate index of the next choice as int16 (I: last+1,
+                     // D: last-1); bits 16..31: the index chosen at depth 0
+    u32 meta;        // bits 0..7: primes chosen above the stratum split; bits 8..31: instance
+};
+static_assert(sizeof(Node) == 48, "node layout");
+
+// Device-side description of one instance side (I or D). Arrays live in device memory.
+struct SideDesc {
+    ModCtx mc;
+    const u64 *p;       // primes (I ascending; D ascending base)
+    const u64 *pR;      // p * 2^64 mod m
+    const U128 *bound;  // I: H[t*n + x] = floor((Icap-1) / (a_x..a_{x+t})); D: G[t*n + i] = ceil(Dmin / (b_i..b_{i-t}))
+    const u64 *binom;   // colex table C(x, j) at [x*(r+1) + j], x <= n, j <= r (ID_COLEX only)
+    u64 koff;           // offset added to every key (disjoint key ranges across a batch)
+    int n, r, side, id_mode, id_bits, split_idx, hmax, pad;
+    u64 xm0, x20;       // initial residue state (the side multiplier)
+};
+
+// Host-owned copy of one side's device tables.
+struct SideTables {
+    SideDesc desc{};
+    DBuf<u64> p, pR, binom;
+    DBuf<U128> bound;
+};
+
+struct SideSpec {  // host description of one side
+    int side = SIDE_I;
+    std::vector<u64> primes;
+    int r = 0;
+    u64 M = 1;
+    u64 mult = 1;                    // key = mult * prod mod M
+    unsigned __int128 cap = 0;       // I: Icap (exclusive); D: Dmin (inclusive)
+    int split_idx = 1 << 30, hmax = 1 << 30;
+    int id_mode = ID_COLEX, id_bits = 8;
+    u64 koff = 0;
+};
+
+void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t);
+
+// ---- kernels (host launchers) -------------------------------------------------------------
+// exclusive scan of u32 or u64 counts into u64 offsets; returns the total (synchronous)
+u64 scan_u32_to_u64(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);
+u64 scan_u64(cf_ctx *ctx, const u64 *in, u64 *out, size_t n);
+
+// stable LSD radix sort of (key, val) by the low `bits` bits of key; in-place via scratch
+void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits);
+
+// tree expansion
+void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, int depth, u32 *cnt);
+// `off` is the exclusive scan of the per-node counts with nn + 1 entries (off[nn] = total)
+void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                         size_t nout, int depth, Node *out);
+void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                       u64 leaf_lo, size_t nleaf, int depth, u64 *keys, u64 *ids);
+void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
+void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out);
+u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
+
+// joins
+u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off);
+void join_sorted_write(cf_ctx *ctx, const u64 *Did, const u64 *Iid, size_t nI, const u64 *lo, const u64 *cnt,
+                       const u64 *off, u64 *od, u64 *oi, u64 *okey, const u64 *Ik);

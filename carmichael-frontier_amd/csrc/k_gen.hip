@@ -1,0 +1,331 @@
+// k_gen.hip -- r-subset residue generation on gfx950.
+//
+// Two generators:
+//  * tree expansion (filtered sides): the r-subsets admitted by the size bounds are the leaves
+//    of a prefix tree. Because every bound is monotone in the next index (products grow with the
+//    index, the stratum count never falls), the admissible children of a node form one
+//    contiguous index range starting at the node's next candidate. A level is expanded by
+//      count (binary search of the range end)  ->  exclusive scan  ->  write (one thread per child)
+//    so work is balanced across lanes however lopsided the subtrees are, and outputs come out
+//    in the depth-first (lexicographic) order of the reference's recursive enumerators:
+//      I side (ascending, prod < Icap, cheapest completion, stratum): src/mitm64_idump_stream2.cpp:47-57
+//      D side (descending, prod >= Dmin reach prune):                 src/mitm64_ddump.cpp:29-44
+//  * rank ranges (unfiltered, compatibility generate()): one lane per combinatorial-number-system
+//    rank, lexicographic unranking as gpu/mitm_gpu2.py:40-52.
+#include "cf_internal.h"
+
+namespace {
+
+constexpr int BLK = 256;
+
+__device__ __forceinline__ int node_nxt(const Node &nd) { return (int)(short)(nd.pos & 0xFFFFu); }
+__device__ __forceinline__ int node_first(const Node &nd) { return (int)(nd.pos >> 16); }
+
+// Number of primes with index in [x+1, x+t] at or above split_idx: the fewest "large" primes any
+// completion after choosing x can contain (indices only grow).
+__device__ __forceinline__ int min_large_after(int x, int t, int split_idx) {
+    int lo = x + 1 > split_idx ? x + 1 : split_idx;
+    int c = x + t - lo + 1;
+    return c > 0 ? c : 0;
+}
+
+// Is index x an admissible choice at a node with product A, `nl` large primes so far, t more
+// choices to follow?  I side: A * a_x * ... * a_{x+t} <= Icap - 1 and the stratum can still hold.
+__device__ __forceinline__ bool valid_i(const SideDesc &s, U128 A, int nl, int x, int t) {
+    const U128 h = s.bound[(size_t)t * s.n + x];
+    if (u128_gt(A, h)) return false;
+    const int nlx = nl + (x >= s.split_idx);
+    return nlx + min_large_after(x, t, s.split_idx) <= s.hmax;
+}
+// D side: A * b_i * b_{i-1} * ... * b_{i-t} >= Dmin, i.e. A >= G[t][i].
+__device__ __forceinline__ bool valid_d(const SideDesc &s, U128 A, int i, int t) {
+    return !u128_lt(A, s.bound[(size_t)t * s.n + i]);
+}
+
+// Children of a depth-`depth` node: returns the count; the children are nxt, nxt+1, ... (I) or
+// nxt, nxt-1, ... (D).
+__device__ int child_count(const SideDesc &s, const Node &nd, int depth) {
+    const int t = s.r - depth - 1;
+    const U128 A = {nd.A_lo, nd.A_hi};
+    const int nxt = node_nxt(nd);
+    if (s.side == SIDE_I) {
+        const int nl = (int)(nd.meta & 0xFFu);
+        int lo = nxt, hi = s.n - t;   // candidates [lo, hi)
+        if (lo >= hi || !valid_i(s, A, nl, lo, t)) return 0;
+        // largest valid x: valid on [lo, v], invalid on (v, hi)
+        int a = lo, b = hi - 1;
+        while (a < b) {
+            int mid = (a + b + 1) >> 1;
+            if (valid_i(s, A, nl, mid, t)) a = mid; else b = mid - 1;
+        }
+        return a - lo + 1;
+    } else {
+        int lo = t, hi = nxt;          // candidates hi, hi-1, ..., lo
+        if (hi < lo || !valid_d(s, A, hi, t)) return 0;
+        int a = lo, b = hi;            // smallest valid i
+        while (a < b) {
+            int mid = (a + b) >> 1;
+            if (valid_d(s, A, mid, t)) b = mid; else a = mid + 1;
+        }
+        return hi - a + 1;
+    }
+}
+
+__device__ __forceinline__ u64 id_update(const SideDesc &s, u64 id, int x, int depth) {
+    switch (s.id_mode) {
+        case ID_COLEX: return id + s.binom[(size_t)x * (s.r + 1) + depth + 1];
+        case ID_PACK_ASC: return id | ((u64)x << (s.id_bits * depth));
+        case ID_PACK_DESC: return id | ((u64)x << (s.id_bits * (s.r - 1 - depth)));
+        default: return id | (1ULL << x);
+    }
+}
+
+// Child number k of a depth-`depth` node.
+__device__ __forceinline__ Node make_child(const SideDesc &s, const Node &nd, int depth, int k) {
+    const int x = (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
+    const u64 p = s.p[x];
+    Node c;
+    const U128 A = u128_mul64(U128{nd.A_lo, nd.A_hi}, p);
+    c.A_lo = A.lo;
+    c.A_hi = A.hi;
+    c.xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
+    c.x2 = nd.x2 * p;
+    c.id = id_update(s, nd.id, x, depth);
+    const int nn = (s.side == SIDE_I) ? x + 1 : x - 1;
+    const u32 first = depth == 0 ? (u32)x : (nd.pos >> 16);
+    c.pos = ((u32)(unsigned short)(short)nn) | (first << 16);
+    c.meta = (nd.meta & ~0xFFu) | ((nd.meta & 0xFFu) + (s.side == SIDE_I && x >= s.split_idx));
+    return c;
+}
+
+__global__ void __launch_bounds__(BLK) k_expand_count(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
+                                                      size_t nn, int depth, u32 *__restrict__ cnt) {
+    for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) {
+        const Node nd = nodes[j];
+        const SideDesc &s = descs[nd.meta >> 8];
+        cnt[j] = (u32)child_count(s, nd, depth);
+    }
+}
+
+// Parent of output j: the p with off[p] <= j < off[p+1] (off = exclusive scan, off[nn] = total),
+// searched within [plo, phi].
+__device__ __forceinline__ size_t find_parent(const u64 *__restrict__ off, size_t plo, size_t phi, u64 j) {
+    while (plo < phi) {
+        size_t mid = (plo + phi + 1) >> 1;
+        if (off[mid] <= j) plo = mid; else phi = mid - 1;
+    }
+    return plo;
+}
+
+// Block-level narrowing: the block covers outputs [j0, j1); its parents lie in [p0, p1].
+__device__ __forceinline__ void block_parent_range(const u64 *__restrict__ off, size_t nn, u64 j0, u64 j1,
+                                                   size_t *p0, size_t *p1) {
+    __shared__ size_t s_p[2];
+    if (threadIdx.x == 0) s_p[0] = find_parent(off, 0, nn - 1, j0);
+    if (threadIdx.x == 1) s_p[1] = find_parent(off, 0, nn - 1, j1 - 1);
+    __syncthreads();
+    *p0 = s_p[0];
+    *p1 = s_p[1];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(BLK) k_expand_write(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
+                                                      size_t nn, const u64 *__restrict__ off, size_t nout, int depth,
+                                                      Node *__restrict__ out) {
+    const size_t ntiles = (nout + BLK - 1) / BLK;
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const u64 j0 = (u64)tile * BLK;
+        const u64 j1 = j0 + BLK < nout ? j0 + BLK : nout;
+        size_t p0, p1;
+        block_parent_range(off, nn, j0, j1, &p0, &p1);
+        const u64 j = j0 + threadIdx.x;
+        if (j < j1) {
+            const size_t p = find_parent(off, p0, p1, j);
+            const Node nd = nodes[p];
+            const SideDesc &s = descs[nd.meta >> 8];
+            out[j] = make_child(s, nd, depth, (int)(j - off[p]));
+        }
+    }
+}
+
+// Leaves [leaf_lo, leaf_lo + nleaf) of the depth-(r-1) nodes; writes record j - leaf_lo.
+__global__ void __launch_bounds__(BLK) k_leaf_write(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
+                                                    size_t nn, const u64 *__restrict__ off, u64 leaf_lo, size_t nleaf,
+                                                    int depth, u64 *__restrict__ keys, u64 *__restrict__ ids) {
+    const size_t ntiles = (nleaf + BLK - 1) / BLK;
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const u64 j0 = leaf_lo + (u64)tile * BLK;
+        const u64 jend = leaf_lo + nleaf;
+        const u64 j1 = j0 + BLK < jend ? j0 + BLK : jend;
+        size_t p0, p1;
+        block_parent_range(off, nn, j0, j1, &p0, &p1);
+        const u64 j = j0 + threadIdx.x;
+        if (j < j1) {
+            const size_t p = find_parent(off, p0, p1, j);
+            const Node nd = nodes[p];
+            const SideDesc &s = descs[nd.meta >> 8];
+            const int k = (int)(j - off[p]);
+            const int x = (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
+            const u64 xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
+            const u64 x2 = nd.x2 * s.p[x];
+            keys[j - leaf_lo] = cf_key(xm, x2, s.mc) + s.koff;
+            ids[j - leaf_lo] = id_update(s, nd.id, x, depth);
+        }
+    }
+}
+
+// Per first index: total leaves below it (atomic accumulation; nodes of one first index are
+// contiguous but a plain atomic is simpler and the node count is small next to the leaves).
+__global__ void k_first_counts(const Node *__restrict__ nodes, const u32 *__restrict__ cnt, size_t nn,
+                               unsigned long long *__restrict__ out) {
+    for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) {
+        const u32 c = cnt[j];
+        if (c) atomicAdd(&out[nodes[j].pos >> 16], (unsigned long long)c);
+    }
+}
+
+// Compatibility generator: lane per CNS rank in [lo, lo + count), lexicographic unranking,
+// key = mult * prod mod M, id = indices packed id_bits wide (gpu/mitm_gpu2.py:36-64).
+__global__ void __launch_bounds__(BLK) k_gen_rank(const SideDesc *__restrict__ descs, u64 lo, size_t count,
+                                                  u64 *__restrict__ keys, u64 *__restrict__ ids) {
+    const SideDesc &s = descs[0];
+    const int n = s.n, R = s.r, stride = R + 1;
+    for (size_t t = (size_t)blockIdx.x * BLK + threadIdx.x; t < count; t += (size_t)gridDim.x * BLK) {
+        u64 rank = lo + t;
+        u64 xm = s.xm0, x2 = s.x20, id = 0;
+        int start = 0;
+        for (int pos = 0; pos < R; ++pos) {
+            const int remaining = R - pos - 1;
+            int c = start;
+            for (; c < n - 1; ++c) {
+                const u64 cntc = s.binom[(size_t)(n - 1 - c) * stride + remaining];
+                if (rank < cntc) break;
+                rank -= cntc;
+            }
+            start = c + 1;
+            xm = cf_mulmod_m(xm, s.pR[c], s.mc);
+            x2 *= s.p[c];
+            id |= (u64)c << (s.id_bits * pos);
+        }
+        keys[t] = cf_key(xm, x2, s.mc) + s.koff;
+        ids[t] = id;
+    }
+}
+
+inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 1u << 20) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    return (unsigned)(g > cap ? cap : g);
+}
+
+}  // namespace
+
+void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, int depth, u32 *cnt) {
+    if (!nn) return;
+    hipLaunchKernelGGL(k_expand_count, dim3(grid_for(nn, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, depth, cnt);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                         size_t nout, int depth, Node *out) {
+    if (!nout) return;
+    hipLaunchKernelGGL(k_expand_write, dim3(grid_for(nout, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, off, nout,
+                       depth, out);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off, u64 leaf_lo,
+                       size_t nleaf, int depth, u64 *keys, u64 *ids) {
+    if (!nleaf) return;
+    hipLaunchKernelGGL(k_leaf_write, dim3(grid_for(nleaf, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, off,
+                       leaf_lo, nleaf, depth, keys, ids);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out) {
+    if (!nn) return;
+    hipLaunchKernelGGL(k_first_counts, dim3(grid_for(nn, BLK, 4096)), dim3(BLK), 0, ctx->stream, nodes, cnt, nn,
+                       (unsigned long long *)out);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_gen_rank, dim3(grid_for(count, BLK)), dim3(BLK), 0, ctx->stream, d_desc, lo, count, keys, ids);
+    CF_HIP(hipGetLastError());
+}
+
+// ---- host: per-side tables ---------------------------------------------------------------
+typedef unsigned __int128 u128;
+
+void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t) {
+    const int n = (int)s.primes.size();
+    const int r = s.r;
+    CF_REQUIRE(n >= 0 && n <= 1024, "pool size must be <= 1024");
+    CF_REQUIRE(r >= 1 && r <= 16, "radius must be in [1, 16]");
+    CF_REQUIRE(s.M >= 1, "modulus must be >= 1");
+    SideDesc d{};
+    d.mc = cf_modctx(s.M);
+    d.n = n;
+    d.r = r;
+    d.side = s.side;
+    d.id_mode = s.id_mode;
+    d.id_bits = s.id_bits;
+    d.split_idx = s.split_idx;
+    d.hmax = s.hmax;
+    d.koff = s.koff;
+    const u64 mult = s.mult % s.M;
+    d.xm0 = d.mc.m == 1 ? 0 : mult % d.mc.m;
+    d.x20 = mult;
+
+    std::vector<u64> pR(n);
+    for (int i = 0; i < n; ++i) pR[i] = d.mc.m == 1 ? 0 : cf_montmul_factor(s.primes[i], d.mc);
+    // bound table, exact: I: floor((Icap-1)/W), W = a_x..a_{x+t}; D: ceil(Dmin/W), W = b_i..b_{i-t}
+    std::vector<U128> bound((size_t)r * (n ? n : 1), U128{0, 0});
+    const u128 ALL = ~(u128)0;
+    for (int tt = 0; tt < r; ++tt) {
+        for (int x = 0; x < n; ++x) {
+            u128 w = 1;
+            bool over = false;
+            bool in_range = (s.side == SIDE_I) ? (x + tt < n) : (x - tt >= 0);
+            if (!in_range) {
+                // never consulted (the candidate range excludes it); make it reject
+                bound[(size_t)tt * n + x] = (s.side == SIDE_I) ? U128{0, 0} : U128{~0ULL, ~0ULL};
+                continue;
+            }
+            for (int j = 0; j <= tt; ++j) {
+                u64 q = s.primes[s.side == SIDE_I ? x + j : x - j];
+                if (w > ALL / q) { over = true; break; }
+                w *= q;
+            }
+            u128 v;
+            if (s.side == SIDE_I) {
+                u128 top = s.cap ? s.cap - 1 : 0;
+                v = over ? 0 : top / w;
+            } else {
+                v = over ? (s.cap ? 1 : 0) : (s.cap + w - 1) / w;
+                if (!over && s.cap > ALL - (w - 1)) v = s.cap / w + (s.cap % w != 0);
+            }
+            bound[(size_t)tt * n + x] = U128{(u64)v, (u64)(v >> 64)};
+        }
+    }
+    std::vector<u64> binom((size_t)(n + 1) * (r + 1), 0);
+    for (int a = 0; a <= n; ++a) {
+        binom[(size_t)a * (r + 1)] = 1;
+        for (int b = 1; b <= r && a > 0; ++b)
+            binom[(size_t)a * (r + 1) + b] = binom[(size_t)(a - 1) * (r + 1) + b - 1] + binom[(size_t)(a - 1) * (r + 1) + b];
+    }
+    t.p = DBuf<u64>(ctx, n ? n : 1);
+    t.pR = DBuf<u64>(ctx, n ? n : 1);
+    t.bound = DBuf<U128>(ctx, bound.size());
+    t.binom = DBuf<u64>(ctx, binom.size());
+    h2d(ctx, t.p.p, s.primes.data(), n);
+    h2d(ctx, t.pR.p, pR.data(), n);
+    h2d(ctx, t.bound.p, bound.data(), bound.size());
+    h2d(ctx, t.binom.p, binom.data(), binom.size());
+    d.p = t.p.p;
+    d.pR = t.pR.p;
+    d.bound = t.bound.p;
+    d.binom = t.binom.p;
+    t.desc = d;
+}

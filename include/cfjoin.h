@@ -1,0 +1,143 @@
+/* cfjoin.h -- C ABI of libcfjoin.so, the MI355X (gfx950) exchange-MITM residue join.
+ *
+ * The reference reaches its GPU path through two Python modules (gpu/mitm_gpu2.py,
+ * gpu/igen_gpu.py) that JIT CUDA kernels through CuPy, plus two C++ record dumpers reached through
+ * pipes (src/mitm64_ddump.cpp, src/mitm64_idump_stream2.cpp). This library replaces the device
+ * side of all four; each entry point below names the reference routine it stands in for.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative status: -1 a violated precondition (the
+ *    reference raises AssertionError for the same conditions), -2 out of device memory, -3 a HIP
+ *    runtime error, -4 an internal consistency check failed. cf_last_error() returns the message of
+ *    the last failure on the calling thread.
+ *  - Pointers named *_dev are device addresses (from cf_alloc or any HIP allocation on the
+ *    context's device); all others are host addresses. Sizes are element counts unless named *bytes.
+ *  - One context = one device + one HIP stream. Calls on one context are ordered; a context must
+ *    not be used from two threads at once.
+ *  - Keys are canonical residues in [0, M), M < 2^64. Ids follow the reference encodings
+ *    (packed indices, 64-bit deletion masks or colex ranks).
+ */
+#ifndef CFJOIN_H
+#define CFJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cf_ctx cf_ctx;
+typedef struct cf_plan cf_plan;
+
+#define CF_OK 0
+#define CF_EINVAL -1
+#define CF_ENOMEM -2
+#define CF_EHIP -3
+#define CF_EINTERNAL -4
+
+/* id encodings (cf_plan_create_*) */
+#define CF_ID_COLEX 0     /* sum_j C(i_j, j+1): src/mitm64_idump_stream2.cpp:54 (IDUMP_RANK=1) */
+#define CF_ID_PACK_ASC 1  /* 8-bit packed ascending: src/mitm64_idump_stream2.cpp:55 */
+#define CF_ID_PACK_DESC 2 /* 6-bit packed ascending indices of a descending D choice: src/mitm64_ddump.cpp:33 */
+#define CF_ID_MASK 3      /* 64-bit deletion mask: src/mitm64_ddump.cpp:32 */
+
+/* ---- version, errors ------------------------------------------------------------------- */
+int cf_version(void);
+const char *cf_last_error(void);
+
+/* ---- context and device memory ----------------------------------------------------------
+ * Replace the CuPy plumbing the drivers use: cp.asarray / cp.asnumpy / .get()
+ * (gpu/mitm_gpu_mstar_driver.py:112,224,254), the memory pool and free_all_blocks()
+ * (gpu/mitm_gpu_mstar_driver.py:114,238), cp.cuda.Device().mem_info (:75). */
+int cf_ctx_create(int device, cf_ctx **out);
+int cf_ctx_destroy(cf_ctx *ctx);
+int cf_alloc(cf_ctx *ctx, uint64_t bytes, void **out_dev);
+int cf_free(cf_ctx *ctx, void *p_dev);
+int cf_pool_release(cf_ctx *ctx);                                   /* free_all_blocks() */
+int cf_pool_stats(cf_ctx *ctx, uint64_t *live_bytes, uint64_t *cached_bytes);
+int cf_mem_info(cf_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+int cf_memcpy_h2d(cf_ctx *ctx, void *dst_dev, const void *src, uint64_t bytes);
+int cf_memcpy_d2h(cf_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes);  /* synchronous */
+int cf_memcpy_d2d(cf_ctx *ctx, void *dst_dev, const void *src_dev, uint64_t bytes);
+int cf_synchronize(cf_ctx *ctx);
+
+/* ---- r-subsets by rank range ------------------------------------------------------------
+ * Replaces gen_records + generate() (gpu/mitm_gpu2.py:25-98): records of the r-combinations of
+ * primes[0..n) with combinatorial-number-system rank in [lo, lo+count), in rank order;
+ * key = mult * prod mod M, id = the r indices packed `bits` wide, position 0 lowest.
+ * Preconditions as gpu/mitm_gpu2.py:83-88 except that any M < 2^64 is accepted. */
+int cf_gen_subsets(cf_ctx *ctx, const uint64_t *primes, int n, int r, int bits, uint64_t M, uint64_t mult,
+                   uint64_t lo, uint64_t count, uint64_t *keys_dev, uint64_t *ids_dev);
+
+/* ---- sort one side ----------------------------------------------------------------------
+ * Replaces sort_side() (gpu/mitm_gpu2.py:101-104): stable sort of (keys, ids) by key, in place. */
+int cf_sort_pairs(cf_ctx *ctx, uint64_t *keys_dev, uint64_t *ids_dev, uint64_t n);
+
+/* ---- join two sorted sides --------------------------------------------------------------
+ * Replaces join_sorted() (gpu/mitm_gpu2.py:107-119): every (D, I) with equal key, I-major then
+ * D order. Output arrays are allocated from the context pool (release with cf_free); when the
+ * join is empty both are NULL. */
+int cf_join_sorted(cf_ctx *ctx, const uint64_t *Dk_dev, const uint64_t *Did_dev, uint64_t nD,
+                   const uint64_t *Ik_dev, const uint64_t *Iid_dev, uint64_t nI,
+                   uint64_t **out_d_dev, uint64_t **out_i_dev, uint64_t *npairs);
+
+/* ---- filtered enumeration plans ---------------------------------------------------------
+ * cf_plan_create_i replaces the igen kernel and its host set-up (gpu/igen_gpu.py:30-111) and
+ * the record set of src/mitm64_idump_stream2.cpp:47-57: ascending r-subsets I of INS[0..n)
+ * with prod(I) < Icap (Icap = icap_hi:icap_lo), at most hmax indices >= split_idx;
+ * key = P0modM * prod(I) mod M. Preconditions of gpu/igen_gpu.py:103-105 (1 <= r <= 16,
+ * n <= 1024, INS strictly ascending); Icap <= 1 gives an empty plan.
+ * cf_plan_create_d replaces src/mitm64_ddump.cpp:29-55: r-subsets D of the ascending base
+ * B[0..nB), nB <= 64, with prod(D) >= Dmin, key = prod(D) mod M, enumerated in the dumper's
+ * descending order (id_mode CF_ID_PACK_DESC or CF_ID_MASK).
+ * Creating a plan expands the enumeration tree down to the last level; records are written
+ * in the reference's depth-first order. */
+int cf_plan_create_i(cf_ctx *ctx, const uint64_t *INS, int n, int r, uint64_t M, uint64_t P0modM,
+                     uint64_t icap_lo, uint64_t icap_hi, int split_idx, int hmax, int id_mode, cf_plan **out);
+int cf_plan_create_d(cf_ctx *ctx, const uint64_t *B, int nB, int r, uint64_t M, uint64_t dmin_lo,
+                     uint64_t dmin_hi, int id_mode, cf_plan **out);
+int cf_plan_total(cf_plan *plan, uint64_t *total);
+/* Records below each first (depth-0) index, counts[0..n): the per-first-index counts of the
+ * count pass of gpu/igen_gpu.py:125-127. */
+int cf_plan_first_counts(cf_plan *plan, uint64_t *counts);
+/* Writes the records whose first index lies in [first_lo, first_hi), in order: one write pass
+ * of gpu/igen_gpu.py:135-139 (there the count/write equality check, here *written). */
+int cf_plan_write(cf_plan *plan, int first_lo, int first_hi, uint64_t *keys_dev, uint64_t *ids_dev,
+                  uint64_t *written);
+int cf_plan_destroy(cf_plan *plan);
+
+/* ---- one whole instance -----------------------------------------------------------------
+ * The device half of gpu/mitm_gpu_mstar_driver.py:209-243 (build_D with DFILTER, GPU I
+ * generation, sort, join against every D chunk) as one call: returns every equal-residue pair
+ * (D id, I id) of the instance, in device memory from the pool. */
+typedef struct cf_instance {
+    uint64_t M;
+    int32_t r;
+    int32_t nB;
+    const uint64_t *B;        /* base, ascending, nB <= 64 */
+    int32_t nI;
+    int32_t hmax;
+    const uint64_t *INS;      /* insertion pool, ascending, nI <= 1024 */
+    uint64_t P0modM;
+    uint64_t icap_lo, icap_hi;  /* exclusive bound on prod(I) */
+    uint64_t dmin_lo, dmin_hi;  /* inclusive bound on prod(D) */
+    int32_t split_idx;
+    int32_t i_id_mode;        /* CF_ID_COLEX or CF_ID_PACK_ASC (8 bits) */
+    int32_t d_id_mode;        /* CF_ID_PACK_DESC (6 bits) or CF_ID_MASK */
+    int32_t pad;
+} cf_instance;
+
+typedef struct cf_stats {
+    uint64_t nD, nI, npairs;
+    uint64_t peak_bytes;      /* largest pool footprint during the call */
+    double ms_total;          /* device time, first kernel to pairs ready */
+    double ms_gen_d, ms_sort_d, ms_gen_i, ms_sort_i, ms_join;
+} cf_stats;
+
+int cf_solve(cf_ctx *ctx, const cf_instance *inst, uint64_t max_chunk, uint64_t **out_d_dev,
+             uint64_t **out_i_dev, uint64_t *npairs, cf_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""GPU parity: every device entry point against the oracle (the CPU restatement), bit-exact.
+
+Run on an MI355X:  python -u -m pytest tests -m gpu -x -v --timeout 120 --timeout-method thread
+Each test prints the case it is about to run (instance, sizes) before touching the device.
+"""
+import itertools
+from math import comb, prod
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cfjoin import instance as I
+from cfjoin import lib as L
+from cfjoin import pipeline
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("CASE", *a, flush=True)
+
+
+def dev(a):
+    from cfjoin import xp
+    return xp.asarray(np.asarray(a, np.uint64))
+
+
+def family(workloads, fam, D, r):
+    f = workloads["families"][fam]
+    row = next(x for x in f["rows"] if int(x["D"]) == D)
+    return I.family_instance(D, int(row["cap"]), int(f["U"]), r)
+
+
+# ---- compatibility generate(): rank ranges ------------------------------------------------
+@pytest.mark.parametrize("M,r,bits,mult", [
+    (1768248177696000, 3, 6, 1),
+    (1768248177696000, 4, 6, 1),
+    (155016423578016000, 2, 10, 12345678901),
+    (17297280, 3, 8, 7),                   # small even modulus, heavy key collisions
+    ((1 << 64) - 59, 3, 6, 3),             # odd modulus near 2^64
+    (1 << 61, 2, 6, 5),                    # pure power of two
+    (1, 2, 6, 1),                          # M = 1: every key is 0
+])
+def test_generate_rank_ranges(M, r, bits, mult):
+    import mitm_gpu2 as G
+    primes = I.pool(1768248177696000, 700)[:64]
+    total = comb(64, r)
+    say("generate", M, r, bits, mult, total)
+    k, i = G.generate(primes, r, M, mult, bits=bits)
+    ek, ei = O.gen_rank(primes, r, M, mult, bits)
+    assert np.array_equal(k.get(), ek) and np.array_equal(i.get(), ei)
+    # rank chunks partition the rank space exactly (gpu/validate_mitm_gpu2.py:62-68)
+    b = [total * j // 5 for j in range(6)]
+    ks = [G.generate(primes, r, M, mult, bits=bits, lo=x, hi=y)[0].get() for x, y in zip(b, b[1:])]
+    assert np.array_equal(np.concatenate(ks), ek)
+
+
+def test_generate_preconditions():
+    import mitm_gpu2 as G
+    with pytest.raises(AssertionError):
+        G.generate([3, 5, 7], 11, 100)
+    with pytest.raises(AssertionError):
+        G.generate(list(range(3, 200, 2)), 3, 100, bits=6)
+    with pytest.raises(AssertionError):
+        G.generate([3, 5, 7], 2, 100, lo=2, hi=1)
+    k, i = G.generate([3, 5, 7], 2, 100, lo=1, hi=1)
+    assert k.size == 0 and i.size == 0
+
+
+# ---- filtered sides: enumeration plans vs the dumper restatements --------------------------
+def plan_records(plan):
+    from cfjoin import xp
+    n = plan.total()
+    k = xp.empty(n)
+    i = xp.empty(n)
+    if n:
+        assert plan.write(0, plan.n, k.ptr, i.ptr) == n
+    return k.get(), i.get()
+
+
+@pytest.mark.parametrize("fam,D,r", [
+    ("M0", 1768248177696000, 3),
+    ("M0", 1768248177696000, 4),
+    ("lambdaN", 10334428238534400, 6),
+    ("M0", 1014485472000, 5),
+    ("lambdaN", 155016423578016000, 5),
+])
+def test_plan_streams_match_dumpers(workloads, fam, D, r):
+    inst = family(workloads, fam, D, r)
+    say("plan", fam, D, r, len(inst.INS), inst.hmax)
+    ek, ei = O.idump(inst.M, r, inst.P0 % inst.M, inst.Icap, inst.INS, inst.split_idx, inst.hmax, rank=True)
+    p = L.Plan.insertion(inst.INS, r, inst.M, inst.P0 % inst.M, inst.Icap, inst.split_idx, inst.hmax, L.CF_ID_COLEX)
+    k, i = plan_records(p)
+    assert len(k) == len(ek)
+    assert np.array_equal(k, ek) and np.array_equal(i, ei), "I stream differs from the dumper order"
+    fc = p.first_counts()
+    exp = [len(O.idump(inst.M, r, inst.P0 % inst.M, inst.Icap, inst.INS, inst.split_idx, inst.hmax, True, x, x + 1)[0])
+           for x in range(len(inst.INS))]
+    assert list(fc) == exp
+    dk, di = O.ddump(inst.M, r, inst.Dmin, inst.B0)
+    pd = L.Plan.deletion(inst.B0, r, inst.M, inst.Dmin, L.CF_ID_PACK_DESC)
+    k, i = plan_records(pd)
+    assert np.array_equal(k, dk) and np.array_equal(i, di), "D stream differs from the dumper order"
+
+
+@pytest.mark.parametrize("r", [1, 2, 3, 5, 8])
+def test_plan_packed_ids_and_strata(r):
+    M = 1768248177696000
+    P = I.pool(M, 6000)
+    B0, INS = P[:64], P[64:64 + 200]
+    P0 = prod(B0)
+    split = 4000
+    split_idx = next((j for j, q in enumerate(INS) if q > split), len(INS))
+    INS = INS[:137]
+    muls = {1: (50, 10 ** 3, 10 ** 4), 2: (50, 10 ** 3, 10 ** 4), 3: (50, 10 ** 3, 10 ** 4), 5: (3, 10, 20), 8: (2, 5, 6)}[r]
+    for hmax, mul in zip((0, 1, r), muls):
+        Icap = prod(INS[:r]) * mul
+        say("packed", r, hmax, Icap.bit_length())
+        ek, ei = O.idump(M, r, P0 % M, Icap, INS, split_idx, hmax, rank=False)
+        p = L.Plan.insertion(INS, r, M, P0 % M, Icap, split_idx, hmax, L.CF_ID_PACK_ASC)
+        k, i = plan_records(p)
+        assert np.array_equal(k, ek) and np.array_equal(i, ei)
+
+
+def test_plan_deletion_bitmask_and_edges():
+    M = 155016423578016000
+    B0 = I.pool(M, 2000)[:64]
+    P0 = prod(B0)
+    for r, Dmin, mode, bm in [(11, prod(B0[-11:]) // 4, L.CF_ID_MASK, True),
+                              (3, 1, L.CF_ID_PACK_DESC, False),
+                              (4, prod(B0[-4:]) + 1, L.CF_ID_PACK_DESC, False)]:
+        say("ddump", r, Dmin.bit_length(), bm)
+        ek, ei = O.ddump(M, r, Dmin, B0, bitmask=bm)
+        k, i = plan_records(L.Plan.deletion(B0, r, M, Dmin, mode))
+        assert np.array_equal(k, ek) and np.array_equal(i, ei)
+    # Icap <= 1: empty plan
+    p = L.Plan.insertion(B0, 3, M, P0 % M, 1, 64, 3)
+    assert p.total() == 0
+
+
+# ---- sort and join ------------------------------------------------------------------------
+@pytest.mark.parametrize("n,M", [(1, 97), (1000, 97), (100000, 1 << 40), (300001, 1768248177696000), (70000, (1 << 64) - 59)])
+def test_sort_side_stable(n, M):
+    import mitm_gpu2 as G
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, M, n, dtype=np.uint64, endpoint=False) if M < (1 << 63) else rng.integers(0, 1 << 63, n, dtype=np.uint64) * 2
+    i = np.arange(n, dtype=np.uint64)
+    say("sort", n, M)
+    ks, is_ = G.sort_side(dev(k), dev(i))
+    o = np.argsort(k, kind="stable")
+    assert np.array_equal(ks.get(), k[o]) and np.array_equal(is_.get(), i[o])
+
+
+def test_join_sorted_matches_oracle():
+    import mitm_gpu2 as G
+    rng = np.random.default_rng(5)
+    for nD, nI, M in [(5000, 20000, 3000), (100, 100, 50), (200000, 300000, 1 << 30), (1, 1, 2)]:
+        say("join", nD, nI, M)
+        Dk = np.sort(rng.integers(0, M, nD, dtype=np.uint64))
+        Ik = np.sort(rng.integers(0, M, nI, dtype=np.uint64))
+        Did = rng.integers(0, 1 << 60, nD, dtype=np.uint64)
+        Iid = rng.integers(0, 1 << 60, nI, dtype=np.uint64)
+        pd, pi = G.join_sorted(dev(Dk), dev(Did), dev(Ik), dev(Iid))
+        ed, ei = O.join_sorted(Dk, Did, Ik, Iid)
+        assert np.array_equal(pd.get(), ed) and np.array_equal(pi.get(), ei)
+
+
+# ---- igen_gpu.generate_chunks: chunk boundaries and totals --------------------------------
+def test_generate_chunks_boundaries(workloads):
+    import igen_gpu as IG
+    inst = family(workloads, "M0", 1768248177696000, 3)
+    say("chunks", len(inst.INS))
+    ek, ei = O.idump(inst.M, 3, inst.P0 % inst.M, inst.Icap, inst.INS, inst.split_idx, inst.hmax, rank=True)
+    per = [len(O.idump(inst.M, 3, inst.P0 % inst.M, inst.Icap, inst.INS, inst.split_idx, inst.hmax, True, x, x + 1)[0])
+           for x in range(len(inst.INS))]
+    n = len(inst.INS)
+    for max_records, block_bytes in [(50_000, 512 << 20), (10 ** 9, 8 * n * n * 7), (1, 8 * n * n)]:
+        gen = IG.generate_chunks(inst.INS, 3, inst.M, inst.P0, inst.Icap, 4000, inst.hmax, max_records, block_bytes)
+        sizes, ks = [], []
+        while True:
+            try:
+                k, i = next(gen)
+            except StopIteration as st:
+                total = st.value
+                break
+            sizes.append(k.size)
+            ks.append(k.get())
+        # expected cut points, computed from the per-first-index counts
+        ib = max(1, min(n, block_bytes // (8 * n * n)))
+        exp = []
+        a = 0
+        while a < n:
+            b = min(n, a + ib)
+            a2 = a
+            while a2 < b:
+                b2, acc = a2 + 1, per[a2]
+                while b2 < b and acc + per[b2] <= max_records:
+                    acc += per[b2]
+                    b2 += 1
+                if acc:
+                    exp.append(acc)
+                a2 = b2
+            a = b
+        assert sizes == exp and total == len(ek)
+        assert np.array_equal(np.concatenate(ks), ek)
+    k, i, cnt = IG.generate(inst.INS, 3, inst.M, inst.P0, inst.Icap, 4000, inst.hmax)
+    assert cnt == len(ek) and np.array_equal(k.get(), ek) and np.array_equal(i.get(), ei)
+
+
+# ---- whole instances ----------------------------------------------------------------------
+@pytest.mark.parametrize("fam,D,r,max_chunk", [
+    ("lambdaN", 10334428238534400, 6, 0),
+    ("M0", 1014485472000, 5, 0),
+    ("M0", 1768248177696000, 4, 1_000_000),
+    ("M0", 1768248177696000, 5, 0),
+])
+def test_solve_matches_cpu_join(workloads, fam, D, r, max_chunk):
+    inst = family(workloads, fam, D, r)
+    say("solve", fam, D, r, len(inst.INS), max_chunk)
+    res = pipeline.solve(inst, max_chunk=max_chunk)
+    ref = O.cpu_join(inst.M, r, inst.P0 % inst.M, inst.Icap, inst.Dmin, inst.B0, inst.INS, inst.split_idx, inst.hmax,
+                     inst.rank_ids, threads=8)
+    assert (res.nD, res.nI) == (ref["nD"], ref["nI"])
+    assert np.array_equal(res.pair_array(), O.pair_set(ref["pairs_d"], ref["pairs_i"]))
+    for rec in pipeline.recheck(inst, res):
+        assert rec["D"] and rec["I"]
+
+
+@pytest.mark.parametrize("seed,s", [(0, 10), (0, 100), (1, 10 ** 4), (2, None), (3, 1)])
+def test_solve_c1(seed, s):
+    inst = I.c1_instance(seed, s)
+    say("c1", seed, s, inst.M)
+    res = pipeline.solve(inst)
+    Dk, Did, Ik, Iid = O.instance_records(inst)
+    o = np.argsort(Dk, kind="stable")
+    q = np.argsort(Ik, kind="stable")
+    ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
+    assert (res.nD, res.nI) == (len(Dk), len(Ik))
+    assert np.array_equal(res.pair_array(), O.pair_set(ed, ei))

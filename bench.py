@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Residue keys joined per second on one MI355X: batched vs plain, bit-exact match sets.
+
+Workload (default): the lambda-family batch -- every ksmall instance of the 118 divisor moduli of
+lambda(N) = 155 016 423 578 016 000 (bound U = N) at radii 4..7 with r <= rmax(D): 280 instances,
+1.60e9 keys (nD + nI), 79 pairs. One step = one pass of the join over the whole batch:
+  batched -- all 280 instances in one solve (cf_solve_batch), the timed value;
+  plain   -- one solve per instance (cf_solve), timed beside it.
+Every instance's (nD, nI, pairs) is checked against the reference's logged counts, and the pair sets of
+the CPU-baseline sample are checked against the CPU join. Inputs (instance descriptors) are host
+values of a few KB; everything the join touches is generated on the device.
+
+JSON line fields beyond the required ones: plain (value, ms_per_step), batched_vs_plain, roofline
+(dominant kernel, live HIP-event timing on the kernel's stream), cpu_baseline, verify.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "carmichael-frontier_amd"))
+
+PEAK_HBM_GBS = 8000.0
+PROBE_BYTES_PER_RECORD = 16     # the probe reads each partitioned I record (key, id) once
+
+
+def run_batched(pipeline, insts):
+    res, st = pipeline.solve_batch(insts)
+    return res, st
+
+
+def run_plain(pipeline, insts):
+    return [pipeline.solve(x) for x in insts]
+
+
+def verify_counts(insts, res):
+    bad = []
+    for inst, r in zip(insts, res):
+        lg = inst.meta.get("logged")
+        if lg is None or (lg["nD"], lg["nI"], lg["pairs"]) != (r.nD, r.nI, len(r.pairs_d)):
+            bad.append((inst.tag, lg, (r.nD, r.nI, len(r.pairs_d))))
+    return bad
+
+
+def cpu_baseline(insts, res_batched, threads, budget_keys):
+    """The CPU restatement of the reference join (hash the D residues, T threads striped over the first
+    insertion index; src/mitm64.cpp) over a bounded sample of the same instances, pair sets compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    order = sorted(range(len(insts)), key=lambda j: insts[j].meta["logged"]["nD"] + insts[j].meta["logged"]["nI"])
+    sample, keys = [], 0
+    for j in order[::2]:
+        k = insts[j].meta["logged"]["nD"] + insts[j].meta["logged"]["nI"]
+        if keys + k > budget_keys:
+            continue
+        sample.append(j)
+        keys += k
+    t_sum = 0.0
+    mismatches = 0
+    for j in sample:
+        x = insts[j]
+        r = oracle.cpu_join(x.M, x.r, x.P0 % x.M, x.Icap, x.Dmin, x.B0, x.INS, x.split_idx, x.hmax, x.rank_ids,
+                            threads=threads)
+        t_sum += r["td"] + r["ti"]
+        if not np.array_equal(oracle.pair_set(r["pairs_d"], r["pairs_i"]), res_batched[j].pair_array()):
+            mismatches += 1
+    return dict(value=keys / t_sum if t_sum else 0.0, unit="residue keys joined/s", cores=threads, kind="port",
+                sample=f"{len(sample)} of the {len(insts)} batch instances (every other one in size order, "
+                       f"{keys:.3g} keys), D build + I scan time (oracle/oracle.c or_cpu_join)",
+                seconds=round(t_sum, 2), pair_set_mismatches=mismatches)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--family", default="lambdaN", choices=["lambdaN", "M0"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain leg (profiling the batched path alone)")
+    ap.add_argument("--cpu-keys", type=float, default=8.0e8, help="CPU-baseline sample size in keys")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from cfjoin import lib, pipeline
+    from cfjoin import workloads as W
+
+    logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))
+    insts = W.batch_instances(args.family, radii=(4, 5, 6, 7), logged=logged)
+    ctx = lib.default_context()
+    print(f"[bench] {len(insts)} instances of the {args.family} family, radii 4..7", file=sys.stderr, flush=True)
+
+    # warm-up doubles as the bit-exactness check of both paths
+    for _ in range(max(args.warmup, 1)):
+        res_b, _ = run_batched(pipeline, insts)
+    res_p = res_b if args.no_plain else run_plain(pipeline, insts)
+    bad_b = verify_counts(insts, res_b)
+    bad_p = verify_counts(insts, res_p)
+    same = all((a.nD, a.nI) == (b.nD, b.nI) and (a.pair_array() == b.pair_array()).all() for a, b in zip(res_b, res_p))
+    keys = sum(r.nD + r.nI for r in res_b)
+    nI_total = sum(r.nI for r in res_b)
+    print(f"[bench] warm-up done: {keys:.4g} keys, logged-count mismatches batched={len(bad_b)} plain={len(bad_p)}, "
+          f"batched == plain pair sets: {same}", file=sys.stderr, flush=True)
+
+    # ---- timed: batched, with the dominant kernel bracketed by HIP events on its stream
+    dominant = "k_part_probe"
+    ctx.prof_enable(dominant)
+    ctx.prof_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_batched(pipeline, insts)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_launches = ctx.prof_read(dominant)
+    ctx.prof_disable()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = keys / ((t1 - t0) / args.steps)
+
+    # ---- plain, one step
+    plain_ms = plain_value = None
+    if not args.no_plain:
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        run_plain(pipeline, insts)
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        plain_ms = (t3 - t2) * 1e3
+        plain_value = keys / (t3 - t2)
+
+    bytes_per_step = PROBE_BYTES_PER_RECORD * nI_total
+    achieved = bytes_per_step * args.steps / (k_ms / 1e3) / 1e9 if k_ms else 0.0
+    roofline = dict(kernel=dominant, bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
+                    algorithmic=f"{PROBE_BYTES_PER_RECORD} B per probed I record x {nI_total} records per step",
+                    avg_launch_ms=round(k_ms / max(k_launches, 1), 4), launches_per_step=k_launches // args.steps,
+                    kernel_ms_per_step=round(k_ms / args.steps, 3))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(insts, res_b, threads, args.cpu_keys)
+
+    line = {
+        "metric": "residue keys joined/s (batched vs plain, 1 GPU), bit-exact match sets",
+        "value": value,
+        "unit": "residue keys joined/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: instances rebuilt from the reference's census rows; no record data is loaded",
+        "config": {"workload": f"{args.family}-family batch: {len(insts)} instances (moduli x radii 4..7 <= rmax), "
+                               f"{keys} keys (nD+nI)", "instances": len(insts), "keys_per_step": keys,
+                   "parallelism": "single GPU"},
+        "plain": {"value": plain_value, "ms_per_step": plain_ms},
+        "batched_vs_plain": value / plain_value if plain_value else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "verify": {"logged_count_mismatches_batched": len(bad_b),
+                   "logged_count_mismatches_plain": None if args.no_plain else len(bad_p),
+                   "batched_equals_plain": None if args.no_plain else bool(same),
+                   "pairs": sum(len(r.pairs_d) for r in res_b)},
+    }
+    if cpu and cpu["value"]:
+        line["vs_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

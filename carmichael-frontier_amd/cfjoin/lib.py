@@ -24,6 +24,7 @@ c_void_p = ctypes.c_void_p
 P_u64 = ctypes.POINTER(ctypes.c_uint64)
 
 CF_ID_COLEX, CF_ID_PACK_ASC, CF_ID_PACK_DESC, CF_ID_MASK = 0, 1, 2, 3
+CF_SOLVE_PARTITION, CF_SOLVE_SORTMERGE = 0, 1
 
 _lib = None
 _lock = threading.RLock()
@@ -61,6 +62,11 @@ SIGNATURES = {
     "cf_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_void_p, c_u64]),
     "cf_memcpy_d2d": (c_int, [c_void_p, c_void_p, c_void_p, c_u64]),
     "cf_synchronize": (c_int, [c_void_p]),
+    "cf_prof_enable": (c_int, [c_void_p, ctypes.c_char_p]),
+    "cf_prof_disable": (c_int, [c_void_p]),
+    "cf_prof_reset": (c_int, [c_void_p]),
+    "cf_prof_read": (c_int, [c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), P_u64]),
+    "cf_prof_dump": (c_int, [c_void_p, ctypes.c_char_p, c_u64]),
     "cf_gen_subsets": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_u64, c_u64, c_u64, c_u64, c_void_p, c_void_p]),
     "cf_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_u64]),
     "cf_join_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
@@ -73,8 +79,11 @@ SIGNATURES = {
     "cf_plan_first_counts": (c_int, [c_void_p, c_void_p]),
     "cf_plan_write": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, P_u64]),
     "cf_plan_destroy": (c_int, [c_void_p]),
-    "cf_solve": (c_int, [c_void_p, ctypes.POINTER(cf_instance), c_u64, ctypes.POINTER(c_void_p),
+    "cf_solve": (c_int, [c_void_p, ctypes.POINTER(cf_instance), c_int, c_u64, ctypes.POINTER(c_void_p),
                          ctypes.POINTER(c_void_p), P_u64, ctypes.POINTER(cf_stats)]),
+    "cf_solve_batch": (c_int, [c_void_p, ctypes.POINTER(cf_instance), c_int, ctypes.POINTER(c_void_p),
+                               ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), P_u64, P_u64,
+                               ctypes.POINTER(cf_stats)]),
 }
 
 
@@ -152,6 +161,29 @@ class Context:
 
     def synchronize(self) -> None:
         check(load().cf_synchronize(self.handle), "cf_synchronize")
+
+    def prof_enable(self, kernel: Optional[str] = None) -> None:
+        check(load().cf_prof_enable(self.handle, kernel.encode() if kernel else None), "cf_prof_enable")
+
+    def prof_disable(self) -> None:
+        check(load().cf_prof_disable(self.handle), "cf_prof_disable")
+
+    def prof_reset(self) -> None:
+        check(load().cf_prof_reset(self.handle), "cf_prof_reset")
+
+    def prof_dump(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 16)
+        check(load().cf_prof_dump(self.handle, buf, len(buf)), "cf_prof_dump")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, n = line.split()
+            out[name] = (float(ms), int(n))
+        return out
+
+    def prof_read(self, kernel: str) -> Tuple[float, int]:
+        ms, n = ctypes.c_double(), c_u64()
+        check(load().cf_prof_read(self.handle, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "cf_prof_read")
+        return ms.value, n.value
 
 
 _default: Optional[Context] = None

@@ -1,9 +1,12 @@
 """Whole-instance solves on the device.
 
-solve(inst)  -- one instance per call ("plain"): D side generated with the Dmin filter and
-               sorted, I side expanded and streamed through sort + join in chunks, every
-               equal-residue pair returned. The device half of
-               gpu/mitm_gpu_mstar_driver.py:209-243 (DFILTER=1, IGEN=gpu) in one C call.
+solve(inst)        -- one instance per call ("plain"): every equal-residue pair of the instance.
+                      The device half of gpu/mitm_gpu_mstar_driver.py:209-243 (DFILTER=1,
+                      IGEN=gpu) in one C call; mode "partition" (default) partitions I leaves by
+                      key range and probes the D buckets of their partition, mode "sortmerge"
+                      materializes, sorts and joins I chunks as the reference does.
+solve_batch(insts) -- many instances of any radii in one solve (one key space, one D bucketing,
+                      one set of launches per radius).
 decode(inst, pairs) / recheck(inst, pairs) -- host decode and exact re-check of the hits
                (gpu/mitm_gpu_mstar_driver.py:224-237).
 """
@@ -58,15 +61,18 @@ def make_cinstance(inst, keep: list) -> L.cf_instance:
                          i_id_mode=i_mode, d_id_mode=d_mode, pad=0)
 
 
-def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0) -> SolveResult:
+MODES = {"partition": L.CF_SOLVE_PARTITION, "sortmerge": L.CF_SOLVE_SORTMERGE}
+
+
+def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0, mode: str = "partition") -> SolveResult:
     ctx = ctx or L.default_context()
     keep: List = []
     ci = make_cinstance(inst, keep)
     od, oi = ctypes.c_void_p(), ctypes.c_void_p()
     n = L.c_u64()
     st = L.cf_stats()
-    L.check(L.load().cf_solve(ctx.handle, ctypes.byref(ci), max_chunk, ctypes.byref(od), ctypes.byref(oi),
-                              ctypes.byref(n), ctypes.byref(st)), "cf_solve")
+    L.check(L.load().cf_solve(ctx.handle, ctypes.byref(ci), MODES[mode], max_chunk, ctypes.byref(od),
+                              ctypes.byref(oi), ctypes.byref(n), ctypes.byref(st)), "cf_solve")
     pd = np.empty(n.value, np.uint64)
     pi = np.empty(n.value, np.uint64)
     if n.value:
@@ -75,6 +81,37 @@ def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0) -> SolveRes
         ctx.free(od.value)
         ctx.free(oi.value)
     return SolveResult(pd, pi, int(st.nD), int(st.nI), st.as_dict())
+
+
+def solve_batch(insts, ctx: Optional[L.Context] = None):
+    """Solve many instances together. Returns (list of SolveResult in input order, batch stats)."""
+    ctx = ctx or L.default_context()
+    keep: List = []
+    n = len(insts)
+    arr = (L.cf_instance * n)(*[make_cinstance(x, keep) for x in insts])
+    od, oi, oinst = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    npairs = L.c_u64()
+    counts = np.zeros(2 * n, np.uint64)
+    st = L.cf_stats()
+    L.check(L.load().cf_solve_batch(ctx.handle, arr, n, ctypes.byref(od), ctypes.byref(oi), ctypes.byref(oinst),
+                                    ctypes.byref(npairs), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                    ctypes.byref(st)), "cf_solve_batch")
+    m = npairs.value
+    pd = np.empty(m, np.uint64)
+    pi = np.empty(m, np.uint64)
+    pj = np.empty(m, np.uint64)
+    if m:
+        for host, dev in ((pd, od), (pi, oi), (pj, oinst)):
+            ctx.d2h(host, dev.value)
+            ctx.free(dev.value)
+    order = np.argsort(pj, kind="stable")
+    pd, pi, pj = pd[order], pi[order], pj[order]
+    bounds = np.searchsorted(pj, np.arange(n + 1, dtype=np.uint64))
+    out = []
+    for j in range(n):
+        a, b = int(bounds[j]), int(bounds[j + 1])
+        out.append(SolveResult(pd[a:b].copy(), pi[a:b].copy(), int(counts[2 * j]), int(counts[2 * j + 1])))
+    return out, st.as_dict()
 
 
 def decoders(inst):

@@ -8,25 +8,10 @@
 #include "../../include/cfjoin.h"
 #include "cf_internal.h"
 
-static thread_local std::string g_err;
+thread_local std::string g_err;
 void cf_set_error(const std::string &msg) { g_err = msg; }
 
-#define API_BEGIN try {
-#define API_END                                            \
-    }                                                      \
-    catch (const CfError &e) {                             \
-        g_err = e.msg;                                     \
-        return e.code;                                     \
-    }                                                      \
-    catch (const std::bad_alloc &) {                       \
-        g_err = "host out of memory";                      \
-        return CF_ENOMEM;                                  \
-    }                                                      \
-    catch (const std::exception &e) {                      \
-        g_err = e.what();                                  \
-        return CF_EINTERNAL;                               \
-    }                                                      \
-    return CF_OK;
+
 
 typedef unsigned __int128 u128;
 
@@ -42,7 +27,7 @@ static size_t pool_round(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 
-static size_t g_peak_live = 0;
+size_t g_peak_live = 0;
 
 void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     const size_t sz = pool_round(bytes);
@@ -103,25 +88,52 @@ u64 d2h_u64(cf_ctx *ctx, const u64 *src) {
     return v;
 }
 
-static int bitlen(u64 v) { return v ? 64 - __builtin_clzll(v) : 0; }
+int bitlen(u64 v) { return v ? 64 - __builtin_clzll(v) : 0; }
+
+// ---- kernel timing ---------------------------------------------------------------------------
+static bool prof_selected(cf_ctx *ctx, const char *name) {
+    return ctx->prof_on && (ctx->prof_filter.empty() || ctx->prof_filter == name);
+}
+static hipEvent_t prof_event(cf_ctx *ctx) {
+    if (!ctx->prof_free.empty()) {
+        hipEvent_t e = ctx->prof_free.back();
+        ctx->prof_free.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    CF_HIP(hipEventCreate(&e));
+    return e;
+}
+void cf_prof_begin(cf_ctx *ctx, const char *name, hipEvent_t *ev) {
+    if (!prof_selected(ctx, name)) return;
+    *ev = prof_event(ctx);
+    CF_HIP(hipEventRecord(*ev, ctx->stream));
+}
+void cf_prof_end(cf_ctx *ctx, const char *name, hipEvent_t ev) {
+    if (!prof_selected(ctx, name) || !ev) return;
+    hipEvent_t b = prof_event(ctx);
+    CF_HIP(hipEventRecord(b, ctx->stream));
+    ctx->prof_pending.push_back({name, ev, b});
+}
+static void prof_drain(cf_ctx *ctx) {
+    if (ctx->prof_pending.empty()) return;
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &r : ctx->prof_pending) {
+        float ms = 0;
+        CF_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        auto &t = ctx->prof_totals[r.name];
+        t.first += ms;
+        t.second += 1;
+        ctx->prof_free.push_back(r.a);
+        ctx->prof_free.push_back(r.b);
+    }
+    ctx->prof_pending.clear();
+}
 
 // ---- plans ----------------------------------------------------------------------------------
-struct cf_plan {
-    cf_ctx *ctx = nullptr;
-    SideSpec spec;
-    SideTables tab;
-    DBuf<SideDesc> d_desc;
-    DBuf<Node> leafp;      // depth r-1 nodes
-    size_t nleafp = 0;
-    DBuf<u32> cnt;         // leaves per depth r-1 node
-    DBuf<u64> off;         // exclusive scan of cnt, nleafp + 1 entries
-    u64 total = 0;
-    bool empty = false;
-    std::vector<u64> first_counts;
-    bool have_first = false;
-};
 
-static void plan_expand(cf_plan *pl) {
+
+void plan_expand(cf_plan *pl) {
     cf_ctx *ctx = pl->ctx;
     build_side_tables(ctx, pl->spec, pl->tab);
     pl->d_desc = DBuf<SideDesc>(ctx, 1);
@@ -183,7 +195,7 @@ static void plan_first_counts(cf_plan *pl) {
     pl->have_first = true;
 }
 
-static void plan_write_range(cf_plan *pl, u64 L0, u64 L1, u64 *keys, u64 *ids) {
+void plan_write_range(cf_plan *pl, u64 L0, u64 L1, u64 *keys, u64 *ids) {
     if (L1 <= L0) return;
     launch_leaf_write(pl->ctx, pl->d_desc.p, pl->leafp.p, pl->nleafp, pl->off.p, L0, L1 - L0, pl->tab.desc.r - 1, keys,
                       ids);
@@ -217,6 +229,8 @@ int cf_ctx_destroy(cf_ctx *ctx) {
     for (auto &kv : ctx->free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->live) (void)hipFree(kv.first);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto &r : ctx->prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : ctx->prof_free) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     API_END
@@ -291,6 +305,53 @@ int cf_synchronize(cf_ctx *ctx) {
     API_BEGIN
     CF_REQUIRE(ctx, "NULL context");
     CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int cf_prof_enable(cf_ctx *ctx, const char *kernel) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "NULL context");
+    ctx->prof_on = true;
+    ctx->prof_filter = kernel ? kernel : "";
+    API_END
+}
+
+int cf_prof_disable(cf_ctx *ctx) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "NULL context");
+    prof_drain(ctx);
+    ctx->prof_on = false;
+    API_END
+}
+
+int cf_prof_reset(cf_ctx *ctx) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "NULL context");
+    prof_drain(ctx);
+    ctx->prof_totals.clear();
+    API_END
+}
+
+int cf_prof_read(cf_ctx *ctx, const char *kernel, double *ms, uint64_t *launches) {
+    API_BEGIN
+    CF_REQUIRE(ctx && kernel, "NULL argument");
+    prof_drain(ctx);
+    auto it = ctx->prof_totals.find(kernel);
+    if (ms) *ms = it == ctx->prof_totals.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == ctx->prof_totals.end() ? 0 : it->second.second;
+    API_END
+}
+
+int cf_prof_dump(cf_ctx *ctx, char *buf, uint64_t size) {
+    API_BEGIN
+    CF_REQUIRE(ctx && buf && size, "NULL argument");
+    prof_drain(ctx);
+    std::string text;
+    for (auto &kv : ctx->prof_totals)
+        text += kv.first + " " + std::to_string(kv.second.first) + " " + std::to_string(kv.second.second) + "\n";
+    const size_t m = std::min<size_t>(text.size(), size - 1);
+    std::memcpy(buf, text.data(), m);
+    buf[m] = 0;
     API_END
 }
 
@@ -474,105 +535,6 @@ int cf_plan_write(cf_plan *plan, int first_lo, int first_hi, uint64_t *keys, uin
 int cf_plan_destroy(cf_plan *plan) {
     API_BEGIN
     delete plan;
-    API_END
-}
-
-int cf_solve(cf_ctx *ctx, const cf_instance *inst, uint64_t max_chunk, uint64_t **out_d, uint64_t **out_i,
-             uint64_t *npairs, cf_stats *stats) {
-    API_BEGIN
-    CF_REQUIRE(ctx && inst && out_d && out_i && npairs, "NULL argument");
-    *out_d = nullptr;
-    *out_i = nullptr;
-    *npairs = 0;
-    g_peak_live = 0;
-    hipEvent_t ev[6];
-    for (auto &e : ev) CF_HIP(hipEventCreate(&e));
-    CF_HIP(hipEventRecord(ev[0], ctx->stream));
-    // D side: generate, sort
-    cf_plan *pd = nullptr;
-    int rc = cf_plan_create_d(ctx, inst->B, inst->nB, inst->r, inst->M, inst->dmin_lo, inst->dmin_hi, inst->d_id_mode, &pd);
-    if (rc) throw CfError{rc, g_err};
-    std::unique_ptr<cf_plan> hold_d(pd);
-    const u64 nD = pd->empty ? 0 : pd->total;
-    DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
-    plan_write_range(pd, 0, nD, Dk.p, Did.p);
-    hold_d.reset();
-    CF_HIP(hipEventRecord(ev[1], ctx->stream));
-    const int kbits = bitlen(inst->M - 1 ? inst->M - 1 : 1);
-    radix_sort_pairs(ctx, Dk.p, Did.p, nD, kbits);
-    CF_HIP(hipEventRecord(ev[2], ctx->stream));
-    // I side: plan, then stream leaf ranges of at most max_chunk records through sort + join
-    cf_plan *pi = nullptr;
-    rc = cf_plan_create_i(ctx, inst->INS, inst->nI, inst->r, inst->M, inst->P0modM, inst->icap_lo, inst->icap_hi,
-                          inst->split_idx, inst->hmax, inst->i_id_mode, &pi);
-    if (rc) throw CfError{rc, g_err};
-    std::unique_ptr<cf_plan> hold_i(pi);
-    const u64 nI = pi->empty ? 0 : pi->total;
-    float ms_gen_i = 0, ms_sort_i = 0, ms_join = 0;
-    CF_HIP(hipEventRecord(ev[3], ctx->stream));
-    std::vector<std::pair<DBuf<u64>, DBuf<u64>>> parts;
-    u64 total_pairs = 0;
-    if (nI && nD) {
-        const u64 chunk = max_chunk ? max_chunk : (u64)1 << 28;
-        const u64 cs = std::min<u64>(chunk, nI);
-        DBuf<u64> Ik(ctx, cs), Iid(ctx, cs), lo(ctx, cs), cnt(ctx, cs), off(ctx, cs + 1);
-        for (u64 L = 0; L < nI; L += chunk) {
-            const u64 m = std::min<u64>(chunk, nI - L);
-            hipEvent_t a, b, c, d;
-            CF_HIP(hipEventCreate(&a)); CF_HIP(hipEventCreate(&b)); CF_HIP(hipEventCreate(&c)); CF_HIP(hipEventCreate(&d));
-            CF_HIP(hipEventRecord(a, ctx->stream));
-            plan_write_range(pi, L, L + m, Ik.p, Iid.p);
-            CF_HIP(hipEventRecord(b, ctx->stream));
-            radix_sort_pairs(ctx, Ik.p, Iid.p, m, kbits);
-            CF_HIP(hipEventRecord(c, ctx->stream));
-            const u64 tot = join_sorted_count(ctx, Dk.p, nD, Ik.p, m, lo.p, cnt.p, off.p);
-            if (tot) {
-                DBuf<u64> od(ctx, tot), oi(ctx, tot);
-                join_sorted_write(ctx, Did.p, Iid.p, m, lo.p, cnt.p, off.p, od.p, oi.p, nullptr, Ik.p);
-                parts.emplace_back(std::move(od), std::move(oi));
-                total_pairs += tot;
-            }
-            CF_HIP(hipEventRecord(d, ctx->stream));
-            CF_HIP(hipEventSynchronize(d));
-            float t1, t2, t3;
-            CF_HIP(hipEventElapsedTime(&t1, a, b)); CF_HIP(hipEventElapsedTime(&t2, b, c)); CF_HIP(hipEventElapsedTime(&t3, c, d));
-            ms_gen_i += t1; ms_sort_i += t2; ms_join += t3;
-            (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); (void)hipEventDestroy(d);
-        }
-    }
-    if (total_pairs) {
-        DBuf<u64> od(ctx, total_pairs), oi(ctx, total_pairs);
-        u64 at = 0;
-        for (auto &pr : parts) {
-            CF_HIP(hipMemcpyAsync(od.p + at, pr.first.p, pr.first.n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
-            CF_HIP(hipMemcpyAsync(oi.p + at, pr.second.p, pr.second.n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
-            at += pr.first.n;
-        }
-        parts.clear();
-        *out_d = od.release();
-        *out_i = oi.release();
-    }
-    CF_HIP(hipEventRecord(ev[4], ctx->stream));
-    CF_HIP(hipEventSynchronize(ev[4]));
-    *npairs = total_pairs;
-    if (stats) {
-        float a, b, c, t;
-        CF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-        CF_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
-        CF_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
-        CF_HIP(hipEventElapsedTime(&t, ev[0], ev[4]));
-        stats->nD = nD;
-        stats->nI = nI;
-        stats->npairs = total_pairs;
-        stats->peak_bytes = g_peak_live;
-        stats->ms_total = t;
-        stats->ms_gen_d = a;
-        stats->ms_sort_d = b;
-        stats->ms_gen_i = c + ms_gen_i;   // I plan expansion + leaf writes
-        stats->ms_sort_i = ms_sort_i;
-        stats->ms_join = ms_join;
-    }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     API_END
 }
 

@@ -28,6 +28,25 @@ struct CfError {
         if (!(cond)) throw CfError{-1, (msg)};       \
     } while (0)
 
+extern thread_local std::string g_err;
+#include "../../include/cfjoin.h"
+#define API_BEGIN try {
+#define API_END                                            \
+    }                                                      \
+    catch (const CfError &e) {                             \
+        g_err = e.msg;                                     \
+        return e.code;                                     \
+    }                                                      \
+    catch (const std::bad_alloc &) {                       \
+        g_err = "host out of memory";                      \
+        return CF_ENOMEM;                                  \
+    }                                                      \
+    catch (const std::exception &e) {                      \
+        g_err = e.what();                                  \
+        return CF_EINTERNAL;                               \
+    }                                                      \
+    return CF_OK;
+
 // ---- context: one device, one stream, a caching allocator ---------------------------------
 struct cf_ctx {
     int device = 0;
@@ -38,7 +57,27 @@ struct cf_ctx {
     std::mutex mu;
     void *pinned = nullptr;                       // small pinned staging buffer for scalars
     size_t pinned_bytes = 0;
+    // kernel timing with HIP events on this context's stream (enabled by cf_prof_enable)
+    bool prof_on = false;
+    std::string prof_filter;                      // empty: every kernel
+    struct ProfRec { const char *name; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_free;
+    std::map<std::string, std::pair<double, uint64_t>> prof_totals;   // name -> (ms, launches)
 };
+
+void cf_prof_begin(cf_ctx *ctx, const char *name, hipEvent_t *ev);
+void cf_prof_end(cf_ctx *ctx, const char *name, hipEvent_t ev);
+
+// Launch on the context's stream; bracketed by HIP events when profiling selects the kernel.
+#define CF_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                           \
+    do {                                                                             \
+        hipEvent_t ev_a_ = nullptr;                                                  \
+        cf_prof_begin((ctx), (name), &ev_a_);                                        \
+        hipLaunchKernelGGL(kern, grid, block, shmem, (ctx)->stream, __VA_ARGS__);    \
+        CF_HIP(hipGetLastError());                                                   \
+        cf_prof_end((ctx), (name), ev_a_);                                           \
+    } while (0)
 
 void *ctx_alloc(cf_ctx *ctx, size_t bytes);
 void ctx_free(cf_ctx *ctx, void *p);
@@ -113,6 +152,9 @@ struct SideDesc {
     u64 koff;           // offset added to every key (disjoint key ranges across a batch)
     int n, r, side, id_mode, id_bits, split_idx, hmax, pad;
     u64 xm0, x20;       // initial residue state (the side multiplier)
+    // this instance's fine key buckets: bucket(key) = bbase + mulhi(key * golden, nb)
+    u64 bS;             // unused
+    u32 bbase, nb;
 };
 
 // Host-owned copy of one side's device tables.
@@ -136,10 +178,31 @@ struct SideSpec {  // host description of one side
 
 void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t);
 
+// A filtered enumeration plan: the tree of one side expanded to its last level.
+struct cf_plan {
+    cf_ctx *ctx = nullptr;
+    SideSpec spec;
+    SideTables tab;
+    DBuf<SideDesc> d_desc;
+    DBuf<Node> leafp;      // depth r-1 nodes
+    size_t nleafp = 0;
+    DBuf<u32> cnt;         // leaves per depth r-1 node
+    DBuf<u64> off;         // exclusive scan of cnt, nleafp + 1 entries
+    u64 total = 0;
+    bool empty = false;
+    std::vector<u64> first_counts;
+    bool have_first = false;
+};
+void plan_expand(cf_plan *pl);
+void plan_write_range(cf_plan *pl, u64 L0, u64 L1, u64 *keys, u64 *ids);
+int bitlen(u64 v);
+extern size_t g_peak_live;
+
 // ---- kernels (host launchers) -------------------------------------------------------------
 // exclusive scan of u32 or u64 counts into u64 offsets; returns the total (synchronous)
 u64 scan_u32_to_u64(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);
 u64 scan_u64(cf_ctx *ctx, const u64 *in, u64 *out, size_t n);
+void scan_u32_to_u64_async(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);   // no host sync
 
 // stable LSD radix sort of (key, val) by the low `bits` bits of key; in-place via scratch
 void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits);
@@ -154,6 +217,22 @@ void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, s
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
 void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out);
 u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
+
+// radix-partitioned join (k_part.hip)
+void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
+void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                       int depth, u32 *bcount);
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                         u64 n, int depth, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
+void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                int depth, int pshift, int P, u32 *pcount);
+void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                  int depth, int pshift, int P, const u64 *pstart, u32 *pcur, u64 *Ik, u64 *Iid);
+void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const SideDesc *descs, const u64 *koffs, int ninst, u64 nI,
+                const u64 *bstart,
+                const u64 *Dk, const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
+void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
 
 // joins
 u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off);

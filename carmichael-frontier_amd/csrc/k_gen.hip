@@ -12,14 +12,12 @@
 //      D side (descending, prod >= Dmin reach prune):                 src/mitm64_ddump.cpp:29-44
 //  * rank ranges (unfiltered, compatibility generate()): one lane per combinatorial-number-system
 //    rank, lexicographic unranking as gpu/mitm_gpu2.py:40-52.
-#include "cf_internal.h"
+#include "tree_dev.h"
 
 namespace {
 
 constexpr int BLK = 256;
 
-__device__ __forceinline__ int node_nxt(const Node &nd) { return (int)(short)(nd.pos & 0xFFFFu); }
-__device__ __forceinline__ int node_first(const Node &nd) { return (int)(nd.pos >> 16); }
 
 // Number of primes with index in [x+1, x+t] at or above split_idx: the fewest "large" primes any
 // completion after choosing x can contain (indices only grow).
@@ -71,15 +69,6 @@ __device__ int child_count(const SideDesc &s, const Node &nd, int depth) {
     }
 }
 
-__device__ __forceinline__ u64 id_update(const SideDesc &s, u64 id, int x, int depth) {
-    switch (s.id_mode) {
-        case ID_COLEX: return id + s.binom[(size_t)x * (s.r + 1) + depth + 1];
-        case ID_PACK_ASC: return id | ((u64)x << (s.id_bits * depth));
-        case ID_PACK_DESC: return id | ((u64)x << (s.id_bits * (s.r - 1 - depth)));
-        default: return id | (1ULL << x);
-    }
-}
-
 // Child number k of a depth-`depth` node.
 __device__ __forceinline__ Node make_child(const SideDesc &s, const Node &nd, int depth, int k) {
     const int x = (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
@@ -105,16 +94,6 @@ __global__ void __launch_bounds__(BLK) k_expand_count(const SideDesc *__restrict
         const SideDesc &s = descs[nd.meta >> 8];
         cnt[j] = (u32)child_count(s, nd, depth);
     }
-}
-
-// Parent of output j: the p with off[p] <= j < off[p+1] (off = exclusive scan, off[nn] = total),
-// searched within [plo, phi].
-__device__ __forceinline__ size_t find_parent(const u64 *__restrict__ off, size_t plo, size_t phi, u64 j) {
-    while (plo < phi) {
-        size_t mid = (plo + phi + 1) >> 1;
-        if (off[mid] <= j) plo = mid; else phi = mid - 1;
-    }
-    return plo;
 }
 
 // Block-level narrowing: the block covers outputs [j0, j1); its parents lie in [p0, p1].
@@ -222,37 +201,32 @@ inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 1u << 20) {
 
 void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, int depth, u32 *cnt) {
     if (!nn) return;
-    hipLaunchKernelGGL(k_expand_count, dim3(grid_for(nn, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, depth, cnt);
-    CF_HIP(hipGetLastError());
+    CF_LAUNCH(ctx, "k_expand_count", k_expand_count, dim3(grid_for(nn, BLK)), dim3(BLK), 0, d_desc, nodes, nn, depth, cnt);
 }
 
 void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                          size_t nout, int depth, Node *out) {
     if (!nout) return;
-    hipLaunchKernelGGL(k_expand_write, dim3(grid_for(nout, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, off, nout,
+    CF_LAUNCH(ctx, "k_expand_write", k_expand_write, dim3(grid_for(nout, BLK)), dim3(BLK), 0, d_desc, nodes, nn, off, nout,
                        depth, out);
-    CF_HIP(hipGetLastError());
 }
 
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off, u64 leaf_lo,
                        size_t nleaf, int depth, u64 *keys, u64 *ids) {
     if (!nleaf) return;
-    hipLaunchKernelGGL(k_leaf_write, dim3(grid_for(nleaf, BLK)), dim3(BLK), 0, ctx->stream, d_desc, nodes, nn, off,
+    CF_LAUNCH(ctx, "k_leaf_write", k_leaf_write, dim3(grid_for(nleaf, BLK)), dim3(BLK), 0, d_desc, nodes, nn, off,
                        leaf_lo, nleaf, depth, keys, ids);
-    CF_HIP(hipGetLastError());
 }
 
 void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out) {
     if (!nn) return;
-    hipLaunchKernelGGL(k_first_counts, dim3(grid_for(nn, BLK, 4096)), dim3(BLK), 0, ctx->stream, nodes, cnt, nn,
+    CF_LAUNCH(ctx, "k_first_counts", k_first_counts, dim3(grid_for(nn, BLK, 4096)), dim3(BLK), 0, nodes, cnt, nn,
                        (unsigned long long *)out);
-    CF_HIP(hipGetLastError());
 }
 
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids) {
     if (!count) return;
-    hipLaunchKernelGGL(k_gen_rank, dim3(grid_for(count, BLK)), dim3(BLK), 0, ctx->stream, d_desc, lo, count, keys, ids);
-    CF_HIP(hipGetLastError());
+    CF_LAUNCH(ctx, "k_gen_rank", k_gen_rank, dim3(grid_for(count, BLK)), dim3(BLK), 0, d_desc, lo, count, keys, ids);
 }
 
 // ---- host: per-side tables ---------------------------------------------------------------

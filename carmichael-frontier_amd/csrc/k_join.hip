@@ -63,15 +63,13 @@ inline unsigned grid_for(u64 n) {
 
 u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off) {
     if (!nI) return 0;
-    hipLaunchKernelGGL(k_join_count, dim3(grid_for(nI)), dim3(BLK), 0, ctx->stream, Dk, (u64)nD, Ik, (u64)nI, lo, cnt);
-    CF_HIP(hipGetLastError());
+    CF_LAUNCH(ctx, "k_join_count", k_join_count, dim3(grid_for(nI)), dim3(BLK), 0, Dk, (u64)nD, Ik, (u64)nI, lo, cnt);
     return scan_u64(ctx, cnt, off, nI);
 }
 
 void join_sorted_write(cf_ctx *ctx, const u64 *Did, const u64 *Iid, size_t nI, const u64 *lo, const u64 *cnt,
                        const u64 *off, u64 *od, u64 *oi, u64 *okey, const u64 *Ik) {
     if (!nI) return;
-    hipLaunchKernelGGL(k_join_write, dim3(grid_for(nI)), dim3(BLK), 0, ctx->stream, Did, Iid, (u64)nI, lo, cnt, off, od,
+    CF_LAUNCH(ctx, "k_join_write", k_join_write, dim3(grid_for(nI)), dim3(BLK), 0, Did, Iid, (u64)nI, lo, cnt, off, od,
                        oi, okey, Ik);
-    CF_HIP(hipGetLastError());
 }

@@ -89,16 +89,13 @@ template <typename T>
 void scan_impl(cf_ctx *ctx, const T *in, u64 *out, size_t n) {
     const size_t nb = (n + ST - 1) / ST;
     if (nb <= 1) {
-        hipLaunchKernelGGL((k_scan_final<T>), dim3(1), dim3(SB), 0, ctx->stream, in, n, (const u64 *)nullptr, out);
-        CF_HIP(hipGetLastError());
+        CF_LAUNCH(ctx, "k_scan_final", (k_scan_final<T>), dim3(1), dim3(SB), 0, in, n, (const u64 *)nullptr, out);
         return;
     }
     DBuf<u64> part(ctx, nb), part_off(ctx, nb + 1);
-    hipLaunchKernelGGL((k_scan_reduce<T>), dim3((unsigned)nb), dim3(SB), 0, ctx->stream, in, n, part.p);
-    CF_HIP(hipGetLastError());
+    CF_LAUNCH(ctx, "k_scan_reduce", (k_scan_reduce<T>), dim3((unsigned)nb), dim3(SB), 0, in, n, part.p);
     scan_impl<u64>(ctx, part.p, part_off.p, nb);
-    hipLaunchKernelGGL((k_scan_final<T>), dim3((unsigned)nb), dim3(SB), 0, ctx->stream, in, n, (const u64 *)part_off.p, out);
-    CF_HIP(hipGetLastError());
+    CF_LAUNCH(ctx, "k_scan_final", (k_scan_final<T>), dim3((unsigned)nb), dim3(SB), 0, in, n, (const u64 *)part_off.p, out);
 }
 
 // ---- radix sort -----------------------------------------------------------------------------
@@ -220,8 +217,7 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n) {
     if (n) {
         size_t g = (n + SB * 8 - 1) / (SB * 8);
         g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
-        hipLaunchKernelGGL(k_max, dim3((unsigned)g), dim3(SB), 0, ctx->stream, a, n, (unsigned long long *)m.p);
-        CF_HIP(hipGetLastError());
+        CF_LAUNCH(ctx, "k_max", k_max, dim3((unsigned)g), dim3(SB), 0, a, n, (unsigned long long *)m.p);
     }
     return d2h_u64(ctx, m.p);
 }
@@ -249,11 +245,9 @@ void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
     const unsigned grid = (unsigned)(ntiles < 65536 ? ntiles : 65536);
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = ps * 8;
-        hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(RB), 0, ctx->stream, ka, n, shift, hist.p, ntiles);
-        CF_HIP(hipGetLastError());
+        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist, dim3(grid), dim3(RB), 0, ka, n, shift, hist.p, ntiles);
         scan_impl<u32>(ctx, hist.p, off.p, ntiles * RADIX);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(RB), 0, ctx->stream, ka, va, n, shift, off.p, ntiles, kb, vb);
-        CF_HIP(hipGetLastError());
+        CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter, dim3(grid), dim3(RB), 0, ka, va, n, shift, off.p, ntiles, kb, vb);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

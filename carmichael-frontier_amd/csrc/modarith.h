@@ -17,6 +17,7 @@
 
 typedef uint64_t u64;
 typedef unsigned int u32;
+typedef unsigned char u8_t;
 
 #define CF_HD __host__ __device__ __forceinline__
 

@@ -1,0 +1,472 @@
+// solve.hip -- whole-instance and batched solves (host orchestration of the gfx950 kernels).
+//
+// Partitioned solve (default) of one or many instances (the device half of
+// gpu/mitm_gpu_mstar_driver.py:184-243 for every instance of a batch):
+//   1. D side of every instance: enumeration tree expanded level by level for all instances of
+//      one radius at once; instances live in disjoint key ranges [koff, koff + M) of one key
+//      space, so cross-instance keys never collide;
+//   2. D records counting-sorted into fine key buckets (k_part.hip);
+//   3. I side expanded likewise; its leaves, in chunks, are counted per coarse partition,
+//      scattered by partition and probed against the D buckets of their partition.
+// Sort-merge solve (CF_SOLVE_SORTMERGE): the reference's shape -- I chunks materialized, radix
+// sorted and joined against the sorted D side with the searchsorted semantics of
+// gpu/mitm_gpu2.py:107-119.
+#include <algorithm>
+#include <map>
+
+#include "cf_internal.h"
+
+typedef unsigned __int128 u128;
+
+namespace {
+
+struct Frontier {
+    DBuf<Node> nodes;
+    size_t nn = 0;
+    DBuf<u32> cnt;
+    DBuf<u64> off;
+    u64 total = 0;
+};
+
+// Expand the given depth-0 roots (one radius r) to the last level.
+Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const std::vector<Node> &roots, int r) {
+    Frontier f;
+    if (roots.empty()) return f;
+    DBuf<Node> cur(ctx, roots.size());
+    h2d(ctx, cur.p, roots.data(), roots.size());
+    size_t ncur = roots.size();
+    for (int depth = 0; depth + 1 < r; ++depth) {
+        DBuf<u32> c(ctx, ncur);
+        launch_expand_count(ctx, d_desc, cur.p, ncur, depth, c.p);
+        DBuf<u64> o(ctx, ncur + 1);
+        const u64 tot = scan_u32_to_u64(ctx, c.p, o.p, ncur);
+        if (tot == 0) return f;
+        DBuf<Node> nx(ctx, tot);
+        launch_expand_write(ctx, d_desc, cur.p, ncur, o.p, tot, depth, nx.p);
+        cur = std::move(nx);
+        ncur = tot;
+    }
+    f.cnt = DBuf<u32>(ctx, ncur);
+    launch_expand_count(ctx, d_desc, cur.p, ncur, r - 1, f.cnt.p);
+    f.off = DBuf<u64>(ctx, ncur + 1);
+    f.total = scan_u32_to_u64(ctx, f.cnt.p, f.off.p, ncur);
+    f.nodes = std::move(cur);
+    f.nn = ncur;
+    return f;
+}
+
+Node root_node(const SideDesc &sd, int side, int inst) {
+    Node nd{};
+    nd.A_lo = 1;
+    nd.A_hi = 0;
+    nd.xm = sd.xm0;
+    nd.x2 = sd.x20;
+    nd.id = 0;
+    const int nxt = side == SIDE_I ? 0 : sd.n - 1;
+    nd.pos = (u32)(unsigned short)(short)nxt;
+    nd.meta = (u32)inst << 8;
+    return nd;
+}
+
+}  // namespace
+
+// Validated side specs of one instance (the preconditions of cf_plan_create_d / _i).
+SideSpec spec_of_d(const cf_instance &in, u64 koff) {
+    CF_REQUIRE(in.nB >= 1 && in.nB <= 64 && in.r >= 1 && in.r <= 16, "need nB <= 64 base primes and 1 <= r <= 16");
+    CF_REQUIRE(in.M >= 1, "need 1 <= M < 2^64");
+    for (int i = 1; i < in.nB; ++i) CF_REQUIRE(in.B[i - 1] < in.B[i], "base must be strictly ascending");
+    CF_REQUIRE(in.d_id_mode == ID_PACK_DESC || in.d_id_mode == ID_MASK, "D ids are 6-bit packed or deletion masks");
+    if (in.d_id_mode == ID_PACK_DESC) CF_REQUIRE(in.r <= 10, "6-bit packed ids need r <= 10");
+    u128 w = 1;
+    for (int j = 0; j < in.r && j < in.nB; ++j) {
+        const u64 q = in.B[in.nB - 1 - j];
+        CF_REQUIRE(w <= (~(u128)0) / q, "products of r base primes exceed 128 bits");
+        w *= q;
+    }
+    SideSpec s;
+    s.side = SIDE_D;
+    s.primes.assign(in.B, in.B + in.nB);
+    s.r = in.r;
+    s.M = in.M;
+    s.mult = 1;
+    s.cap = ((u128)in.dmin_hi << 64) | in.dmin_lo;
+    s.id_mode = in.d_id_mode;
+    s.id_bits = 6;
+    s.koff = koff;
+    return s;
+}
+
+SideSpec spec_of_i(const cf_instance &in, u64 koff) {
+    CF_REQUIRE(in.r >= 1 && in.r <= 16 && in.nI >= 0 && in.nI <= 1024, "need 1 <= r <= 16 and n <= 1024");
+    CF_REQUIRE(in.M >= 1, "need 1 <= M < 2^64");
+    for (int i = 1; i < in.nI; ++i) CF_REQUIRE(in.INS[i - 1] < in.INS[i], "INS must be strictly ascending");
+    CF_REQUIRE(in.i_id_mode == ID_COLEX || in.i_id_mode == ID_PACK_ASC, "I ids are colex ranks or 8-bit packed");
+    if (in.i_id_mode == ID_PACK_ASC) CF_REQUIRE(in.nI <= 256 && in.r <= 8, "8-bit packed ids need n <= 256 and r <= 8");
+    if (in.i_id_mode == ID_COLEX) {
+        u128 c = 1;
+        bool over = false;
+        for (int j = 0; j < in.r && j < in.nI; ++j) {
+            c = c * (u128)(in.nI - j) / (u128)(j + 1);
+            if (c > (u128)~0ULL) over = true;
+        }
+        CF_REQUIRE(!over, "rank ids would overflow u64 (C(n, r) >= 2^64)");
+    }
+    SideSpec s;
+    s.side = SIDE_I;
+    s.primes.assign(in.INS, in.INS + in.nI);
+    s.r = in.r;
+    s.M = in.M;
+    s.mult = in.P0modM;
+    s.cap = ((u128)in.icap_hi << 64) | in.icap_lo;
+    s.split_idx = in.split_idx;
+    s.hmax = in.hmax;
+    s.id_mode = in.i_id_mode;
+    s.id_bits = 8;
+    s.koff = koff;
+    return s;
+}
+
+namespace {
+
+struct BatchOut {
+    DBuf<u64> pd, pi, pinst;
+    u64 npairs = 0;
+    std::vector<u64> nD, nI;
+};
+
+struct PhaseClock {
+    cf_ctx *ctx;
+    std::vector<hipEvent_t> ev;
+    explicit PhaseClock(cf_ctx *c) : ctx(c) {}
+    void mark() {
+        hipEvent_t e;
+        CF_HIP(hipEventCreate(&e));
+        CF_HIP(hipEventRecord(e, ctx->stream));
+        ev.push_back(e);
+    }
+    double ms(size_t a, size_t b) {
+        float t = 0;
+        CF_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+        return t;
+    }
+    ~PhaseClock() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// Per-instance fine buckets: instance j gets nb_j = a power of two near nD_j / 2 buckets at
+// bbase_j, addressed by a multiplicative hash of the key. P <= 128 coarse partitions are the top
+// bits of the global bucket index.
+struct Partitioning {
+    u64 NB = 0;                 // total buckets
+    int P = 1, pshift = 0;
+};
+
+Partitioning assign_buckets(const cf_instance *insts, int n, const std::vector<u64> &nD, std::vector<SideDesc> &dd,
+                            std::vector<SideDesc> &id) {
+    Partitioning pt;
+    u64 base = 0;
+    for (int j = 0; j < n; ++j) {
+        u64 nb = 1;
+        while (nb < nD[j] / 2) nb <<= 1;
+        for (SideDesc *d : {&dd[j], &id[j]}) {
+            d->bS = 0;
+            d->bbase = (u32)base;
+            d->nb = (u32)nb;
+        }
+        base += nb;
+    }
+    CF_REQUIRE(base < (1ULL << 32), "too many key buckets for one solve: split the batch");
+    pt.NB = base;
+    int lg = 0;
+    while ((1ULL << lg) < pt.NB) ++lg;
+    const int lgP = lg < 7 ? lg : 7;
+    pt.P = 1 << lgP;
+    pt.pshift = lg - lgP;
+    return pt;
+}
+
+void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_stats *st, u64 max_chunk) {
+    // key offsets: instance j owns keys [koff_j, koff_j + M_j); the union must stay below 2^64 - 1
+    std::vector<u64> koff(n);
+    u128 K = 0;
+    for (int j = 0; j < n; ++j) {
+        koff[j] = (u64)K;
+        K += insts[j].M;
+    }
+    CF_REQUIRE(K < (u128)~0ULL, "the batch's moduli sum to 2^64 or more: split the batch");
+    std::vector<SideTables> dt(n), it(n);
+    std::vector<SideDesc> dd(n), id(n);
+    std::vector<bool> i_live(n, true), d_live(n, true);
+    for (int j = 0; j < n; ++j) {
+        SideSpec sd = spec_of_d(insts[j], koff[j]);
+        SideSpec si = spec_of_i(insts[j], koff[j]);
+        build_side_tables(ctx, sd, dt[j]);
+        build_side_tables(ctx, si, it[j]);
+        dd[j] = dt[j].desc;
+        id[j] = it[j].desc;
+        if (si.cap <= 1 || (int)si.primes.size() < si.r) i_live[j] = false;
+        if ((int)sd.primes.size() < sd.r) d_live[j] = false;
+    }
+    DBuf<SideDesc> d_desc(ctx, n), i_desc(ctx, n);
+    DBuf<u64> koffs(ctx, n);
+    h2d(ctx, d_desc.p, dd.data(), n);
+    h2d(ctx, i_desc.p, id.data(), n);
+    h2d(ctx, koffs.p, koff.data(), n);
+    std::map<int, std::vector<int>> groups;
+    for (int j = 0; j < n; ++j) groups[insts[j].r].push_back(j);
+
+    PhaseClock clk(ctx);
+    clk.mark();  // 0
+    DBuf<u64> dcount(ctx, n), icount(ctx, n);
+    fill_u64(ctx, dcount.p, n, 0);
+    fill_u64(ctx, icount.p, n, 0);
+    // ---- D side: expand every radius group to its last level
+    std::vector<Frontier> dfs;
+    std::vector<int> dfs_r;
+    u64 nD = 0;
+    for (auto &g : groups) {
+        std::vector<Node> roots;
+        for (int j : g.second)
+            if (d_live[j]) roots.push_back(root_node(dd[j], SIDE_D, j));
+        Frontier f = expand_to_last(ctx, d_desc.p, roots, g.first);
+        if (!f.total) continue;
+        inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, dcount.p);
+        nD += f.total;
+        dfs.push_back(std::move(f));
+        dfs_r.push_back(g.first);
+    }
+    clk.mark();  // 1
+    // ---- D side: counting sort of the records into fine buckets sized per instance
+    std::vector<u64> nDj(n, 0);
+    d2h(ctx, nDj.data(), dcount.p, n);
+    const Partitioning pt = assign_buckets(insts, n, nDj, dd, id);
+    h2d(ctx, d_desc.p, dd.data(), n);
+    h2d(ctx, i_desc.p, id.data(), n);
+    DBuf<u32> bcount(ctx, pt.NB);
+    DBuf<u64> bstart(ctx, pt.NB + 1);
+    DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
+    fill_u32(ctx, bcount.p, pt.NB, 0);
+    for (size_t g = 0; g < dfs.size(); ++g)
+        part_bucket_count(ctx, d_desc.p, dfs[g].nodes.p, dfs[g].nn, dfs[g].off.p, 0, dfs[g].total, dfs_r[g] - 1,
+                          bcount.p);
+    scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
+    fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
+    for (size_t g = 0; g < dfs.size(); ++g)
+        part_bucket_scatter(ctx, d_desc.p, dfs[g].nodes.p, dfs[g].nn, dfs[g].off.p, 0, dfs[g].total, dfs_r[g] - 1,
+                            bstart.p, bcount.p, Dk.p, Did.p);
+    dfs.clear();
+    clk.mark();  // 2
+    clk.mark();  // 3
+    // ---- I side: expand, then per chunk of leaves: count per partition, scatter, probe
+    const u64 chunk = max_chunk ? max_chunk : ((u64)1 << 27);
+    u64 cap = 1 << 16;
+    DBuf<u64> cnt_pairs(ctx, 1);
+    DBuf<u32> pcount(ctx, pt.P), pcur(ctx, pt.P);
+    DBuf<u64> pstart(ctx, pt.P + 1);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        out.pd = DBuf<u64>(ctx, cap);
+        out.pi = DBuf<u64>(ctx, cap);
+        out.pinst = DBuf<u64>(ctx, cap);
+        fill_u64(ctx, cnt_pairs.p, 1, 0);
+        fill_u64(ctx, icount.p, n, 0);
+        DBuf<u64> Ik, Iid;
+        for (auto &g : groups) {
+            std::vector<Node> roots;
+            for (int j : g.second)
+                if (i_live[j]) roots.push_back(root_node(id[j], SIDE_I, j));
+            Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
+            if (!f.total) continue;
+            inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
+            if (!nD) continue;
+            for (u64 L = 0; L < f.total; L += chunk) {
+                const u64 m = std::min<u64>(chunk, f.total - L);
+                if (Ik.n < m) {
+                    Ik = DBuf<u64>(ctx, m);
+                    Iid = DBuf<u64>(ctx, m);
+                }
+                fill_u32(ctx, pcount.p, pt.P, 0);
+                part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.pshift, pt.P, pcount.p);
+                scan_u32_to_u64_async(ctx, pcount.p, pstart.p, pt.P);
+                fill_u32(ctx, pcur.p, pt.P, 0);
+                part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.pshift, pt.P, pstart.p,
+                             pcur.p, Ik.p, Iid.p);
+                part_probe(ctx, Ik.p, Iid.p, i_desc.p, koffs.p, n, m, bstart.p, Dk.p, Did.p,
+                           (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
+            }
+        }
+        const u64 np = d2h_u64(ctx, cnt_pairs.p);
+        out.npairs = np;
+        if (np <= cap) break;
+        cap = np;
+    }
+    clk.mark();  // 4
+    out.nD.assign(n, 0);
+    out.nI.assign(n, 0);
+    d2h(ctx, out.nD.data(), dcount.p, n);
+    d2h(ctx, out.nI.data(), icount.p, n);
+    if (st) {
+        st->nD = 0;
+        st->nI = 0;
+        for (int j = 0; j < n; ++j) {
+            st->nD += out.nD[j];
+            st->nI += out.nI[j];
+        }
+        st->npairs = out.npairs;
+        st->peak_bytes = g_peak_live;
+        st->ms_total = clk.ms(0, 4);
+        st->ms_gen_d = clk.ms(0, 1);
+        st->ms_sort_d = clk.ms(1, 2);
+        st->ms_gen_i = clk.ms(3, 4);
+        st->ms_sort_i = 0;
+        st->ms_join = 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_d, uint64_t **out_i,
+                   uint64_t **out_inst, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats) {
+    API_BEGIN
+    CF_REQUIRE(ctx && insts && n >= 1 && out_d && out_i && out_inst && npairs, "NULL argument");
+    *out_d = *out_i = *out_inst = nullptr;
+    *npairs = 0;
+    g_peak_live = 0;
+    BatchOut bo;
+    solve_part(ctx, insts, n, bo, stats, 0);
+    if (bo.npairs) {
+        *out_d = bo.pd.release();
+        *out_i = bo.pi.release();
+        *out_inst = bo.pinst.release();
+    }
+    *npairs = bo.npairs;
+    if (inst_counts)
+        for (int j = 0; j < n; ++j) {
+            inst_counts[2 * j] = bo.nD[j];
+            inst_counts[2 * j + 1] = bo.nI[j];
+        }
+    API_END
+}
+
+int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk, uint64_t **out_d, uint64_t **out_i,
+             uint64_t *npairs, cf_stats *stats) {
+    API_BEGIN
+    CF_REQUIRE(ctx && inst && out_d && out_i && npairs, "NULL argument");
+    CF_REQUIRE(mode == CF_SOLVE_PARTITION || mode == CF_SOLVE_SORTMERGE, "unknown solve mode");
+    *out_d = nullptr;
+    *out_i = nullptr;
+    *npairs = 0;
+    g_peak_live = 0;
+    if (mode == CF_SOLVE_PARTITION) {
+        BatchOut bo;
+        solve_part(ctx, inst, 1, bo, stats, max_chunk);
+        if (bo.npairs) {
+            *out_d = bo.pd.release();
+            *out_i = bo.pi.release();
+        }
+        *npairs = bo.npairs;
+        return CF_OK;
+    }
+    // ---- sort-merge: D generated and sorted; I streamed in chunks through sort + sorted join
+    hipEvent_t ev[6];
+    for (auto &e : ev) CF_HIP(hipEventCreate(&e));
+    CF_HIP(hipEventRecord(ev[0], ctx->stream));
+    cf_plan *pd = nullptr;
+    int rc = cf_plan_create_d(ctx, inst->B, inst->nB, inst->r, inst->M, inst->dmin_lo, inst->dmin_hi, inst->d_id_mode, &pd);
+    if (rc) throw CfError{rc, g_err};
+    std::unique_ptr<cf_plan> hold_d(pd);
+    const u64 nD = pd->empty ? 0 : pd->total;
+    DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
+    plan_write_range(pd, 0, nD, Dk.p, Did.p);
+    hold_d.reset();
+    CF_HIP(hipEventRecord(ev[1], ctx->stream));
+    const int kbits = bitlen(inst->M - 1 ? inst->M - 1 : 1);
+    radix_sort_pairs(ctx, Dk.p, Did.p, nD, kbits);
+    CF_HIP(hipEventRecord(ev[2], ctx->stream));
+    cf_plan *pi = nullptr;
+    rc = cf_plan_create_i(ctx, inst->INS, inst->nI, inst->r, inst->M, inst->P0modM, inst->icap_lo, inst->icap_hi,
+                          inst->split_idx, inst->hmax, inst->i_id_mode, &pi);
+    if (rc) throw CfError{rc, g_err};
+    std::unique_ptr<cf_plan> hold_i(pi);
+    const u64 nI = pi->empty ? 0 : pi->total;
+    float ms_gen_i = 0, ms_sort_i = 0, ms_join = 0;
+    CF_HIP(hipEventRecord(ev[3], ctx->stream));
+    std::vector<std::pair<DBuf<u64>, DBuf<u64>>> parts;
+    u64 total_pairs = 0;
+    if (nI && nD) {
+        const u64 chunk = max_chunk ? max_chunk : (u64)1 << 28;
+        const u64 cs = std::min<u64>(chunk, nI);
+        DBuf<u64> Ik(ctx, cs), Iid(ctx, cs), lo(ctx, cs), cnt(ctx, cs), off(ctx, cs + 1);
+        hipEvent_t a, b, c, d;
+        CF_HIP(hipEventCreate(&a));
+        CF_HIP(hipEventCreate(&b));
+        CF_HIP(hipEventCreate(&c));
+        CF_HIP(hipEventCreate(&d));
+        for (u64 L = 0; L < nI; L += chunk) {
+            const u64 m = std::min<u64>(chunk, nI - L);
+            CF_HIP(hipEventRecord(a, ctx->stream));
+            plan_write_range(pi, L, L + m, Ik.p, Iid.p);
+            CF_HIP(hipEventRecord(b, ctx->stream));
+            radix_sort_pairs(ctx, Ik.p, Iid.p, m, kbits);
+            CF_HIP(hipEventRecord(c, ctx->stream));
+            const u64 tot = join_sorted_count(ctx, Dk.p, nD, Ik.p, m, lo.p, cnt.p, off.p);
+            if (tot) {
+                DBuf<u64> od(ctx, tot), oi(ctx, tot);
+                join_sorted_write(ctx, Did.p, Iid.p, m, lo.p, cnt.p, off.p, od.p, oi.p, nullptr, Ik.p);
+                parts.emplace_back(std::move(od), std::move(oi));
+                total_pairs += tot;
+            }
+            CF_HIP(hipEventRecord(d, ctx->stream));
+            CF_HIP(hipEventSynchronize(d));
+            float t1, t2, t3;
+            CF_HIP(hipEventElapsedTime(&t1, a, b));
+            CF_HIP(hipEventElapsedTime(&t2, b, c));
+            CF_HIP(hipEventElapsedTime(&t3, c, d));
+            ms_gen_i += t1;
+            ms_sort_i += t2;
+            ms_join += t3;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipEventDestroy(c);
+        (void)hipEventDestroy(d);
+    }
+    if (total_pairs) {
+        DBuf<u64> od(ctx, total_pairs), oi(ctx, total_pairs);
+        u64 at = 0;
+        for (auto &pr : parts) {
+            CF_HIP(hipMemcpyAsync(od.p + at, pr.first.p, pr.first.n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(oi.p + at, pr.second.p, pr.second.n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+            at += pr.first.n;
+        }
+        parts.clear();
+        *out_d = od.release();
+        *out_i = oi.release();
+    }
+    CF_HIP(hipEventRecord(ev[4], ctx->stream));
+    CF_HIP(hipEventSynchronize(ev[4]));
+    *npairs = total_pairs;
+    if (stats) {
+        float a, b, c, t;
+        CF_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        CF_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        CF_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+        CF_HIP(hipEventElapsedTime(&t, ev[0], ev[4]));
+        stats->nD = nD;
+        stats->nI = nI;
+        stats->npairs = total_pairs;
+        stats->peak_bytes = g_peak_live;
+        stats->ms_total = t;
+        stats->ms_gen_d = a;
+        stats->ms_sort_d = b;
+        stats->ms_gen_i = c + ms_gen_i;
+        stats->ms_sort_i = ms_sort_i;
+        stats->ms_join = ms_join;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    API_END
+}
+
+}  // extern "C"

@@ -61,6 +61,18 @@ int cf_memcpy_d2h(cf_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes); 
 int cf_memcpy_d2d(cf_ctx *ctx, void *dst_dev, const void *src_dev, uint64_t bytes);
 int cf_synchronize(cf_ctx *ctx);
 
+/* ---- kernel timing (no counterpart in the reference; its drivers time phases on the host,
+ * gpu/mitm_gpu_mstar_driver.py:55,241) -------------------------------------------------------
+ * While enabled, launches of the named kernel (all kernels if kernel == NULL) are bracketed by
+ * HIP events on the context's stream; cf_prof_read returns the summed device time and the
+ * launch count since the last reset. */
+int cf_prof_enable(cf_ctx *ctx, const char *kernel);
+int cf_prof_disable(cf_ctx *ctx);
+int cf_prof_reset(cf_ctx *ctx);
+int cf_prof_read(cf_ctx *ctx, const char *kernel, double *ms, uint64_t *launches);
+/* every timed kernel as text lines "name total_ms launches" (truncated to size - 1 bytes) */
+int cf_prof_dump(cf_ctx *ctx, char *buf, uint64_t size);
+
 /* ---- r-subsets by rank range ------------------------------------------------------------
  * Replaces gen_records + generate() (gpu/mitm_gpu2.py:25-98): records of the r-combinations of
  * primes[0..n) with combinatorial-number-system rank in [lo, lo+count), in rank order;
@@ -106,10 +118,11 @@ int cf_plan_write(cf_plan *plan, int first_lo, int first_hi, uint64_t *keys_dev,
                   uint64_t *written);
 int cf_plan_destroy(cf_plan *plan);
 
-/* ---- one whole instance -----------------------------------------------------------------
- * The device half of gpu/mitm_gpu_mstar_driver.py:209-243 (build_D with DFILTER, GPU I
- * generation, sort, join against every D chunk) as one call: returns every equal-residue pair
- * (D id, I id) of the instance, in device memory from the pool. */
+/* ---- whole instances ---------------------------------------------------------------------
+ * The device half of gpu/mitm_gpu_mstar_driver.py:184-243 (build_D with DFILTER, I generation,
+ * sort, join against every D chunk) for one instance per call (cf_solve) or for many instances
+ * in one set of launches (cf_solve_batch): every equal-residue pair (D id, I id). Output arrays
+ * come from the context pool (release with cf_free); all NULL when there is no pair. */
 typedef struct cf_instance {
     uint64_t M;
     int32_t r;
@@ -134,8 +147,21 @@ typedef struct cf_stats {
     double ms_gen_d, ms_sort_d, ms_gen_i, ms_sort_i, ms_join;
 } cf_stats;
 
-int cf_solve(cf_ctx *ctx, const cf_instance *inst, uint64_t max_chunk, uint64_t **out_d_dev,
+/* mode: CF_SOLVE_PARTITION -- D records counting-sorted into fine key buckets; I leaves, in
+ *       chunks of at most max_chunk (0: 2^27), partitioned by key range and probed against the
+ *       D buckets of their partition; CF_SOLVE_SORTMERGE -- I chunks radix sorted and joined as
+ *       join_sorted(). Same pair set. */
+#define CF_SOLVE_PARTITION 0
+#define CF_SOLVE_SORTMERGE 1
+int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk, uint64_t **out_d_dev,
              uint64_t **out_i_dev, uint64_t *npairs, cf_stats *stats);
+
+/* n instances of any radii in one solve: each instance's keys occupy their own range of one key
+ * space, so one D sort, one probe table and one set of launches per radius serve the batch.
+ * out_inst_dev[k] is the index (into insts) of pair k; inst_counts (host, 2n entries) receives
+ * nD, nI of every instance. Requires the sum of the moduli to stay below 2^64 - 1. */
+int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_d_dev, uint64_t **out_i_dev,
+                   uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);
 
 #ifdef __cplusplus
 }

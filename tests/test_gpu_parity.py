@@ -16,6 +16,8 @@ from cfjoin import pipeline
 
 pytestmark = pytest.mark.gpu
 
+GOLDEN_LOGGED = __import__('os').path.join(__import__('os').path.dirname(__file__), 'golden', 'logged_instances.json')
+
 
 def say(*a):
     print("CASE", *a, flush=True)
@@ -218,11 +220,14 @@ def test_generate_chunks_boundaries(workloads):
 def test_solve_matches_cpu_join(workloads, fam, D, r, max_chunk):
     inst = family(workloads, fam, D, r)
     say("solve", fam, D, r, len(inst.INS), max_chunk)
-    res = pipeline.solve(inst, max_chunk=max_chunk)
+    res = pipeline.solve(inst, max_chunk=max_chunk, mode="sortmerge" if max_chunk else "partition")
     ref = O.cpu_join(inst.M, r, inst.P0 % inst.M, inst.Icap, inst.Dmin, inst.B0, inst.INS, inst.split_idx, inst.hmax,
                      inst.rank_ids, threads=8)
     assert (res.nD, res.nI) == (ref["nD"], ref["nI"])
     assert np.array_equal(res.pair_array(), O.pair_set(ref["pairs_d"], ref["pairs_i"]))
+    if max_chunk:
+        res2 = pipeline.solve(inst, max_chunk=max_chunk // 7 + 1, mode="partition")
+        assert np.array_equal(res2.pair_array(), res.pair_array())
     for rec in pipeline.recheck(inst, res):
         assert rec["D"] and rec["I"]
 
@@ -238,3 +243,35 @@ def test_solve_c1(seed, s):
     ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
     assert (res.nD, res.nI) == (len(Dk), len(Ik))
     assert np.array_equal(res.pair_array(), O.pair_set(ed, ei))
+
+
+# ---- fused and batched solves ---------------------------------------------------------------
+def test_partition_equals_sortmerge_and_logged(workloads):
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    for fam, D, r in [("M0", 1768248177696000, 5), ("lambdaN", 155016423578016000, 5), ("M0", 1014485472000, 5)]:
+        inst = family(workloads, fam, D, r)
+        say("partition", fam, D, r)
+        a = pipeline.solve(inst, mode="partition")
+        b = pipeline.solve(inst, mode="sortmerge")
+        lg = logged[(fam, D, r)]
+        assert (a.nD, a.nI, len(a.pairs_d)) == (b.nD, b.nI, len(b.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"])
+        assert np.array_equal(a.pair_array(), b.pair_array())
+
+
+def test_batch_equals_alone(workloads):
+    """Every instance's pair set is identical solved alone and solved inside a mixed batch."""
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    insts = W.batch_instances("lambdaN", radii=(3, 4, 5, 6), logged=logged, limit=60)
+    insts += [I.c1_instance(0, 100), I.c1_instance(1, None)]
+    say("batch", len(insts))
+    res, st = pipeline.solve_batch(insts)
+    assert len(res) == len(insts)
+    for inst, rb in zip(insts, res):
+        ra = pipeline.solve(inst)
+        assert (rb.nD, rb.nI) == (ra.nD, ra.nI), inst.tag
+        assert np.array_equal(rb.pair_array(), ra.pair_array()), inst.tag
+        lg = inst.meta.get("logged")
+        if lg:
+            assert (rb.nD, rb.nI, len(rb.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]), inst.tag

@@ -152,9 +152,6 @@ struct SideDesc {
     u64 koff;           // offset added to every key (disjoint key ranges across a batch)
     int n, r, side, id_mode, id_bits, split_idx, hmax, pad;
     u64 xm0, x20;       // initial residue state (the side multiplier)
-    // this instance's fine key buckets: bucket(key) = bbase + mulhi(key * golden, nb)
-    u64 bS;             // unused
-    u32 bbase, nb;
 };
 
 // Host-owned copy of one side's device tables.
@@ -222,16 +219,17 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, u32 *bcount);
+                       int depth, u64 NB, u32 *bcount);
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
+                         u64 n, int depth, u64 NB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
+void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, u32 *T);
 void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, int pshift, int P, u32 *pcount);
+                int depth, u64 NB, int pshift, int P, u32 *pcount);
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, int pshift, int P, const u64 *pstart, u32 *pcur, u64 *Ik, u64 *Iid);
-void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const SideDesc *descs, const u64 *koffs, int ninst, u64 nI,
-                const u64 *bstart,
-                const u64 *Dk, const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
+                  int depth, u64 NB, int pshift, int P, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *Ik, u64 *Iid);
+void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const u64 *pbase, const u32 *pfill, int P, u64 nslots,
+                const u32 *ovf, const u64 *koffs, int ninst, u64 NB, const u32 *T, const u64 *bstart, const u64 *Dk,
+                const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
 
 // joins

@@ -1,12 +1,15 @@
 // k_part.hip -- radix-partitioned residue join on gfx950.
 //
-// Keys of one solve live in one key space (instance j owns [koff_j, koff_j + M_j)). Instance j
-// owns the fine buckets [bbase_j, bbase_j + nb_j), nb_j near half its D count, reached by
-// multiplicative hashing  b = bbase_j + mulhi(key * 0x9E3779B97F4A7C15, nb_j). Hashing, not key
-// order, because the D keys are far from uniform: for small r the products prod(D) stay below M
-// and are their own residues, all in a sliver of [0, M). The top bits of b give P <= 128 coarse
-// partitions. Sizing the buckets per instance keeps them at a few records however uneven the
-// moduli and sides of a batch are.
+// Keys of one solve live in one key space (instance j owns [koff_j, koff_j + M_j), so keys of
+// different instances never collide). The space is cut into NB (a power of two, about a third of
+// the D records) fine buckets by multiplicative hashing  b = mulhi(key * 0x9E3779B97F4A7C15, NB).
+// Hashing, not key order, because keys are far from uniform: for small r the products prod(D) stay
+// below M and are their own residues, all in a sliver of [0, M), and a batch mixes moduli from
+// 1e11 to 1e17. The top bits of b give P <= 128 coarse partitions.
+//
+// Probe table: per bucket 8 u32 fingerprints (32 bytes, one memory sector) of its first 8 D
+// records; a probe reads its bucket's 32 bytes once and touches the D records only on a
+// fingerprint hit (exact key check) or when the bucket holds more than 8 records.
 //
 //  D side: records are counted per fine bucket and scattered into bucket order (a counting sort
 //          on b: no full key sort is needed, a probe scans its bucket, 2-4 records on average).
@@ -23,10 +26,10 @@
 namespace {
 
 constexpr int PB = 256;            // threads per block
-constexpr int PI = 16;             // leaves per thread (count and probe passes)
-constexpr int PT = PB * PI;        // 4096 leaves per tile
-constexpr int SI = 8;              // leaves per thread in the scatter (bounds its registers and LDS)
-constexpr int ST = PB * SI;        // 2048 leaves per scatter tile
+constexpr int LPT = 8;             // consecutive leaves per thread in the leaf kernels
+constexpr int LT = PB * LPT;       // 2048 leaves per tile
+constexpr int PI = 16;             // records per thread in the probe
+constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 
 struct LeafSrc {
@@ -36,94 +39,152 @@ struct LeafSrc {
     const u64 *off;   // exclusive scan of leaves per node, nn + 1 entries
     u64 lo, n;        // leaves [lo, lo + n)
     int depth;        // r - 1
+    const u64 *tp;    // per tile of LT leaves: owners of its first and last leaf (2 entries per tile)
 };
 
-__device__ __forceinline__ u64 bucket_of(const SideDesc &s, u64 key) {
-    return s.bbase + cf_mulhi(key * 0x9E3779B97F4A7C15ULL, s.nb);
-}
+__device__ __forceinline__ u64 bucket_of(u64 key, u64 NB) { return cf_mulhi(key * 0x9E3779B97F4A7C15ULL, NB); }
+// nonzero 32-bit fingerprint from an independent multiplier; 0 marks an empty slot
+__device__ __forceinline__ u32 fingerprint(u64 key) { return (u32)((key * 0xD6E8FEB86659FD93ULL) >> 32) | 1u; }
+constexpr int TSLOTS = 8;
 
-__device__ __forceinline__ void leaf_eval(const LeafSrc &L, size_t p, u64 j, u64 &key, u64 &id, u32 &inst) {
-    const Node nd = L.nodes[p];
-    inst = nd.meta >> 8;
-    const SideDesc &s = L.descs[inst];
-    const int x = child_index(s, nd, (int)(j - L.off[p]));
-    const u64 xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
-    const u64 x2 = nd.x2 * s.p[x];
-    key = cf_key(xm, x2, s.mc) + s.koff;
-    id = id_update(s, nd.id, x, L.depth);
-}
-
-// Parents of the first and last leaf of a tile, shared through LDS.
-__device__ __forceinline__ void tile_parents(const LeafSrc &L, u64 j0, u64 j1, size_t *p0, size_t *p1) {
-    __shared__ size_t s_p[2];
-    if (threadIdx.x == 0) s_p[0] = find_parent(L.off, 0, L.nn - 1, j0);
-    if (threadIdx.x == 1) s_p[1] = find_parent(L.off, 0, L.nn - 1, j1 - 1);
-    __syncthreads();
-    *p0 = s_p[0];
-    *p1 = s_p[1];
+// Tile set-up: the nodes owning leaves [j0, j1) are p0 .. p0 + np - 1 (every last-level node owns at
+// least one leaf, so np <= LT); their leaf offsets go to LDS: s_off[q] = off[p0 + q], q <= np.
+__device__ __forceinline__ void tile_setup(const LeafSrc &L, size_t t, u64 *s_off, size_t &p0, u32 &np) {
+    p0 = L.tp[2 * t];
+    np = (u32)(L.tp[2 * t + 1] - p0 + 1);
+    const u32 nc = np <= (u32)LT ? np : (u32)LT;
+    for (u32 i = threadIdx.x; i <= nc; i += PB) s_off[i] = L.off[p0 + i];
     __syncthreads();
 }
 
-// ---- D side: counting sort by fine bucket ---------------------------------------------------
-__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, u32 *__restrict__ bcount) {
-    const size_t ntiles = (L.n + PB - 1) / PB;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * PB, jend = L.lo + L.n;
-        const u64 j1 = j0 + PB < jend ? j0 + PB : jend;
-        size_t p0, p1;
-        tile_parents(L, j0, j1, &p0, &p1);
-        const u64 j = j0 + threadIdx.x;
-        if (j < j1) {
-            u64 key, id;
-            u32 inst;
-            leaf_eval(L, find_parent(L.off, p0, p1, j), j, key, id, inst);
-            atomicAdd(&bcount[bucket_of(L.descs[inst], key)], 1u);
+// The leaves [jt, jt + LPT) of one thread, walked in order: the owning node is found once (binary
+// search of the tile's offsets in LDS), then advanced as the leaf index crosses node boundaries.
+struct LeafWalk {
+    const LeafSrc &L;
+    const u64 *s_off;
+    size_t p0;
+    u32 np, q;
+    u64 lo_off, hi_off;
+    Node nd;
+    const SideDesc *s;
+
+    __device__ __forceinline__ LeafWalk(const LeafSrc &L_, const u64 *s_off_, size_t p0_, u32 np_, u64 jt)
+        : L(L_), s_off(s_off_), p0(p0_), np(np_) {
+        u32 a = 0, b = np - 1;
+        while (a < b) {
+            const u32 mid = (a + b + 1) >> 1;
+            if (off_at(mid) <= jt) a = mid; else b = mid - 1;
         }
+        q = a;
+        load();
+    }
+    __device__ __forceinline__ u64 off_at(u32 i) const { return np <= (u32)LT ? s_off[i] : L.off[p0 + i]; }
+    __device__ __forceinline__ void load() {
+        lo_off = off_at(q);
+        hi_off = off_at(q + 1);
+        nd = L.nodes[p0 + q];
+        s = &L.descs[nd.meta >> 8];
+    }
+    // key, id and instance of leaf j (j >= every earlier j of this walk)
+    __device__ __forceinline__ void eval(u64 j, u64 &key, u64 &id, u32 &inst) {
+        while (j >= hi_off) {
+            ++q;
+            load();
+        }
+        inst = nd.meta >> 8;
+        const int x = child_index(*s, nd, (int)(j - lo_off));
+        const u64 xm = cf_mulmod_m(nd.xm, s->pR[x], s->mc);
+        const u64 x2 = nd.x2 * s->p[x];
+        key = cf_key(xm, x2, s->mc) + s->koff;
+        id = id_update(*s, nd.id, x, L.depth);
+    }
+};
+
+// Owners of the first and the last leaf of every LT-leaf tile of [lo, lo + n): one binary search
+// per lane, all tiles at once, so no leaf kernel starts a tile with a serial search.
+__global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off, size_t nn, u64 lo, u64 n,
+                                                     u64 *__restrict__ tp) {
+    const u64 ntiles = (n + LT - 1) / LT;
+    for (u64 i = (u64)blockIdx.x * PB + threadIdx.x; i < 2 * ntiles; i += (u64)gridDim.x * PB) {
+        const u64 t = i >> 1;
+        const u64 j0 = lo + t * LT;
+        const u64 j = (i & 1) ? (j0 + LT < lo + n ? j0 + LT : lo + n) - 1 : j0;
+        tp[i] = find_parent(off, 0, nn - 1, j);
     }
 }
 
-__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, const u64 *__restrict__ bstart,
+// ---- D side: counting sort by fine bucket ---------------------------------------------------
+__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, u64 NB, u32 *__restrict__ bcount) {
+    __shared__ u64 s_off[LT + 1];
+    const size_t ntiles = (L.n + LT - 1) / LT;
+    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
+        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
+        size_t p0;
+        u32 np;
+        tile_setup(L, t, s_off, p0, np);
+        const u64 jt = j0 + (u64)threadIdx.x * LPT;
+        if (jt < j1) {
+            LeafWalk w(L, s_off, p0, np, jt);
+            for (int k = 0; k < LPT && jt + k < j1; ++k) {
+                u64 key, id;
+                u32 inst;
+                w.eval(jt + k, key, id, inst);
+                atomicAdd(&bcount[bucket_of(key, NB)], 1u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, u64 NB, const u64 *__restrict__ bstart,
                                                        u32 *__restrict__ bcur, u64 *__restrict__ Dk,
                                                        u64 *__restrict__ Did) {
-    const size_t ntiles = (L.n + PB - 1) / PB;
+    __shared__ u64 s_off[LT + 1];
+    const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * PB, jend = L.lo + L.n;
-        const u64 j1 = j0 + PB < jend ? j0 + PB : jend;
-        size_t p0, p1;
-        tile_parents(L, j0, j1, &p0, &p1);
-        const u64 j = j0 + threadIdx.x;
-        if (j < j1) {
-            u64 key, id;
-            u32 inst;
-            leaf_eval(L, find_parent(L.off, p0, p1, j), j, key, id, inst);
-            const u64 b = bucket_of(L.descs[inst], key);
-            const u64 at = bstart[b] + atomicAdd(&bcur[b], 1u);
-            Dk[at] = key;
-            Did[at] = id;
+        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
+        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
+        size_t p0;
+        u32 np;
+        tile_setup(L, t, s_off, p0, np);
+        const u64 jt = j0 + (u64)threadIdx.x * LPT;
+        if (jt < j1) {
+            LeafWalk w(L, s_off, p0, np, jt);
+            for (int k = 0; k < LPT && jt + k < j1; ++k) {
+                u64 key, id;
+                u32 inst;
+                w.eval(jt + k, key, id, inst);
+                const u64 b = bucket_of(key, NB);
+                const u64 at = bstart[b] + atomicAdd(&bcur[b], 1u);
+                Dk[at] = key;
+                Did[at] = id;
+            }
         }
+        __syncthreads();
     }
 }
 
 // ---- I side: partition by the top bits of the fine bucket -----------------------------------
-__global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, int pshift, int P, u32 *__restrict__ pcount) {
+__global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, u64 NB, int pshift, int P, u32 *__restrict__ pcount) {
+    __shared__ u64 s_off[LT + 1];
     __shared__ u32 h[256];
-    const size_t ntiles = (L.n + PT - 1) / PT;
+    const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * PT, jend = L.lo + L.n;
-        const u64 j1 = j0 + PT < jend ? j0 + PT : jend;
+        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
+        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
         h[threadIdx.x] = 0;
-        size_t p0, p1;
-        tile_parents(L, j0, j1, &p0, &p1);
-        size_t pp = p0;
-#pragma unroll 4
-        for (int it = 0; it < PI; ++it) {
-            const u64 j = j0 + (u64)it * PB + threadIdx.x;
-            if (j < j1) {
-                pp = find_parent(L.off, pp, p1, j);
+        size_t p0;
+        u32 np;
+        tile_setup(L, t, s_off, p0, np);
+        const u64 jt = j0 + (u64)threadIdx.x * LPT;
+        if (jt < j1) {
+            LeafWalk w(L, s_off, p0, np, jt);
+            for (int k = 0; k < LPT && jt + k < j1; ++k) {
                 u64 key, id;
                 u32 inst;
-                leaf_eval(L, pp, j, key, id, inst);
-                atomicAdd(&h[(u32)(bucket_of(L.descs[inst], key) >> pshift)], 1u);
+                w.eval(jt + k, key, id, inst);
+                atomicAdd(&h[(u32)(bucket_of(key, NB) >> pshift)], 1u);
             }
         }
         __syncthreads();
@@ -132,71 +193,85 @@ __global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, int pshift, int P,
     }
 }
 
-__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, int pshift, int P, const u64 *__restrict__ pstart,
-                                                     u32 *__restrict__ pcur, u64 *__restrict__ Ik, u64 *__restrict__ Iid) {
+// Partition p owns output slots [pbase[p], pbase[p + 1]); pcur[p] counts the slots taken. With a
+// capacity layout (exact == 0) a partition that would run past its end sets *ovf and writes
+// nothing for that tile; the caller then redoes the chunk with exact counts (exact == 1).
+__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshift, int P, const u64 *__restrict__ pbase,
+                                                     u32 *__restrict__ pcur, u32 *__restrict__ ovf,
+                                                     u64 *__restrict__ Ik, u64 *__restrict__ Iid) {
+    __shared__ u64 s_off[LT + 1];
     __shared__ u32 h[256];
     __shared__ u32 loc[256];
     __shared__ u64 gbase[256];
     __shared__ u32 s_w[PB / 64];
-    __shared__ u64 s_k[ST];
-    __shared__ u64 s_i[ST];
-    __shared__ u8_t s_p[ST];
-    const size_t ntiles = (L.n + ST - 1) / ST;
+    __shared__ u64 s_k[LT];
+    __shared__ u64 s_i[LT];
+    __shared__ u8_t s_p[LT];
+    constexpr u64 NONE = ~0ULL;
+    const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * ST, jend = L.lo + L.n;
-        const u64 j1 = j0 + ST < jend ? j0 + ST : jend;
+        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
+        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
         const int cnt = (int)(j1 - j0);
         h[threadIdx.x] = 0;
-        size_t p0, p1;
-        tile_parents(L, j0, j1, &p0, &p1);
-        u64 key[SI], id[SI];
-        u32 part[SI], rank[SI];
-        size_t pp = p0;
+        size_t p0;
+        u32 np;
+        tile_setup(L, t, s_off, p0, np);
+        u64 key[LPT], id[LPT];
+        u32 part[LPT], rank[LPT];
+        const u64 jt = j0 + (u64)threadIdx.x * LPT;
+        if (jt < j1) {
+            LeafWalk w(L, s_off, p0, np, jt);
 #pragma unroll
-        for (int it = 0; it < SI; ++it) {
-            const u64 j = j0 + (u64)it * PB + threadIdx.x;
-            if (j < j1) {
-                u32 inst;
-                pp = find_parent(L.off, pp, p1, j);
-                leaf_eval(L, pp, j, key[it], id[it], inst);
-                part[it] = (u32)(bucket_of(L.descs[inst], key[it]) >> pshift);
-                rank[it] = atomicAdd(&h[part[it]], 1u);
+            for (int k = 0; k < LPT; ++k) {
+                if (jt + k < j1) {
+                    u32 inst;
+                    w.eval(jt + k, key[k], id[k], inst);
+                    part[k] = (u32)(bucket_of(key[k], NB) >> pshift);
+                    rank[k] = atomicAdd(&h[part[k]], 1u);
+                }
             }
         }
         __syncthreads();
-        // local exclusive offsets of the partitions within the tile (block scan of 256 counts),
-        // and one global reservation per non-empty partition
-        {
+        {   // local exclusive offsets of the partitions within the tile; one reservation per partition
             const u32 v = h[threadIdx.x];
-            const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
             u32 inc = v;
 #pragma unroll
             for (int d = 1; d < 64; d <<= 1) {
                 const u32 o = __shfl_up(inc, d, 64);
                 if (lane >= d) inc += o;
             }
-            if (lane == 63) s_w[w] = inc;
+            if (lane == 63) s_w[wv] = inc;
             __syncthreads();
             u32 pre = 0;
-            for (int q = 0; q < w; ++q) pre += s_w[q];
+            for (int q = 0; q < wv; ++q) pre += s_w[q];
             loc[threadIdx.x] = pre + inc - v;
-            gbase[threadIdx.x] = (threadIdx.x < (unsigned)P && v) ? pstart[threadIdx.x] + atomicAdd(&pcur[threadIdx.x], v) : 0;
+            u64 gb = NONE;
+            if (threadIdx.x < (unsigned)P && v) {
+                const u64 r0 = atomicAdd(&pcur[threadIdx.x], v);
+                const u64 b0 = pbase[threadIdx.x] + r0;
+                if (b0 + v <= pbase[threadIdx.x + 1]) gb = b0;
+                else atomicOr(ovf, 1u);
+            }
+            gbase[threadIdx.x] = gb;
         }
         __syncthreads();
+        if (jt < j1) {
 #pragma unroll
-        for (int it = 0; it < SI; ++it) {
-            const int le = it * PB + threadIdx.x;
-            if (le < cnt) {
-                const u32 at = loc[part[it]] + rank[it];
-                s_k[at] = key[it];
-                s_i[at] = id[it];
-                s_p[at] = (u8_t)part[it];
+            for (int k = 0; k < LPT; ++k) {
+                if (jt + k < j1) {
+                    const u32 at = loc[part[k]] + rank[k];
+                    s_k[at] = key[k];
+                    s_i[at] = id[k];
+                    s_p[at] = (u8_t)part[k];
+                }
             }
         }
         __syncthreads();
-        // staged slot q belongs to partition s_p[q]; its runs are contiguous in the output
         for (int q = threadIdx.x; q < cnt; q += PB) {
             const u32 pi = s_p[q];
+            if (gbase[pi] == NONE) continue;
             const u64 at = gbase[pi] + (u64)(q - (int)loc[pi]);
             Ik[at] = s_k[q];
             Iid[at] = s_i[q];
@@ -205,7 +280,6 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, int pshift, int 
     }
 }
 
-// ---- probe: partitioned I records against the bucketed D records ----------------------------
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).
 __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ninst, u64 key) {
     int a = 0, b = ninst - 1;
@@ -216,39 +290,108 @@ __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ni
     return (u32)a;
 }
 
+__global__ void __launch_bounds__(PB) k_table_build(const u64 *__restrict__ bstart, const u64 *__restrict__ Dk, u64 NB,
+                                                   u32 *__restrict__ T) {
+    for (u64 b = (u64)blockIdx.x * PB + threadIdx.x; b < NB; b += (u64)gridDim.x * PB) {
+        const u64 lo = bstart[b], cnt = bstart[b + 1] - lo;
+        u32 f[TSLOTS];
+#pragma unroll
+        for (int k = 0; k < TSLOTS; ++k) f[k] = (u64)k < cnt ? fingerprint(Dk[lo + k]) : 0u;
+        uint4 *dst = (uint4 *)(T + b * TSLOTS);
+        dst[0] = make_uint4(f[0], f[1], f[2], f[3]);
+        dst[1] = make_uint4(f[4], f[5], f[6], f[7]);
+    }
+}
+
+// One partitioned I record against its bucket: every D record with an equal key emits a pair.
+__device__ __forceinline__ void probe_emit(u64 key, u64 e, u64 b, uint4 t0, uint4 t1, const u64 *__restrict__ Iid,
+                                           const u64 *__restrict__ koffs, int ninst, const u64 *__restrict__ bstart,
+                                           const u64 *__restrict__ Dk, const u64 *__restrict__ Did,
+                                           unsigned long long *__restrict__ npairs, u64 cap, u64 *__restrict__ out_d,
+                                           u64 *__restrict__ out_i, u64 *__restrict__ out_inst) {
+    const u32 fp = fingerprint(key);
+    const u32 f[TSLOTS] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < TSLOTS; ++k) any |= (f[k] == fp);
+    const bool full = t1.w != 0;
+    if (!any && !full) return;
+    const u64 lo = bstart[b], hi = bstart[b + 1];
+    for (u64 q = lo; q < hi; ++q) {
+        if (q < lo + TSLOTS && f[q - lo] != fp) continue;
+        if (Dk[q] != key) continue;
+        const unsigned long long at = atomicAdd(npairs, 1ULL);
+        if (at < cap) {
+            out_d[at] = Did[q];
+            out_i[at] = Iid[e];
+            out_inst[at] = inst_of_key(koffs, ninst, key);
+        }
+    }
+}
+
+constexpr int PROBE_BATCH = 4;     // records per thread whose loads are issued before any compare
+
+// Probes the slots [pbase[p], pbase[p] + pfill[p]) of every partition p (a record layout with
+// gaps when the scatter used capacities). Does nothing if the scatter overflowed (*ovf).
 __global__ void __launch_bounds__(PB) k_part_probe(const u64 *__restrict__ Ik, const u64 *__restrict__ Iid,
-                                                   const SideDesc *__restrict__ descs, const u64 *__restrict__ koffs,
-                                                   int ninst, u64 nI,
+                                                   const u64 *__restrict__ pbase, const u32 *__restrict__ pfill, int P,
+                                                   const u32 *__restrict__ ovf, const u64 *__restrict__ koffs,
+                                                   int ninst, u64 NB, const u32 *__restrict__ T,
                                                    const u64 *__restrict__ bstart, const u64 *__restrict__ Dk,
                                                    const u64 *__restrict__ Did, unsigned long long *__restrict__ npairs,
                                                    u64 cap, u64 *__restrict__ out_d, u64 *__restrict__ out_i,
                                                    u64 *__restrict__ out_inst) {
-    const u64 ntiles = (nI + PT - 1) / PT;
+    __shared__ u64 s_base[257];
+    __shared__ u64 s_end[256];
+    if (*ovf) return;
+    for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
+    for (int i = threadIdx.x; i < P; i += PB) s_end[i] = pbase[i] + pfill[i];
+    __syncthreads();
+    const u64 nslots = s_base[P];
+    const u64 ntiles = (nslots + PT - 1) / PT;
     // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous slice of tiles
     const u32 G = gridDim.x;
     const u32 xcd = blockIdx.x % NXCD;
     const u32 nb_x = G / NXCD + (xcd < G % NXCD ? 1 : 0);     // blocks on this XCD
-    const u32 k = blockIdx.x / NXCD;                          // rank of this block on its XCD
+    const u32 kb = blockIdx.x / NXCD;                         // rank of this block on its XCD
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
-    for (u64 t = t_lo + k; t < t_hi; t += nb_x) {
+    for (u64 t = t_lo + kb; t < t_hi; t += nb_x) {
         const u64 e0 = t * PT;
-#pragma unroll 4
-        for (int it = 0; it < PI; ++it) {
-            const u64 e = e0 + (u64)it * PB + threadIdx.x;
-            if (e >= nI) break;
-            const u64 key = Ik[e];
-            const u32 inst = ninst == 1 ? 0u : inst_of_key(koffs, ninst, key);
-            const u64 b = bucket_of(descs[inst], key);
-            const u64 lo = bstart[b], hi = bstart[b + 1];
-            for (u64 q = lo; q < hi; ++q) {
-                if (Dk[q] == key) {
-                    const unsigned long long at = atomicAdd(npairs, 1ULL);
-                    if (at < cap) {
-                        out_d[at] = Did[q];
-                        out_i[at] = Iid[e];
-                        out_inst[at] = inst;
-                    }
-                }
+        // partition of the tile's first slot
+        int p = 0;
+        {
+            int a = 0, b = P - 1;
+            while (a < b) {
+                const int mid = (a + b + 1) >> 1;
+                if (s_base[mid] <= e0) a = mid; else b = mid - 1;
+            }
+            p = a;
+        }
+        for (int it0 = 0; it0 < PI; it0 += PROBE_BATCH) {
+            u64 key[PROBE_BATCH], b[PROBE_BATCH];
+            bool ok[PROBE_BATCH];
+            uint4 t0[PROBE_BATCH], t1[PROBE_BATCH];
+#pragma unroll
+            for (int q = 0; q < PROBE_BATCH; ++q) {
+                const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
+                int pp = p;
+                while (pp + 1 < P && e >= s_base[pp + 1]) ++pp;
+                ok[q] = e < nslots && e < s_end[pp];
+                key[q] = ok[q] ? Ik[e] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < PROBE_BATCH; ++q) {
+                b[q] = bucket_of(key[q], NB);
+                const uint4 *src = (const uint4 *)(T + b[q] * TSLOTS);
+                t0[q] = src[0];
+                t1[q] = src[1];
+            }
+#pragma unroll
+            for (int q = 0; q < PROBE_BATCH; ++q) {
+                const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
+                if (ok[q])
+                    probe_emit(key[q], e, b[q], t0[q], t1[q], Iid, koffs, ninst, bstart, Dk, Did, npairs, cap, out_d,
+                               out_i, out_inst);
             }
         }
     }
@@ -293,47 +436,64 @@ void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v) {
     CF_LAUNCH(ctx, "k_fill_u32", k_fill_u32, dim3(grid_for(n, PB, 65536)), dim3(PB), 0, a, n, v);
 }
 
+// Leaf source of [lo, lo + n) with its tile owners computed into tp (2 * ceil(n / LT) entries).
+static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                        int depth, DBuf<u64> &tp) {
+    const u64 ntiles = (n + LT - 1) / LT;
+    tp = DBuf<u64>(ctx, 2 * ntiles);
+    CF_LAUNCH(ctx, "k_tile_parents", k_tile_parents, dim3(grid_for(2 * ntiles, PB, 65536)), dim3(PB), 0, off, nn, lo, n,
+              tp.p);
+    return LeafSrc{descs, nodes, nn, off, lo, n, depth, tp.p};
+}
+
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, u32 *bcount) {
+                       int depth, u64 NB, u32 *bcount) {
     if (!n) return;
-    LeafSrc L{descs, nodes, nn, off, lo, n, depth};
-    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, PB)), dim3(PB), 0, L, bcount);
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, bcount);
 }
 
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did) {
+                         u64 n, int depth, u64 NB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did) {
     if (!n) return;
-    LeafSrc L{descs, nodes, nn, off, lo, n, depth};
-    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, PB)), dim3(PB), 0, L, bstart, bcur, Dk, Did);
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, bstart, bcur, Dk, Did);
+}
+
+void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, u32 *T) {
+    CF_LAUNCH(ctx, "k_table_build", k_table_build, dim3(grid_for(NB, PB, 65536)), dim3(PB), 0, bstart, Dk, NB, T);
 }
 
 void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, int pshift, int P, u32 *pcount) {
+                int depth, u64 NB, int pshift, int P, u32 *pcount) {
     if (!n) return;
-    LeafSrc L{descs, nodes, nn, off, lo, n, depth};
-    CF_LAUNCH(ctx, "k_part_count", k_part_count, dim3(grid_for(n, PT)), dim3(PB), 0, L, pshift, P, pcount);
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    CF_LAUNCH(ctx, "k_part_count", k_part_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, pshift, P, pcount);
 }
 
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, int pshift, int P, const u64 *pstart, u32 *pcur, u64 *Ik, u64 *Iid) {
+                  int depth, u64 NB, int pshift, int P, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *Ik, u64 *Iid) {
     if (!n) return;
-    LeafSrc L{descs, nodes, nn, off, lo, n, depth};
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(grid_for(n, ST)), dim3(PB), 0, L, pshift, P, pstart, pcur,
-              Ik, Iid);
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, pshift, P, pbase, pcur,
+              ovf, Ik, Iid);
 }
 
-void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const SideDesc *descs, const u64 *koffs, int ninst, u64 nI,
-                const u64 *bstart,
-                const u64 *Dk, const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst) {
-    if (!nI) return;
-    // enough blocks to fill the chip several times over, a multiple of the XCD count
-    const u64 ntiles = (nI + PT - 1) / PT;
+void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const u64 *pbase, const u32 *pfill, int P, u64 nslots,
+                const u32 *ovf, const u64 *koffs, int ninst, u64 NB, const u32 *T, const u64 *bstart, const u64 *Dk,
+                const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst) {
+    if (!nslots) return;
     // the tile order gives each XCD a slice of tiles, so every XCD needs at least one block:
     // the grid is a non-zero multiple of the XCD count
+    const u64 ntiles = (nslots + PT - 1) / PT;
     unsigned g = (unsigned)(ntiles < 8192 ? ntiles : 8192);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, Ik, Iid, descs, koffs, ninst, nI, bstart, Dk, Did, npairs,
-              cap, od, oi, oinst);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, Ik, Iid, pbase, pfill, P, ovf, koffs, ninst, NB, T,
+              bstart, Dk, Did, npairs, cap, od, oi, oinst);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out) {

@@ -12,6 +12,7 @@
 // sorted and joined against the sorted D side with the searchsorted semantics of
 // gpu/mitm_gpu2.py:107-119.
 #include <algorithm>
+#include <cmath>
 #include <map>
 
 #include "cf_internal.h"
@@ -146,6 +147,7 @@ struct PhaseClock {
     }
     double ms(size_t a, size_t b) {
         float t = 0;
+        CF_HIP(hipEventSynchronize(ev[b]));
         CF_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
         return t;
     }
@@ -154,33 +156,25 @@ struct PhaseClock {
     }
 };
 
-// Per-instance fine buckets: instance j gets nb_j = a power of two near nD_j / 2 buckets at
-// bbase_j, addressed by a multiplicative hash of the key. P <= 128 coarse partitions are the top
-// bits of the global bucket index.
+// Fine buckets over the key space of one radius group: NB = a power of two >= nD / 3 (about
+// 1.5-3 D records per bucket), addressed by a multiplicative hash. The probe table holds 32 bytes
+// per bucket; P coarse partitions (top bits of the bucket index) are chosen so that one
+// partition's slice of the table is about 2 MB and stays in an XCD's 4 MB L2 while its records
+// are probed (16 <= P <= 256).
 struct Partitioning {
-    u64 NB = 0;                 // total buckets
+    u64 NB = 1;
     int P = 1, pshift = 0;
 };
 
-Partitioning assign_buckets(const cf_instance *insts, int n, const std::vector<u64> &nD, std::vector<SideDesc> &dd,
-                            std::vector<SideDesc> &id) {
+Partitioning assign_buckets(u64 nD) {
     Partitioning pt;
-    u64 base = 0;
-    for (int j = 0; j < n; ++j) {
-        u64 nb = 1;
-        while (nb < nD[j] / 2) nb <<= 1;
-        for (SideDesc *d : {&dd[j], &id[j]}) {
-            d->bS = 0;
-            d->bbase = (u32)base;
-            d->nb = (u32)nb;
-        }
-        base += nb;
-    }
-    CF_REQUIRE(base < (1ULL << 32), "too many key buckets for one solve: split the batch");
-    pt.NB = base;
     int lg = 0;
-    while ((1ULL << lg) < pt.NB) ++lg;
-    const int lgP = lg < 7 ? lg : 7;
+    while ((1ULL << lg) < nD / 3 || lg < 4) ++lg;
+    CF_REQUIRE(lg <= 40, "too many key buckets for one solve: split the batch");
+    pt.NB = 1ULL << lg;
+    int lgP = 4;
+    while (lgP < 8 && (pt.NB * 32 >> lgP) > (2u << 20)) ++lgP;
+    if (lgP > lg) lgP = lg;
     pt.P = 1 << lgP;
     pt.pshift = lg - lgP;
     return pt;
@@ -219,88 +213,109 @@ void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_
     PhaseClock clk(ctx);
     clk.mark();  // 0
     DBuf<u64> dcount(ctx, n), icount(ctx, n);
-    fill_u64(ctx, dcount.p, n, 0);
-    fill_u64(ctx, icount.p, n, 0);
-    // ---- D side: expand every radius group to its last level
-    std::vector<Frontier> dfs;
-    std::vector<int> dfs_r;
-    u64 nD = 0;
-    for (auto &g : groups) {
-        std::vector<Node> roots;
-        for (int j : g.second)
-            if (d_live[j]) roots.push_back(root_node(dd[j], SIDE_D, j));
-        Frontier f = expand_to_last(ctx, d_desc.p, roots, g.first);
-        if (!f.total) continue;
-        inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, dcount.p);
-        nD += f.total;
-        dfs.push_back(std::move(f));
-        dfs_r.push_back(g.first);
-    }
-    clk.mark();  // 1
-    // ---- D side: counting sort of the records into fine buckets sized per instance
-    std::vector<u64> nDj(n, 0);
-    d2h(ctx, nDj.data(), dcount.p, n);
-    const Partitioning pt = assign_buckets(insts, n, nDj, dd, id);
-    h2d(ctx, d_desc.p, dd.data(), n);
-    h2d(ctx, i_desc.p, id.data(), n);
-    DBuf<u32> bcount(ctx, pt.NB);
-    DBuf<u64> bstart(ctx, pt.NB + 1);
-    DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
-    fill_u32(ctx, bcount.p, pt.NB, 0);
-    for (size_t g = 0; g < dfs.size(); ++g)
-        part_bucket_count(ctx, d_desc.p, dfs[g].nodes.p, dfs[g].nn, dfs[g].off.p, 0, dfs[g].total, dfs_r[g] - 1,
-                          bcount.p);
-    scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
-    fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
-    for (size_t g = 0; g < dfs.size(); ++g)
-        part_bucket_scatter(ctx, d_desc.p, dfs[g].nodes.p, dfs[g].nn, dfs[g].off.p, 0, dfs[g].total, dfs_r[g] - 1,
-                            bstart.p, bcount.p, Dk.p, Did.p);
-    dfs.clear();
-    clk.mark();  // 2
-    clk.mark();  // 3
-    // ---- I side: expand, then per chunk of leaves: count per partition, scatter, probe
     const u64 chunk = max_chunk ? max_chunk : ((u64)1 << 27);
     u64 cap = 1 << 16;
     DBuf<u64> cnt_pairs(ctx, 1);
-    DBuf<u32> pcount(ctx, pt.P), pcur(ctx, pt.P);
-    DBuf<u64> pstart(ctx, pt.P + 1);
+    DBuf<u32> ovf(ctx, 1);
+    std::vector<u64> pbase_h;
+    double ms_d = 0, ms_i = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         out.pd = DBuf<u64>(ctx, cap);
         out.pi = DBuf<u64>(ctx, cap);
         out.pinst = DBuf<u64>(ctx, cap);
         fill_u64(ctx, cnt_pairs.p, 1, 0);
+        fill_u64(ctx, dcount.p, n, 0);
         fill_u64(ctx, icount.p, n, 0);
-        DBuf<u64> Ik, Iid;
+        ms_d = ms_i = 0;
+        // one radius group at a time: its D side is bucketed into its own probe table, then its I
+        // side is generated, partitioned and probed against it
         for (auto &g : groups) {
+            PhaseClock gc(ctx);
+            gc.mark();
             std::vector<Node> roots;
+            for (int j : g.second)
+                if (d_live[j]) roots.push_back(root_node(dd[j], SIDE_D, j));
+            Frontier fd = expand_to_last(ctx, d_desc.p, roots, g.first);
+            const u64 nD = fd.total;
+            if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
+            const Partitioning pt = assign_buckets(nD);
+            DBuf<u32> bcount(ctx, pt.NB);
+            DBuf<u64> bstart(ctx, pt.NB + 1);
+            DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
+            DBuf<u32> T(ctx, pt.NB * 8);
+            if (nD) {
+                fill_u32(ctx, bcount.p, pt.NB, 0);
+                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.NB, bcount.p);
+                scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
+                fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
+                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.NB, bstart.p,
+                                    bcount.p, Dk.p, Did.p);
+                part_table_build(ctx, bstart.p, Dk.p, pt.NB, T.p);
+            }
+            fd = Frontier();
+            gc.mark();
+            // ---- I side: expand, then per chunk of leaves: scatter by partition into slots sized
+            // from the expected share (hashing spreads keys evenly), probe; a chunk whose scatter
+            // overflowed a partition (heavily repeated keys) is redone with exact counts
+            roots.clear();
             for (int j : g.second)
                 if (i_live[j]) roots.push_back(root_node(id[j], SIDE_I, j));
             Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
-            if (!f.total) continue;
-            inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
-            if (!nD) continue;
-            for (u64 L = 0; L < f.total; L += chunk) {
-                const u64 m = std::min<u64>(chunk, f.total - L);
-                if (Ik.n < m) {
-                    Ik = DBuf<u64>(ctx, m);
-                    Iid = DBuf<u64>(ctx, m);
+            if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
+            if (f.total && nD) {
+                DBuf<u32> pcount(ctx, pt.P), pcur(ctx, pt.P);
+                DBuf<u64> pstart(ctx, pt.P + 1), pbase(ctx, pt.P + 1);
+                pbase_h.assign(pt.P + 1, 0);
+                DBuf<u64> Ik, Iid;
+                u64 cap_m = 0;
+                for (u64 L = 0; L < f.total; L += chunk) {
+                    const u64 m = std::min<u64>(chunk, f.total - L);
+                    const double mean = (double)m / pt.P;
+                    u64 capP = (u64)(mean * 1.05 + 16.0 * std::sqrt(mean) + 64.0);
+                    capP = (capP + 63) / 64 * 64;
+                    const u64 nslots = capP * pt.P;
+                    if (cap_m < std::max(nslots, m)) {
+                        cap_m = std::max(nslots, m);
+                        Ik = DBuf<u64>(ctx, cap_m);
+                        Iid = DBuf<u64>(ctx, cap_m);
+                    }
+                    for (int p = 0; p <= pt.P; ++p) pbase_h[p] = (u64)p * capP;
+                    h2d(ctx, pbase.p, pbase_h.data(), pt.P + 1);
+                    fill_u32(ctx, pcur.p, pt.P, 0);
+                    fill_u32(ctx, ovf.p, 1, 0);
+                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
+                                 pbase.p, pcur.p, ovf.p, Ik.p, Iid.p);
+                    u32 overflow = 0;
+                    d2h(ctx, &overflow, ovf.p, 1);
+                    if (!overflow) {
+                        part_probe(ctx, Ik.p, Iid.p, pbase.p, pcur.p, pt.P, nslots, ovf.p, koffs.p, n, pt.NB, T.p,
+                                   bstart.p, Dk.p, Did.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p,
+                                   out.pinst.p);
+                        continue;
+                    }
+                    // exact layout: count per partition, scan, scatter, probe
+                    fill_u32(ctx, pcount.p, pt.P, 0);
+                    part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
+                               pcount.p);
+                    scan_u32_to_u64_async(ctx, pcount.p, pstart.p, pt.P);
+                    fill_u32(ctx, pcur.p, pt.P, 0);
+                    fill_u32(ctx, ovf.p, 1, 0);
+                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
+                                 pstart.p, pcur.p, ovf.p, Ik.p, Iid.p);
+                    part_probe(ctx, Ik.p, Iid.p, pstart.p, pcur.p, pt.P, m, ovf.p, koffs.p, n, pt.NB, T.p, bstart.p,
+                               Dk.p, Did.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
-                fill_u32(ctx, pcount.p, pt.P, 0);
-                part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.pshift, pt.P, pcount.p);
-                scan_u32_to_u64_async(ctx, pcount.p, pstart.p, pt.P);
-                fill_u32(ctx, pcur.p, pt.P, 0);
-                part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.pshift, pt.P, pstart.p,
-                             pcur.p, Ik.p, Iid.p);
-                part_probe(ctx, Ik.p, Iid.p, i_desc.p, koffs.p, n, m, bstart.p, Dk.p, Did.p,
-                           (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
             }
+            gc.mark();
+            ms_d += gc.ms(0, 1);
+            ms_i += gc.ms(1, 2);
         }
         const u64 np = d2h_u64(ctx, cnt_pairs.p);
         out.npairs = np;
         if (np <= cap) break;
         cap = np;
     }
-    clk.mark();  // 4
+    clk.mark();  // 1
     out.nD.assign(n, 0);
     out.nI.assign(n, 0);
     d2h(ctx, out.nD.data(), dcount.p, n);
@@ -314,10 +329,10 @@ void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_
         }
         st->npairs = out.npairs;
         st->peak_bytes = g_peak_live;
-        st->ms_total = clk.ms(0, 4);
-        st->ms_gen_d = clk.ms(0, 1);
-        st->ms_sort_d = clk.ms(1, 2);
-        st->ms_gen_i = clk.ms(3, 4);
+        st->ms_total = clk.ms(0, 1);
+        st->ms_gen_d = ms_d;            // D expansion, bucketing and probe table
+        st->ms_sort_d = 0;
+        st->ms_gen_i = ms_i;            // I expansion, partition scatter and probe
         st->ms_sort_i = 0;
         st->ms_join = 0;
     }

@@ -275,3 +275,21 @@ def test_batch_equals_alone(workloads):
         lg = inst.meta.get("logged")
         if lg:
             assert (rb.nD, rb.nI, len(rb.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]), inst.tag
+
+
+@pytest.mark.parametrize("M", [64, 1024, 2 ** 20 + 7])
+def test_solve_heavy_collisions(M):
+    """Tiny moduli: keys repeat thousands of times, so partitions overflow their expected share and
+    the chunk is redone with exact counts; the pair multiset must still equal the oracle join."""
+    P = [p for p in range(3, 2000) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
+    B0, INS = P[:20], P[20:80]
+    inst = I.Instance(M=M, r=3, B0=B0, INS=INS, P0=prod(B0), Icap=prod(INS[-3:]) + 1, Dmin=1, split=None, hmax=3,
+                      tag=f"collide{M}")
+    say("collide", M)
+    res = pipeline.solve(inst)
+    Dk, Did, Ik, Iid = O.instance_records(inst)
+    o = np.argsort(Dk, kind="stable")
+    q = np.argsort(Ik, kind="stable")
+    ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
+    assert (res.nD, res.nI) == (len(Dk), len(Ik))
+    assert np.array_equal(res.pair_array(), O.pair_set(ed, ei))

@@ -83,35 +83,68 @@ def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0, mode: str =
     return SolveResult(pd, pi, int(st.nD), int(st.nI), st.as_dict())
 
 
+class Batch:
+    """Instances prepared on the device (cf_batch_create): descriptors and exact bound tables are
+    uploaded once; solve() is device work only and can be repeated."""
+
+    def __init__(self, insts, ctx: Optional[L.Context] = None):
+        self.ctx = ctx or L.default_context()
+        self.insts = list(insts)
+        keep: List = []
+        n = len(self.insts)
+        arr = (L.cf_instance * n)(*[make_cinstance(x, keep) for x in self.insts])
+        h = ctypes.c_void_p()
+        L.check(L.load().cf_batch_create(self.ctx.handle, arr, n, ctypes.byref(h)), "cf_batch_create")
+        self.handle = h
+
+    def solve(self, max_chunk: int = 0):
+        """Returns (list of SolveResult in input order, stats dict)."""
+        ctx = self.ctx
+        n = len(self.insts)
+        od, oi, oinst = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        npairs = L.c_u64()
+        counts = np.zeros(2 * n, np.uint64)
+        st = L.cf_stats()
+        L.check(L.load().cf_batch_solve(self.handle, max_chunk, ctypes.byref(od), ctypes.byref(oi),
+                                        ctypes.byref(oinst), ctypes.byref(npairs),
+                                        counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)),
+                "cf_batch_solve")
+        m = npairs.value
+        pd = np.empty(m, np.uint64)
+        pi = np.empty(m, np.uint64)
+        pj = np.empty(m, np.uint64)
+        if m:
+            for host, dev in ((pd, od), (pi, oi), (pj, oinst)):
+                ctx.d2h(host, dev.value)
+                ctx.free(dev.value)
+        order = np.argsort(pj, kind="stable")
+        pd, pi, pj = pd[order], pi[order], pj[order]
+        bounds = np.searchsorted(pj, np.arange(n + 1, dtype=np.uint64))
+        out = []
+        for j in range(n):
+            a, b = int(bounds[j]), int(bounds[j + 1])
+            out.append(SolveResult(pd[a:b].copy(), pi[a:b].copy(), int(counts[2 * j]), int(counts[2 * j + 1])))
+        return out, st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            L.load().cf_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solve_batch(insts, ctx: Optional[L.Context] = None):
     """Solve many instances together. Returns (list of SolveResult in input order, batch stats)."""
-    ctx = ctx or L.default_context()
-    keep: List = []
-    n = len(insts)
-    arr = (L.cf_instance * n)(*[make_cinstance(x, keep) for x in insts])
-    od, oi, oinst = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
-    npairs = L.c_u64()
-    counts = np.zeros(2 * n, np.uint64)
-    st = L.cf_stats()
-    L.check(L.load().cf_solve_batch(ctx.handle, arr, n, ctypes.byref(od), ctypes.byref(oi), ctypes.byref(oinst),
-                                    ctypes.byref(npairs), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
-                                    ctypes.byref(st)), "cf_solve_batch")
-    m = npairs.value
-    pd = np.empty(m, np.uint64)
-    pi = np.empty(m, np.uint64)
-    pj = np.empty(m, np.uint64)
-    if m:
-        for host, dev in ((pd, od), (pi, oi), (pj, oinst)):
-            ctx.d2h(host, dev.value)
-            ctx.free(dev.value)
-    order = np.argsort(pj, kind="stable")
-    pd, pi, pj = pd[order], pi[order], pj[order]
-    bounds = np.searchsorted(pj, np.arange(n + 1, dtype=np.uint64))
-    out = []
-    for j in range(n):
-        a, b = int(bounds[j]), int(bounds[j + 1])
-        out.append(SolveResult(pd[a:b].copy(), pi[a:b].copy(), int(counts[2 * j]), int(counts[2 * j + 1])))
-    return out, st.as_dict()
+    b = Batch(insts, ctx)
+    try:
+        return b.solve()
+    finally:
+        b.close()
 
 
 def decoders(inst):

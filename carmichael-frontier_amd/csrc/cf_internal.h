@@ -154,11 +154,10 @@ struct SideDesc {
     u64 xm0, x20;       // initial residue state (the side multiplier)
 };
 
-// Host-owned copy of one side's device tables.
+// Host-owned copy of one side's device tables (one allocation holding all four arrays).
 struct SideTables {
     SideDesc desc{};
-    DBuf<u64> p, pR, binom;
-    DBuf<U128> bound;
+    DBuf<u64> p;
 };
 
 struct SideSpec {  // host description of one side
@@ -174,6 +173,8 @@ struct SideSpec {  // host description of one side
 };
 
 void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t);
+void side_tables_host(const SideSpec &s, SideDesc &d, std::vector<u64> &buf, size_t off[4]);
+void side_tables_bind(SideDesc &d, const u64 *dev, const size_t off[4]);
 
 // A filtered enumeration plan: the tree of one side expanded to its last level.
 struct cf_plan {
@@ -219,17 +220,18 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, u64 NB, u32 *bcount);
+                       int depth, int lgNB, u32 *bcount);
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, u64 NB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
-void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, u32 *T);
+                         u64 n, int depth, int lgNB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
+void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, unsigned short *T);
 void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, u64 NB, int pshift, int P, u32 *pcount);
+                int depth, int lgP, u32 *pcount);
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, u64 NB, int pshift, int P, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *Ik, u64 *Iid);
-void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const u64 *pbase, const u32 *pfill, int P, u64 nslots,
-                const u32 *ovf, const u64 *koffs, int ninst, u64 NB, const u32 *T, const u64 *bstart, const u64 *Dk,
-                const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
+                  int depth, int lgNB, int lgP, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *rec);
+void part_probe(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, int depth,
+                int lgNB, int lgP, const u64 *rec, const u64 *pbase, const u32 *pfill, u64 nslots, const u32 *ovf,
+                const u64 *koffs, int ninst, const unsigned short *T, const u64 *bstart, const u64 *Dk, const u64 *Did,
+                unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
 
 // joins

@@ -232,13 +232,23 @@ void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, 
 // ---- host: per-side tables ---------------------------------------------------------------
 typedef unsigned __int128 u128;
 
-void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t) {
+static inline u128 div128(u128 a, u128 b) {
+    // 64-bit division when both fit (the common case), else the 128-bit routine
+    if ((a >> 64) == 0 && (b >> 64) == 0) return (u128)((u64)a / (u64)b);
+    return a / b;
+}
+
+// Host tables of one side, appended to buf (u64 words, U128 entries 16-byte aligned):
+//   p[n] primes, pR[n] = p * 2^64 mod m, bound[r*n], binom[(n+1)*(r+1)];
+// off[0..3] receive the word offsets of the four arrays. Fills every field of d except the
+// four device pointers.
+void side_tables_host(const SideSpec &s, SideDesc &d, std::vector<u64> &buf, size_t off[4]) {
     const int n = (int)s.primes.size();
     const int r = s.r;
     CF_REQUIRE(n >= 0 && n <= 1024, "pool size must be <= 1024");
     CF_REQUIRE(r >= 1 && r <= 16, "radius must be in [1, 16]");
     CF_REQUIRE(s.M >= 1, "modulus must be >= 1");
-    SideDesc d{};
+    d = SideDesc{};
     d.mc = cf_modctx(s.M);
     d.n = n;
     d.r = r;
@@ -251,55 +261,73 @@ void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t) {
     const u64 mult = s.mult % s.M;
     d.xm0 = d.mc.m == 1 ? 0 : mult % d.mc.m;
     d.x20 = mult;
-
-    std::vector<u64> pR(n);
-    for (int i = 0; i < n; ++i) pR[i] = d.mc.m == 1 ? 0 : cf_montmul_factor(s.primes[i], d.mc);
-    // bound table, exact: I: floor((Icap-1)/W), W = a_x..a_{x+t}; D: ceil(Dmin/W), W = b_i..b_{i-t}
-    std::vector<U128> bound((size_t)r * (n ? n : 1), U128{0, 0});
+    const int nn = n ? n : 1;
+    auto take = [&](size_t words) {
+        size_t at = (buf.size() + 1) & ~(size_t)1;   // 16-byte alignment for every array
+        buf.resize(at + words, 0);
+        return at;
+    };
+    off[0] = take(nn);
+    off[1] = take(nn);
+    off[2] = take((size_t)r * nn * 2);
+    off[3] = take((size_t)(n + 1) * (r + 1));
+    for (int i = 0; i < n; ++i) {
+        buf[off[0] + i] = s.primes[i];
+        buf[off[1] + i] = d.mc.m == 1 ? 0 : cf_montmul_factor(s.primes[i], d.mc);
+    }
+    // bound table, exact: I: floor((Icap - 1) / W), W = a_x .. a_{x+t}; D: ceil(Dmin / W),
+    // W = b_i .. b_{i-t}; the window grows one prime per t. Entries whose window leaves the pool
+    // are never consulted (the candidate ranges exclude them) and are set to reject.
     const u128 ALL = ~(u128)0;
-    for (int tt = 0; tt < r; ++tt) {
-        for (int x = 0; x < n; ++x) {
-            u128 w = 1;
-            bool over = false;
-            bool in_range = (s.side == SIDE_I) ? (x + tt < n) : (x - tt >= 0);
-            if (!in_range) {
-                // never consulted (the candidate range excludes it); make it reject
-                bound[(size_t)tt * n + x] = (s.side == SIDE_I) ? U128{0, 0} : U128{~0ULL, ~0ULL};
-                continue;
-            }
-            for (int j = 0; j <= tt; ++j) {
-                u64 q = s.primes[s.side == SIDE_I ? x + j : x - j];
-                if (w > ALL / q) { over = true; break; }
-                w *= q;
-            }
+    const u128 top = s.cap ? s.cap - 1 : 0;
+    for (int x = 0; x < n; ++x) {
+        u128 w = 1;
+        bool over = false;
+        for (int tt = 0; tt < r; ++tt) {
+            const int y = s.side == SIDE_I ? x + tt : x - tt;
             u128 v;
-            if (s.side == SIDE_I) {
-                u128 top = s.cap ? s.cap - 1 : 0;
-                v = over ? 0 : top / w;
+            if (y < 0 || y >= n) {
+                v = s.side == SIDE_I ? 0 : ALL;
             } else {
-                v = over ? (s.cap ? 1 : 0) : (s.cap + w - 1) / w;
-                if (!over && s.cap > ALL - (w - 1)) v = s.cap / w + (s.cap % w != 0);
+                const u64 q = s.primes[y];
+                if (!over && w > ALL / q) over = true;
+                if (!over) w *= q;
+                if (s.side == SIDE_I) {
+                    v = over ? 0 : div128(top, w);
+                } else {
+                    if (over) {
+                        v = s.cap ? 1 : 0;
+                    } else {
+                        const u128 qq = div128(s.cap, w);
+                        v = qq + (qq * w != s.cap);
+                    }
+                }
             }
-            bound[(size_t)tt * n + x] = U128{(u64)v, (u64)(v >> 64)};
+            const size_t e = off[2] + 2 * ((size_t)tt * nn + x);
+            buf[e] = (u64)v;
+            buf[e + 1] = (u64)(v >> 64);
         }
     }
-    std::vector<u64> binom((size_t)(n + 1) * (r + 1), 0);
     for (int a = 0; a <= n; ++a) {
-        binom[(size_t)a * (r + 1)] = 1;
+        buf[off[3] + (size_t)a * (r + 1)] = 1;
         for (int b = 1; b <= r && a > 0; ++b)
-            binom[(size_t)a * (r + 1) + b] = binom[(size_t)(a - 1) * (r + 1) + b - 1] + binom[(size_t)(a - 1) * (r + 1) + b];
+            buf[off[3] + (size_t)a * (r + 1) + b] =
+                buf[off[3] + (size_t)(a - 1) * (r + 1) + b - 1] + buf[off[3] + (size_t)(a - 1) * (r + 1) + b];
     }
-    t.p = DBuf<u64>(ctx, n ? n : 1);
-    t.pR = DBuf<u64>(ctx, n ? n : 1);
-    t.bound = DBuf<U128>(ctx, bound.size());
-    t.binom = DBuf<u64>(ctx, binom.size());
-    h2d(ctx, t.p.p, s.primes.data(), n);
-    h2d(ctx, t.pR.p, pR.data(), n);
-    h2d(ctx, t.bound.p, bound.data(), bound.size());
-    h2d(ctx, t.binom.p, binom.data(), binom.size());
-    d.p = t.p.p;
-    d.pR = t.pR.p;
-    d.bound = t.bound.p;
-    d.binom = t.binom.p;
-    t.desc = d;
+}
+
+void side_tables_bind(SideDesc &d, const u64 *dev, const size_t off[4]) {
+    d.p = dev + off[0];
+    d.pR = dev + off[1];
+    d.bound = (const U128 *)(dev + off[2]);
+    d.binom = dev + off[3];
+}
+
+void build_side_tables(cf_ctx *ctx, const SideSpec &s, SideTables &t) {
+    std::vector<u64> buf;
+    size_t off[4];
+    side_tables_host(s, t.desc, buf, off);
+    t.p = DBuf<u64>(ctx, buf.size());
+    h2d(ctx, t.p.p, buf.data(), buf.size());
+    side_tables_bind(t.desc, t.p.p, off);
 }

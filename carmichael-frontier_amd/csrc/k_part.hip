@@ -1,25 +1,26 @@
 // k_part.hip -- radix-partitioned residue join on gfx950.
 //
 // Keys of one solve live in one key space (instance j owns [koff_j, koff_j + M_j), so keys of
-// different instances never collide). The space is cut into NB (a power of two, about a third of
-// the D records) fine buckets by multiplicative hashing  b = mulhi(key * 0x9E3779B97F4A7C15, NB).
-// Hashing, not key order, because keys are far from uniform: for small r the products prod(D) stay
-// below M and are their own residues, all in a sliver of [0, M), and a batch mixes moduli from
-// 1e11 to 1e17. The top bits of b give P <= 128 coarse partitions.
+// different instances never collide). The space is cut into NB = 2^lgNB (about a third of the D
+// records) fine buckets by multiplicative hashing (the top bits of key * 0x9E3779B97F4A7C15).
+// Hashing, not key order, because keys are far from uniform: for small r the products prod(D)
+// stay below M and are their own residues, all in a sliver of [0, M), and a batch mixes moduli
+// from 1e11 to 1e17. The top lgP bits of the hash give P <= 256 coarse partitions.
 //
-// Probe table: per bucket 8 u32 fingerprints (32 bytes, one memory sector) of its first 8 D
-// records; a probe reads its bucket's 32 bytes once and touches the D records only on a
-// fingerprint hit (exact key check) or when the bucket holds more than 8 records.
+// Probe table: per bucket 8 fingerprint slots of 16 bits (16 bytes) for its first 8 D records; a
+// probe reads its bucket's 16 bytes and touches the D records only on a fingerprint hit (then the
+// exact key is recomputed and checked) or when the bucket holds more than 8 records.
 //
 //  D side: records are counted per fine bucket and scattered into bucket order (a counting sort
 //          on b: no full key sort is needed, a probe scans its bucket, 2-4 records on average).
-//  I side: leaves are counted per partition (LDS histogram), then scattered by partition: each
-//          block ranks its 2048 leaves by partition in LDS and writes one contiguous run per
-//          partition (16 leaves on average with 128 partitions: a 128-byte run per array).
+//  I side: leaves are scattered by partition as 8-byte records (bucket offset, fingerprint, leaf
+//          index): each block ranks its 2048 leaves by partition in LDS and writes one contiguous
+//          run per partition into slots sized from the expected share.
 //  Probe:  the partitioned I records are read once, in partition order, with an XCD-aware tile
 //          order (each of the 8 XCDs walks its own contiguous slice of tiles) so the D buckets
 //          of the partitions in flight stay in that XCD's L2. Every equal key emits a pair.
-// Algorithmic HBM traffic per I record: 16 B written by the scatter + 16 B read by the probe.
+// HBM traffic per I record: 8 B written by the scatter + 8 B read by the probe (plus the 16-byte
+// table bucket it looks up).
 // Join semantics: every equal-key (D, I) pair (gpu/mitm_gpu2.py:107-119).
 #include "tree_dev.h"
 
@@ -42,9 +43,19 @@ struct LeafSrc {
     const u64 *tp;    // per tile of LT leaves: owners of its first and last leaf (2 entries per tile)
 };
 
-__device__ __forceinline__ u64 bucket_of(u64 key, u64 NB) { return cf_mulhi(key * 0x9E3779B97F4A7C15ULL, NB); }
-// nonzero 32-bit fingerprint from an independent multiplier; 0 marks an empty slot
-__device__ __forceinline__ u32 fingerprint(u64 key) { return (u32)((key * 0xD6E8FEB86659FD93ULL) >> 32) | 1u; }
+// Hashing of a key: h = key * 0x9E3779B97F4A7C15 (odd multiplier, so a bijection on u64).
+//   fine bucket    b = top lgNB bits of h          (lgNB <= 28: bits 36..63)
+//   partition      p = top lgP bits of h           (the first lgP bits of b)
+//   fingerprint    f = bits 20..31 of h            (independent of b)
+// An I record is 8 bytes: low word (b mod 2^(lgNB - lgP)) << 12 | f, high word the leaf's index
+// in its chunk; the exact key and id of a leaf are recomputed from the resident node tree only
+// when its fingerprint hits the D table. Table slots hold 0x8000 | f (0 = empty).
+struct KeyHash {
+    int lgNB, lgP;
+};
+__device__ __forceinline__ u64 khash(u64 key) { return key * 0x9E3779B97F4A7C15ULL; }
+__device__ __forceinline__ u64 hbucket(u64 h, int lgNB) { return h >> (64 - lgNB); }
+__device__ __forceinline__ u32 hfp(u64 h) { return (u32)(h >> 20) & 0xFFFu; }
 constexpr int TSLOTS = 8;
 
 // Tile set-up: the nodes owning leaves [j0, j1) are p0 .. p0 + np - 1 (every last-level node owns at
@@ -85,6 +96,17 @@ struct LeafWalk {
         nd = L.nodes[p0 + q];
         s = &L.descs[nd.meta >> 8];
     }
+    // key of leaf j (j >= every earlier j of this walk)
+    __device__ __forceinline__ void eval_key(u64 j, u64 &key) {
+        while (j >= hi_off) {
+            ++q;
+            load();
+        }
+        const int x = child_index(*s, nd, (int)(j - lo_off));
+        const u64 xm = cf_mulmod_m(nd.xm, s->pR[x], s->mc);
+        const u64 x2 = nd.x2 * s->p[x];
+        key = cf_key(xm, x2, s->mc) + s->koff;
+    }
     // key, id and instance of leaf j (j >= every earlier j of this walk)
     __device__ __forceinline__ void eval(u64 j, u64 &key, u64 &id, u32 &inst) {
         while (j >= hi_off) {
@@ -114,7 +136,7 @@ __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off
 }
 
 // ---- D side: counting sort by fine bucket ---------------------------------------------------
-__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, u64 NB, u32 *__restrict__ bcount) {
+__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, int lgNB, u32 *__restrict__ bcount) {
     __shared__ u64 s_off[LT + 1];
     const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -130,14 +152,14 @@ __global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, u64 NB, u32 *__r
                 u64 key, id;
                 u32 inst;
                 w.eval(jt + k, key, id, inst);
-                atomicAdd(&bcount[bucket_of(key, NB)], 1u);
+                atomicAdd(&bcount[hbucket(khash(key), lgNB)], 1u);
             }
         }
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, u64 NB, const u64 *__restrict__ bstart,
+__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, int lgNB, const u64 *__restrict__ bstart,
                                                        u32 *__restrict__ bcur, u64 *__restrict__ Dk,
                                                        u64 *__restrict__ Did) {
     __shared__ u64 s_off[LT + 1];
@@ -155,7 +177,7 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, u64 NB, const 
                 u64 key, id;
                 u32 inst;
                 w.eval(jt + k, key, id, inst);
-                const u64 b = bucket_of(key, NB);
+                const u64 b = hbucket(khash(key), lgNB);
                 const u64 at = bstart[b] + atomicAdd(&bcur[b], 1u);
                 Dk[at] = key;
                 Did[at] = id;
@@ -165,10 +187,11 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, u64 NB, const 
     }
 }
 
-// ---- I side: partition by the top bits of the fine bucket -----------------------------------
-__global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, u64 NB, int pshift, int P, u32 *__restrict__ pcount) {
+// ---- I side: partition by the top bits of the key hash --------------------------------------
+__global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, int lgP, u32 *__restrict__ pcount) {
     __shared__ u64 s_off[LT + 1];
     __shared__ u32 h[256];
+    const int P = 1 << lgP;
     const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
@@ -184,7 +207,7 @@ __global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, u64 NB, int pshift
                 u64 key, id;
                 u32 inst;
                 w.eval(jt + k, key, id, inst);
-                atomicAdd(&h[(u32)(bucket_of(key, NB) >> pshift)], 1u);
+                atomicAdd(&h[(u32)(khash(key) >> (64 - lgP))], 1u);
             }
         }
         __syncthreads();
@@ -193,21 +216,22 @@ __global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, u64 NB, int pshift
     }
 }
 
-// Partition p owns output slots [pbase[p], pbase[p + 1]); pcur[p] counts the slots taken. With a
-// capacity layout (exact == 0) a partition that would run past its end sets *ovf and writes
-// nothing for that tile; the caller then redoes the chunk with exact counts (exact == 1).
-__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshift, int P, const u64 *__restrict__ pbase,
+// Partition p owns record slots [pbase[p], pbase[p + 1]); pcur[p] counts the slots taken. With a
+// capacity layout a partition that would run past its end sets *ovf and writes nothing for that
+// tile; the caller then redoes the chunk with exact counts as the layout.
+__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ pbase,
                                                      u32 *__restrict__ pcur, u32 *__restrict__ ovf,
-                                                     u64 *__restrict__ Ik, u64 *__restrict__ Iid) {
+                                                     u64 *__restrict__ rec) {
     __shared__ u64 s_off[LT + 1];
     __shared__ u32 h[256];
     __shared__ u32 loc[256];
     __shared__ u64 gbase[256];
     __shared__ u32 s_w[PB / 64];
-    __shared__ u64 s_k[LT];
-    __shared__ u64 s_i[LT];
+    __shared__ u64 s_r[LT];
     __shared__ u8_t s_p[LT];
     constexpr u64 NONE = ~0ULL;
+    const int P = 1 << kh.lgP;
+    const u64 bomask = (1ULL << (kh.lgNB - kh.lgP)) - 1;
     const size_t ntiles = (L.n + LT - 1) / LT;
     for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
@@ -217,7 +241,7 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshi
         size_t p0;
         u32 np;
         tile_setup(L, t, s_off, p0, np);
-        u64 key[LPT], id[LPT];
+        u64 r[LPT];
         u32 part[LPT], rank[LPT];
         const u64 jt = j0 + (u64)threadIdx.x * LPT;
         if (jt < j1) {
@@ -225,9 +249,13 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshi
 #pragma unroll
             for (int k = 0; k < LPT; ++k) {
                 if (jt + k < j1) {
+                    u64 key, id;
                     u32 inst;
-                    w.eval(jt + k, key[k], id[k], inst);
-                    part[k] = (u32)(bucket_of(key[k], NB) >> pshift);
+                    w.eval_key(jt + k, key);
+                    const u64 hh = khash(key);
+                    part[k] = (u32)(hh >> (64 - kh.lgP));
+                    const u32 lo = (u32)((hbucket(hh, kh.lgNB) & bomask) << 12) | hfp(hh);
+                    r[k] = ((u64)(jt + k - L.lo) << 32) | lo;
                     rank[k] = atomicAdd(&h[part[k]], 1u);
                 }
             }
@@ -262,8 +290,7 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshi
             for (int k = 0; k < LPT; ++k) {
                 if (jt + k < j1) {
                     const u32 at = loc[part[k]] + rank[k];
-                    s_k[at] = key[k];
-                    s_i[at] = id[k];
+                    s_r[at] = r[k];
                     s_p[at] = (u8_t)part[k];
                 }
             }
@@ -272,11 +299,22 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, u64 NB, int pshi
         for (int q = threadIdx.x; q < cnt; q += PB) {
             const u32 pi = s_p[q];
             if (gbase[pi] == NONE) continue;
-            const u64 at = gbase[pi] + (u64)(q - (int)loc[pi]);
-            Ik[at] = s_k[q];
-            Iid[at] = s_i[q];
+            rec[gbase[pi] + (u64)(q - (int)loc[pi])] = s_r[q];
         }
         __syncthreads();
+    }
+}
+
+// ---- probe table: 8 fingerprint slots of 16 bits per bucket ---------------------------------
+__global__ void __launch_bounds__(PB) k_table_build(const u64 *__restrict__ bstart, const u64 *__restrict__ Dk, u64 NB,
+                                                   unsigned short *__restrict__ T) {
+    for (u64 b = (u64)blockIdx.x * PB + threadIdx.x; b < NB; b += (u64)gridDim.x * PB) {
+        const u64 lo = bstart[b], cnt = bstart[b + 1] - lo;
+        u32 f[TSLOTS];
+#pragma unroll
+        for (int k = 0; k < TSLOTS; ++k) f[k] = (u64)k < cnt ? (0x8000u | hfp(khash(Dk[lo + k]))) : 0u;
+        *(uint4 *)(T + b * TSLOTS) = make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16),
+                                                f[6] | (f[7] << 16));
     }
 }
 
@@ -290,63 +328,41 @@ __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ni
     return (u32)a;
 }
 
-__global__ void __launch_bounds__(PB) k_table_build(const u64 *__restrict__ bstart, const u64 *__restrict__ Dk, u64 NB,
-                                                   u32 *__restrict__ T) {
-    for (u64 b = (u64)blockIdx.x * PB + threadIdx.x; b < NB; b += (u64)gridDim.x * PB) {
-        const u64 lo = bstart[b], cnt = bstart[b + 1] - lo;
-        u32 f[TSLOTS];
-#pragma unroll
-        for (int k = 0; k < TSLOTS; ++k) f[k] = (u64)k < cnt ? fingerprint(Dk[lo + k]) : 0u;
-        uint4 *dst = (uint4 *)(T + b * TSLOTS);
-        dst[0] = make_uint4(f[0], f[1], f[2], f[3]);
-        dst[1] = make_uint4(f[4], f[5], f[6], f[7]);
-    }
+// Exact key and id of leaf lo + leaf (rare: only on a fingerprint hit).
+constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
+
+__device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key, u64 &id) {
+    const u64 j = L.lo + leaf;
+    const size_t p = find_parent(L.off, 0, L.nn - 1, j);
+    const Node nd = L.nodes[p];
+    const SideDesc &s = L.descs[nd.meta >> 8];
+    const int x = child_index(s, nd, (int)(j - L.off[p]));
+    const u64 xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
+    const u64 x2 = nd.x2 * s.p[x];
+    key = cf_key(xm, x2, s.mc) + s.koff;
+    id = id_update(s, nd.id, x, L.depth);
 }
 
-// One partitioned I record against its bucket: every D record with an equal key emits a pair.
-__device__ __forceinline__ void probe_emit(u64 key, u64 e, u64 b, uint4 t0, uint4 t1, const u64 *__restrict__ Iid,
-                                           const u64 *__restrict__ koffs, int ninst, const u64 *__restrict__ bstart,
-                                           const u64 *__restrict__ Dk, const u64 *__restrict__ Did,
-                                           unsigned long long *__restrict__ npairs, u64 cap, u64 *__restrict__ out_d,
-                                           u64 *__restrict__ out_i, u64 *__restrict__ out_inst) {
-    const u32 fp = fingerprint(key);
-    const u32 f[TSLOTS] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < TSLOTS; ++k) any |= (f[k] == fp);
-    const bool full = t1.w != 0;
-    if (!any && !full) return;
-    const u64 lo = bstart[b], hi = bstart[b + 1];
-    for (u64 q = lo; q < hi; ++q) {
-        if (q < lo + TSLOTS && f[q - lo] != fp) continue;
-        if (Dk[q] != key) continue;
-        const unsigned long long at = atomicAdd(npairs, 1ULL);
-        if (at < cap) {
-            out_d[at] = Did[q];
-            out_i[at] = Iid[e];
-            out_inst[at] = inst_of_key(koffs, ninst, key);
-        }
-    }
-}
-
-constexpr int PROBE_BATCH = 4;     // records per thread whose loads are issued before any compare
-
-// Probes the slots [pbase[p], pbase[p] + pfill[p]) of every partition p (a record layout with
-// gaps when the scatter used capacities). Does nothing if the scatter overflowed (*ovf).
-__global__ void __launch_bounds__(PB) k_part_probe(const u64 *__restrict__ Ik, const u64 *__restrict__ Iid,
-                                                   const u64 *__restrict__ pbase, const u32 *__restrict__ pfill, int P,
-                                                   const u32 *__restrict__ ovf, const u64 *__restrict__ koffs,
-                                                   int ninst, u64 NB, const u32 *__restrict__ T,
-                                                   const u64 *__restrict__ bstart, const u64 *__restrict__ Dk,
-                                                   const u64 *__restrict__ Did, unsigned long long *__restrict__ npairs,
-                                                   u64 cap, u64 *__restrict__ out_d, u64 *__restrict__ out_i,
-                                                   u64 *__restrict__ out_inst) {
+// Probe, pass 1: each record reads its bucket's 16 bytes; a record whose fingerprint is among the
+// bucket's slots, or whose bucket holds more than 8 D records, is a candidate. Candidates of a
+// 4096-record tile are placed with one global reservation per tile (a block scan of per-thread
+// hit counts), into cand (capacity = nslots, so the list cannot overflow).
+// Does nothing if the scatter overflowed (*ovf).
+__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u64 *__restrict__ rec, const u64 *__restrict__ pbase,
+                                                   const u32 *__restrict__ pfill, const u32 *__restrict__ ovf,
+                                                   const unsigned short *__restrict__ T,
+                                                   unsigned long long *__restrict__ ncand, u64 *__restrict__ cand) {
     __shared__ u64 s_base[257];
     __shared__ u64 s_end[256];
+    __shared__ u32 s_w[PB / 64];
+    __shared__ u64 s_gbase;
     if (*ovf) return;
+    const int P = 1 << kh.lgP;
+    const int boshift = kh.lgNB - kh.lgP;
     for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
     for (int i = threadIdx.x; i < P; i += PB) s_end[i] = pbase[i] + pfill[i];
     __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u64 nslots = s_base[P];
     const u64 ntiles = (nslots + PT - 1) / PT;
     // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous slice of tiles
@@ -357,7 +373,6 @@ __global__ void __launch_bounds__(PB) k_part_probe(const u64 *__restrict__ Ik, c
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
     for (u64 t = t_lo + kb; t < t_hi; t += nb_x) {
         const u64 e0 = t * PT;
-        // partition of the tile's first slot
         int p = 0;
         {
             int a = 0, b = P - 1;
@@ -367,31 +382,103 @@ __global__ void __launch_bounds__(PB) k_part_probe(const u64 *__restrict__ Ik, c
             }
             p = a;
         }
+        u32 hmask = 0;
         for (int it0 = 0; it0 < PI; it0 += PROBE_BATCH) {
-            u64 key[PROBE_BATCH], b[PROBE_BATCH];
+            u32 lo[PROBE_BATCH];
+            u64 b[PROBE_BATCH];
             bool ok[PROBE_BATCH];
-            uint4 t0[PROBE_BATCH], t1[PROBE_BATCH];
+            uint4 tv[PROBE_BATCH];
 #pragma unroll
             for (int q = 0; q < PROBE_BATCH; ++q) {
                 const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
                 int pp = p;
                 while (pp + 1 < P && e >= s_base[pp + 1]) ++pp;
                 ok[q] = e < nslots && e < s_end[pp];
-                key[q] = ok[q] ? Ik[e] : 0;
+                lo[q] = ok[q] ? (u32)rec[e] : 0u;
+                b[q] = ((u64)pp << boshift) | (lo[q] >> 12);
             }
 #pragma unroll
-            for (int q = 0; q < PROBE_BATCH; ++q) {
-                b[q] = bucket_of(key[q], NB);
-                const uint4 *src = (const uint4 *)(T + b[q] * TSLOTS);
-                t0[q] = src[0];
-                t1[q] = src[1];
-            }
+            for (int q = 0; q < PROBE_BATCH; ++q) tv[q] = *(const uint4 *)(T + b[q] * TSLOTS);
 #pragma unroll
             for (int q = 0; q < PROBE_BATCH; ++q) {
-                const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
-                if (ok[q])
-                    probe_emit(key[q], e, b[q], t0[q], t1[q], Iid, koffs, ninst, bstart, Dk, Did, npairs, cap, out_d,
-                               out_i, out_inst);
+                const u32 fs = 0x8000u | (lo[q] & 0xFFFu);
+                const uint4 v = tv[q];
+                const u32 w[4] = {v.x, v.y, v.z, v.w};
+                bool hit = (v.w >> 16) != 0;     // more than 8 D records in the bucket
+#pragma unroll
+                for (int k = 0; k < 4; ++k) hit |= ((w[k] & 0xFFFFu) == fs) | ((w[k] >> 16) == fs);
+                if (hit && ok[q]) hmask |= 1u << (it0 + q);
+            }
+        }
+        // place the tile's candidates: block exclusive scan of the per-thread counts
+        const u32 c = (u32)__popc(hmask);
+        u32 inc = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const u32 o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) s_w[wv] = inc;
+        __syncthreads();
+        u32 pre = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < PB / 64; ++q) {
+            if (q < wv) pre += s_w[q];
+            tot += s_w[q];
+        }
+        if (threadIdx.x == 0) s_gbase = tot ? atomicAdd(ncand, (unsigned long long)tot) : 0;
+        __syncthreads();
+        u64 at = s_gbase + pre + inc - c;
+        while (hmask) {
+            const int it = __ffs(hmask) - 1;
+            hmask &= hmask - 1;
+            cand[at++] = e0 + (u64)it * PB + threadIdx.x;
+        }
+        __syncthreads();
+    }
+}
+
+// Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
+// with the D records of its bucket; every equal key emits a pair.
+__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const u64 *__restrict__ rec,
+                                                         const u64 *__restrict__ pbase,
+                                                         const unsigned long long *__restrict__ ncand,
+                                                         const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
+                                                         int ninst, const unsigned short *__restrict__ T,
+                                                         const u64 *__restrict__ bstart, const u64 *__restrict__ Dk,
+                                                         const u64 *__restrict__ Did,
+                                                         unsigned long long *__restrict__ npairs, u64 cap,
+                                                         u64 *__restrict__ out_d, u64 *__restrict__ out_i,
+                                                         u64 *__restrict__ out_inst) {
+    __shared__ u64 s_base[257];
+    const int P = 1 << kh.lgP;
+    const int boshift = kh.lgNB - kh.lgP;
+    for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
+    __syncthreads();
+    const u64 nc = *ncand;
+    for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
+        const u64 e = cand[c];
+        int a = 0, bb = P - 1;
+        while (a < bb) {
+            const int mid = (a + bb + 1) >> 1;
+            if (s_base[mid] <= e) a = mid; else bb = mid - 1;
+        }
+        const u64 r = rec[e];
+        const u64 b = ((u64)a << boshift) | ((u32)r >> 12);
+        const u32 fs = 0x8000u | ((u32)r & 0xFFFu);
+        const uint4 v = *(const uint4 *)(T + b * TSLOTS);
+        const u32 w[4] = {v.x, v.y, v.z, v.w};
+        u64 key, iid;
+        leaf_exact(L, r >> 32, key, iid);
+        const u64 lo = bstart[b], hi = bstart[b + 1];
+        for (u64 q = lo; q < hi; ++q) {
+            if (q < lo + TSLOTS && ((w[(q - lo) >> 1] >> (16 * ((q - lo) & 1))) & 0xFFFFu) != fs) continue;
+            if (Dk[q] != key) continue;
+            const unsigned long long at = atomicAdd(npairs, 1ULL);
+            if (at < cap) {
+                out_d[at] = Did[q];
+                out_i[at] = iid;
+                out_inst[at] = inst_of_key(koffs, ninst, key);
             }
         }
     }
@@ -447,53 +534,61 @@ static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
 }
 
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, u64 NB, u32 *bcount) {
+                       int depth, int lgNB, u32 *bcount) {
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, bcount);
+    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgNB, bcount);
 }
 
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, u64 NB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did) {
+                         u64 n, int depth, int lgNB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did) {
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, bstart, bcur, Dk, Did);
+    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgNB, bstart, bcur, Dk,
+              Did);
 }
 
-void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, u32 *T) {
+void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, unsigned short *T) {
     CF_LAUNCH(ctx, "k_table_build", k_table_build, dim3(grid_for(NB, PB, 65536)), dim3(PB), 0, bstart, Dk, NB, T);
 }
 
 void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, u64 NB, int pshift, int P, u32 *pcount) {
+                int depth, int lgP, u32 *pcount) {
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_part_count", k_part_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, pshift, P, pcount);
+    CF_LAUNCH(ctx, "k_part_count", k_part_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgP, pcount);
 }
 
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, u64 NB, int pshift, int P, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *Ik, u64 *Iid) {
+                  int depth, int lgNB, int lgP, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *rec) {
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, NB, pshift, P, pbase, pcur,
-              ovf, Ik, Iid);
+    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, KeyHash{lgNB, lgP}, pbase,
+              pcur, ovf, rec);
 }
 
-void part_probe(cf_ctx *ctx, const u64 *Ik, const u64 *Iid, const u64 *pbase, const u32 *pfill, int P, u64 nslots,
-                const u32 *ovf, const u64 *koffs, int ninst, u64 NB, const u32 *T, const u64 *bstart, const u64 *Dk,
-                const u64 *Did, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst) {
+void part_probe(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, int depth,
+                int lgNB, int lgP, const u64 *rec, const u64 *pbase, const u32 *pfill, u64 nslots, const u32 *ovf,
+                const u64 *koffs, int ninst, const unsigned short *T, const u64 *bstart, const u64 *Dk, const u64 *Did,
+                unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst) {
     if (!nslots) return;
+    const KeyHash kh{lgNB, lgP};
+    DBuf<u64> ncand(ctx, 1), cand(ctx, nslots);
+    fill_u64(ctx, ncand.p, 1, 0);
     // the tile order gives each XCD a slice of tiles, so every XCD needs at least one block:
     // the grid is a non-zero multiple of the XCD count
     const u64 ntiles = (nslots + PT - 1) / PT;
     unsigned g = (unsigned)(ntiles < 8192 ? ntiles : 8192);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, Ik, Iid, pbase, pfill, P, ovf, koffs, ninst, NB, T,
-              bstart, Dk, Did, npairs, cap, od, oi, oinst);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, rec, pbase, pfill, ovf, T,
+              (unsigned long long *)ncand.p, cand.p);
+    LeafSrc L{descs, nodes, nn, off, lo, 0, depth, nullptr};
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, rec, pbase,
+              (const unsigned long long *)ncand.p, cand.p, koffs, ninst, T, bstart, Dk, Did, npairs, cap, od, oi, oinst);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out) {

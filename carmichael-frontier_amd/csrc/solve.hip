@@ -13,6 +13,7 @@
 // gpu/mitm_gpu2.py:107-119.
 #include <algorithm>
 #include <cmath>
+#include <array>
 #include <map>
 
 #include "cf_internal.h"
@@ -156,59 +157,102 @@ struct PhaseClock {
     }
 };
 
-// Fine buckets over the key space of one radius group: NB = a power of two >= nD / 3 (about
-// 1.5-3 D records per bucket), addressed by a multiplicative hash. The probe table holds 32 bytes
-// per bucket; P coarse partitions (top bits of the bucket index) are chosen so that one
-// partition's slice of the table is about 2 MB and stays in an XCD's 4 MB L2 while its records
-// are probed (16 <= P <= 256).
+// Fine buckets over the key space of one radius group: NB = 2^lgNB >= nD / 2 (1-2 D records per
+// bucket on average, so a bucket rarely outgrows its 8 table slots), addressed by the top bits of a multiplicative key hash. The probe table
+// holds 16 bytes per bucket; P = 2^lgP coarse partitions (the top hash bits) are chosen so one
+// partition's slice of the table is about 2 MB and stays in an XCD's 4 MB L2 while its records are
+// probed (16 <= P <= 256), and so a record's bucket offset inside its partition fits 20 bits.
 struct Partitioning {
     u64 NB = 1;
-    int P = 1, pshift = 0;
+    int lgNB = 0, lgP = 0;
+    int P() const { return 1 << lgP; }
 };
 
 Partitioning assign_buckets(u64 nD) {
     Partitioning pt;
-    int lg = 0;
-    while ((1ULL << lg) < nD / 3 || lg < 4) ++lg;
-    CF_REQUIRE(lg <= 40, "too many key buckets for one solve: split the batch");
+    int lg = 4;
+    while ((1ULL << lg) < nD / 2) ++lg;
+    CF_REQUIRE(lg <= 28, "more than 2^28 key buckets for one radius group: split the batch");
+    pt.lgNB = lg;
     pt.NB = 1ULL << lg;
     int lgP = 4;
-    while (lgP < 8 && (pt.NB * 32 >> lgP) > (2u << 20)) ++lgP;
+    while (lgP < 8 && ((pt.NB * 16) >> lgP) > (2u << 20)) ++lgP;
+    while (lg - lgP > 20) ++lgP;
     if (lgP > lg) lgP = lg;
-    pt.P = 1 << lgP;
-    pt.pshift = lg - lgP;
+    pt.lgP = lgP;
     return pt;
 }
 
-void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_stats *st, u64 max_chunk) {
+}  // namespace
+
+// A prepared batch: every instance validated, its exact bound tables computed on the host and
+// uploaded in one allocation; solving it is device work only.
+struct cf_batch {
+    cf_ctx *ctx = nullptr;
+    int n = 0;
+    std::vector<u64> koff;
+    std::vector<SideDesc> dd, id;
+    std::vector<bool> i_live, d_live;
+    std::map<int, std::vector<int>> groups;
+    DBuf<u64> store;
+    DBuf<SideDesc> d_desc, i_desc;
+    DBuf<u64> koffs;
+};
+
+namespace {
+
+void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
+    cf_ctx *ctx = b->ctx;
+    b->n = n;
+    b->koff.resize(n);
     // key offsets: instance j owns keys [koff_j, koff_j + M_j); the union must stay below 2^64 - 1
-    std::vector<u64> koff(n);
     u128 K = 0;
     for (int j = 0; j < n; ++j) {
-        koff[j] = (u64)K;
+        b->koff[j] = (u64)K;
         K += insts[j].M;
     }
     CF_REQUIRE(K < (u128)~0ULL, "the batch's moduli sum to 2^64 or more: split the batch");
-    std::vector<SideTables> dt(n), it(n);
-    std::vector<SideDesc> dd(n), id(n);
-    std::vector<bool> i_live(n, true), d_live(n, true);
+    b->dd.resize(n);
+    b->id.resize(n);
+    b->i_live.assign(n, true);
+    b->d_live.assign(n, true);
+    std::vector<u64> buf;
+    std::vector<std::array<size_t, 4>> offd(n), offi(n);
     for (int j = 0; j < n; ++j) {
-        SideSpec sd = spec_of_d(insts[j], koff[j]);
-        SideSpec si = spec_of_i(insts[j], koff[j]);
-        build_side_tables(ctx, sd, dt[j]);
-        build_side_tables(ctx, si, it[j]);
-        dd[j] = dt[j].desc;
-        id[j] = it[j].desc;
-        if (si.cap <= 1 || (int)si.primes.size() < si.r) i_live[j] = false;
-        if ((int)sd.primes.size() < sd.r) d_live[j] = false;
+        SideSpec sd = spec_of_d(insts[j], b->koff[j]);
+        SideSpec si = spec_of_i(insts[j], b->koff[j]);
+        side_tables_host(sd, b->dd[j], buf, offd[j].data());
+        side_tables_host(si, b->id[j], buf, offi[j].data());
+        if (si.cap <= 1 || (int)si.primes.size() < si.r) b->i_live[j] = false;
+        if ((int)sd.primes.size() < sd.r) b->d_live[j] = false;
+        b->groups[insts[j].r].push_back(j);
     }
-    DBuf<SideDesc> d_desc(ctx, n), i_desc(ctx, n);
-    DBuf<u64> koffs(ctx, n);
-    h2d(ctx, d_desc.p, dd.data(), n);
-    h2d(ctx, i_desc.p, id.data(), n);
-    h2d(ctx, koffs.p, koff.data(), n);
-    std::map<int, std::vector<int>> groups;
-    for (int j = 0; j < n; ++j) groups[insts[j].r].push_back(j);
+    b->store = DBuf<u64>(ctx, buf.size() ? buf.size() : 1);
+    h2d(ctx, b->store.p, buf.data(), buf.size());
+    for (int j = 0; j < n; ++j) {
+        side_tables_bind(b->dd[j], b->store.p, offd[j].data());
+        side_tables_bind(b->id[j], b->store.p, offi[j].data());
+    }
+    b->d_desc = DBuf<SideDesc>(ctx, n);
+    b->i_desc = DBuf<SideDesc>(ctx, n);
+    b->koffs = DBuf<u64>(ctx, n);
+    h2d(ctx, b->d_desc.p, b->dd.data(), n);
+    h2d(ctx, b->i_desc.p, b->id.data(), n);
+    h2d(ctx, b->koffs.p, b->koff.data(), n);
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
+    cf_ctx *ctx = b->ctx;
+    const int n = b->n;
+    const auto &dd = b->dd;
+    const auto &id = b->id;
+    const auto &d_live = b->d_live;
+    const auto &i_live = b->i_live;
+    const auto &groups = b->groups;
+    const DBuf<SideDesc> &d_desc = b->d_desc;
+    const DBuf<SideDesc> &i_desc = b->i_desc;
+    const DBuf<u64> &koffs = b->koffs;
 
     PhaseClock clk(ctx);
     clk.mark();  // 0
@@ -242,13 +286,13 @@ void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_
             DBuf<u32> bcount(ctx, pt.NB);
             DBuf<u64> bstart(ctx, pt.NB + 1);
             DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
-            DBuf<u32> T(ctx, pt.NB * 8);
+            DBuf<unsigned short> T(ctx, pt.NB * 8);
             if (nD) {
                 fill_u32(ctx, bcount.p, pt.NB, 0);
-                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.NB, bcount.p);
+                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.lgNB, bcount.p);
                 scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
                 fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
-                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.NB, bstart.p,
+                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.lgNB, bstart.p,
                                     bcount.p, Dk.p, Did.p);
                 part_table_build(ctx, bstart.p, Dk.p, pt.NB, T.p);
             }
@@ -263,47 +307,47 @@ void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_
             Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
             if (f.total && nD) {
-                DBuf<u32> pcount(ctx, pt.P), pcur(ctx, pt.P);
-                DBuf<u64> pstart(ctx, pt.P + 1), pbase(ctx, pt.P + 1);
-                pbase_h.assign(pt.P + 1, 0);
-                DBuf<u64> Ik, Iid;
+                const int P = pt.P();
+                DBuf<u32> pcount(ctx, P), pcur(ctx, P);
+                DBuf<u64> pstart(ctx, P + 1), pbase(ctx, P + 1);
+                pbase_h.assign(P + 1, 0);
+                DBuf<u64> rec;
                 u64 cap_m = 0;
                 for (u64 L = 0; L < f.total; L += chunk) {
                     const u64 m = std::min<u64>(chunk, f.total - L);
-                    const double mean = (double)m / pt.P;
+                    const double mean = (double)m / P;
                     u64 capP = (u64)(mean * 1.05 + 16.0 * std::sqrt(mean) + 64.0);
                     capP = (capP + 63) / 64 * 64;
-                    const u64 nslots = capP * pt.P;
+                    const u64 nslots = capP * P;
                     if (cap_m < std::max(nslots, m)) {
                         cap_m = std::max(nslots, m);
-                        Ik = DBuf<u64>(ctx, cap_m);
-                        Iid = DBuf<u64>(ctx, cap_m);
+                        rec = DBuf<u64>(ctx, cap_m);
                     }
-                    for (int p = 0; p <= pt.P; ++p) pbase_h[p] = (u64)p * capP;
-                    h2d(ctx, pbase.p, pbase_h.data(), pt.P + 1);
-                    fill_u32(ctx, pcur.p, pt.P, 0);
+                    for (int p = 0; p <= P; ++p) pbase_h[p] = (u64)p * capP;
+                    h2d(ctx, pbase.p, pbase_h.data(), P + 1);
+                    fill_u32(ctx, pcur.p, P, 0);
                     fill_u32(ctx, ovf.p, 1, 0);
-                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
-                                 pbase.p, pcur.p, ovf.p, Ik.p, Iid.p);
+                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgNB, pt.lgP, pbase.p,
+                                 pcur.p, ovf.p, rec.p);
                     u32 overflow = 0;
                     d2h(ctx, &overflow, ovf.p, 1);
-                    if (!overflow) {
-                        part_probe(ctx, Ik.p, Iid.p, pbase.p, pcur.p, pt.P, nslots, ovf.p, koffs.p, n, pt.NB, T.p,
-                                   bstart.p, Dk.p, Did.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p,
-                                   out.pinst.p);
-                        continue;
+                    const u64 *layout = pbase.p;
+                    u64 slots = nslots;
+                    if (overflow) {
+                        // exact layout: count per partition, scan, scatter again
+                        fill_u32(ctx, pcount.p, P, 0);
+                        part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgP, pcount.p);
+                        scan_u32_to_u64_async(ctx, pcount.p, pstart.p, P);
+                        fill_u32(ctx, pcur.p, P, 0);
+                        fill_u32(ctx, ovf.p, 1, 0);
+                        part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgNB, pt.lgP,
+                                     pstart.p, pcur.p, ovf.p, rec.p);
+                        layout = pstart.p;
+                        slots = m;
                     }
-                    // exact layout: count per partition, scan, scatter, probe
-                    fill_u32(ctx, pcount.p, pt.P, 0);
-                    part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
-                               pcount.p);
-                    scan_u32_to_u64_async(ctx, pcount.p, pstart.p, pt.P);
-                    fill_u32(ctx, pcur.p, pt.P, 0);
-                    fill_u32(ctx, ovf.p, 1, 0);
-                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.NB, pt.pshift, pt.P,
-                                 pstart.p, pcur.p, ovf.p, Ik.p, Iid.p);
-                    part_probe(ctx, Ik.p, Iid.p, pstart.p, pcur.p, pt.P, m, ovf.p, koffs.p, n, pt.NB, T.p, bstart.p,
-                               Dk.p, Did.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
+                    part_probe(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, pt.lgNB, pt.lgP, rec.p, layout,
+                               pcur.p, slots, ovf.p, koffs.p, n, T.p, bstart.p, Dk.p, Did.p,
+                               (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }
             gc.mark();
@@ -342,15 +386,25 @@ void solve_part(cf_ctx *ctx, const cf_instance *insts, int n, BatchOut &out, cf_
 
 extern "C" {
 
-int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_d, uint64_t **out_i,
-                   uint64_t **out_inst, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats) {
+int cf_batch_create(cf_ctx *ctx, const cf_instance *insts, int n, cf_batch **out) {
     API_BEGIN
-    CF_REQUIRE(ctx && insts && n >= 1 && out_d && out_i && out_inst && npairs, "NULL argument");
+    CF_REQUIRE(ctx && insts && n >= 1 && out, "NULL argument");
+    std::unique_ptr<cf_batch> b(new cf_batch());
+    b->ctx = ctx;
+    batch_prepare(b.get(), insts, n);
+    *out = b.release();
+    API_END
+}
+
+int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t **out_i, uint64_t **out_inst,
+                   uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats) {
+    API_BEGIN
+    CF_REQUIRE(b && out_d && out_i && out_inst && npairs, "NULL argument");
     *out_d = *out_i = *out_inst = nullptr;
     *npairs = 0;
     g_peak_live = 0;
     BatchOut bo;
-    solve_part(ctx, insts, n, bo, stats, 0);
+    solve_part(b, bo, stats, max_chunk);
     if (bo.npairs) {
         *out_d = bo.pd.release();
         *out_i = bo.pi.release();
@@ -358,11 +412,27 @@ int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_
     }
     *npairs = bo.npairs;
     if (inst_counts)
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < b->n; ++j) {
             inst_counts[2 * j] = bo.nD[j];
             inst_counts[2 * j + 1] = bo.nI[j];
         }
     API_END
+}
+
+int cf_batch_destroy(cf_batch *b) {
+    API_BEGIN
+    delete b;
+    API_END
+}
+
+int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_d, uint64_t **out_i,
+                   uint64_t **out_inst, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats) {
+    cf_batch *b = nullptr;
+    int rc = cf_batch_create(ctx, insts, n, &b);
+    if (rc) return rc;
+    rc = cf_batch_solve(b, 0, out_d, out_i, out_inst, npairs, inst_counts, stats);
+    cf_batch_destroy(b);
+    return rc;
 }
 
 int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk, uint64_t **out_d, uint64_t **out_i,
@@ -375,8 +445,11 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
     *npairs = 0;
     g_peak_live = 0;
     if (mode == CF_SOLVE_PARTITION) {
+        cf_batch b;
+        b.ctx = ctx;
+        batch_prepare(&b, inst, 1);
         BatchOut bo;
-        solve_part(ctx, inst, 1, bo, stats, max_chunk);
+        solve_part(&b, bo, stats, max_chunk);
         if (bo.npairs) {
             *out_d = bo.pd.release();
             *out_i = bo.pi.release();

@@ -28,6 +28,7 @@ extern "C" {
 
 typedef struct cf_ctx cf_ctx;
 typedef struct cf_plan cf_plan;
+typedef struct cf_batch cf_batch;
 
 #define CF_OK 0
 #define CF_EINVAL -1
@@ -162,6 +163,16 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
  * nD, nI of every instance. Requires the sum of the moduli to stay below 2^64 - 1. */
 int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_d_dev, uint64_t **out_i_dev,
                    uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);
+
+/* The same in two steps: cf_batch_create validates the instances and puts their descriptors and
+ * exact bound tables on the device (the host-side instance construction the reference's drivers
+ * do before any GPU work, gpu/mitm_gpu_mstar_driver.py:184-208, gpu/igen_gpu.py:91-111);
+ * cf_batch_solve is then device work only and may be called any number of times. Instances may be
+ * freed after cf_batch_create. max_chunk bounds the I records partitioned at once (0: 2^27). */
+int cf_batch_create(cf_ctx *ctx, const cf_instance *insts, int n, cf_batch **out);
+int cf_batch_solve(cf_batch *batch, uint64_t max_chunk, uint64_t **out_d_dev, uint64_t **out_i_dev,
+                   uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);
+int cf_batch_destroy(cf_batch *batch);
 
 #ifdef __cplusplus
 }

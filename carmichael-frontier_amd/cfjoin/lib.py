@@ -24,7 +24,7 @@ c_void_p = ctypes.c_void_p
 P_u64 = ctypes.POINTER(ctypes.c_uint64)
 
 CF_ID_COLEX, CF_ID_PACK_ASC, CF_ID_PACK_DESC, CF_ID_MASK = 0, 1, 2, 3
-CF_SOLVE_PARTITION, CF_SOLVE_SORTMERGE = 0, 1
+CF_SOLVE_PARTITION, CF_SOLVE_SORTMERGE, CF_SOLVE_PARTITION_DGLOBAL = 0, 1, 2
 
 _lib = None
 _lock = threading.RLock()

@@ -61,7 +61,8 @@ def make_cinstance(inst, keep: list) -> L.cf_instance:
                          i_id_mode=i_mode, d_id_mode=d_mode, pad=0)
 
 
-MODES = {"partition": L.CF_SOLVE_PARTITION, "sortmerge": L.CF_SOLVE_SORTMERGE}
+MODES = {"partition": L.CF_SOLVE_PARTITION, "sortmerge": L.CF_SOLVE_SORTMERGE,
+         "partition_dglobal": L.CF_SOLVE_PARTITION_DGLOBAL}
 
 
 def solve(inst, ctx: Optional[L.Context] = None, max_chunk: int = 0, mode: str = "partition") -> SolveResult:

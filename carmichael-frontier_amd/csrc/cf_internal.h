@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "keyhash.h"
 #include "modarith.h"
 
 // ---- errors -----------------------------------------------------------------------------------
@@ -150,7 +151,9 @@ struct SideDesc {
     const U128 *bound;  // I: H[t*n + x] = floor((Icap-1) / (a_x..a_{x+t})); D: G[t*n + i] = ceil(Dmin / (b_i..b_{i-t}))
     const u64 *binom;   // colex table C(x, j) at [x*(r+1) + j], x <= n, j <= r (ID_COLEX only)
     u64 koff;           // offset added to every key (disjoint key ranges across a batch)
-    int n, r, side, id_mode, id_bits, split_idx, hmax, pad;
+    int n, r, side, id_mode, id_bits, split_idx, hmax;
+    int leaf32;         // 1: every prime of the instance (both sides) is below 2^32, so join leaves take the
+                        // 32-bit scaled step (cf_redc32_mul); 0: the plain 64-bit step
     u64 xm0, x20;       // initial residue state (the side multiplier)
 };
 
@@ -219,19 +222,29 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 // radix-partitioned join (k_part.hip)
 void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, int lgNB, u32 *bcount);
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, int lgNB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did);
-void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, unsigned short *T);
-void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, int lgP, u32 *pcount);
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+                       int depth, const KeyHash &kh, u32 *bcount);
+// DR: n records of 16 bytes (exact key, id) in bucket order; T: 16 u16 slots per bucket, zeroed
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+                         int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
+// D side bucketed through LDS (hash ranges, then a counting sort per range); false when the
+// bucket geometry is too large for it (the caller then uses the per-record atomic path)
+// kerr: set non-zero if a record falls outside its range's buckets (an internal inconsistency;
+// the record is then dropped instead of written out of bounds)
+bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
+                const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr);
+// I leaves [lo, lo + n) partitioned exactly: n records (recL, recI) in partition order; pbase gets
+// the P + 1 partition starts
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, int lgNB, int lgP, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *rec);
-void part_probe(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, int depth,
-                int lgNB, int lgP, const u64 *rec, const u64 *pbase, const u32 *pfill, u64 nslots, const u32 *ovf,
-                const u64 *koffs, int ninst, const unsigned short *T, const u64 *bstart, const u64 *Dk, const u64 *Did,
-                unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst);
+                  int depth, const KeyHash &kh, u64 *pbase, u32 *recL, u32 *recI);
+// xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const u32 *recL, const u64 *pbase, u64 nslots,
+                      const unsigned short *T, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand);
+void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                      int depth, const KeyHash &kh, const u32 *recL, const u32 *recI, const u64 *pbase,
+                      const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
+                      const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
+                      u64 *oinst);
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
 
 // joins

@@ -1,26 +1,32 @@
 // k_part.hip -- radix-partitioned residue join on gfx950.
 //
 // Keys of one solve live in one key space (instance j owns [koff_j, koff_j + M_j), so keys of
-// different instances never collide). The space is cut into NB = 2^lgNB (about a third of the D
-// records) fine buckets by multiplicative hashing (the top bits of key * 0x9E3779B97F4A7C15).
-// Hashing, not key order, because keys are far from uniform: for small r the products prod(D)
-// stay below M and are their own residues, all in a sliver of [0, M), and a batch mixes moduli
-// from 1e11 to 1e17. The top lgP bits of the hash give P <= 256 coarse partitions.
+// different instances never collide). Every leaf (a D or an I record) is one residue step below
+// a node of its enumeration tree; the step is the 32-bit scaled one (cf_redc32_mul) when every
+// prime of the instance is below 2^32, so a leaf's residue state is (xm * 2^-32 mod m, x2 mod 2^a)
+// on both sides and equal residues still meet.
 //
-// Probe table: per bucket 8 fingerprint slots of 16 bits (16 bytes) for its first 8 D records; a
-// probe reads its bucket's 16 bytes and touches the D records only on a fingerprint hit (then the
-// exact key is recomputed and checked) or when the bucket holds more than 8 records.
+// Hash of a leaf: h = (xm + rot32(x2 mod 2^a) + koff) * 0x9E3779B97F4A7C15 -- no CRT step is needed
+// to place a record, only to compare one exactly. h gives the record's partition, hash range,
+// bucket and fingerprint (keyhash.h); NB is chosen for about 5 D records per bucket, and a
+// partition's buckets are contiguous, so its slice of the probe table is NB * 32 / P bytes (about
+// 2 MB).
 //
-//  D side: records are counted per fine bucket and scattered into bucket order (a counting sort
-//          on b: no full key sort is needed, a probe scans its bucket, 2-4 records on average).
-//  I side: leaves are scattered by partition as 8-byte records (bucket offset, fingerprint, leaf
-//          index): each block ranks its 2048 leaves by partition in LDS and writes one contiguous
-//          run per partition into slots sized from the expected share.
-//  Probe:  the partitioned I records are read once, in partition order, with an XCD-aware tile
-//          order (each of the 8 XCDs walks its own contiguous slice of tiles) so the D buckets
-//          of the partitions in flight stay in that XCD's L2. Every equal key emits a pair.
-// HBM traffic per I record: 8 B written by the scatter + 8 B read by the probe (plus the 16-byte
-// table bucket it looks up).
+// Probe table T: 16 slots of 16 bits per bucket (32 bytes), slot k holding 0x8000 | f of the
+// bucket's k-th D record; slot 7 also carries 0x4000 when the bucket holds more than 8 records. A
+// probe reads the first 16 bytes and the second only under that flag (about 7% of buckets at 5 per
+// bucket), and touches the D records only on a fingerprint hit or a full bucket (16 or more D
+// records), then compares exact keys.
+//
+//  D side: records counted per bucket and scattered into bucket order (DRec = exact key, id); the
+//          scatter writes the table slots as it goes.
+//  I side: leaves scattered by partition as two 4-byte words (recL = bucket offset << fpb | f,
+//          recI = leaf index in the chunk): each block ranks its 2048 leaves by partition in LDS
+//          and writes one run per partition into slots sized from the expected share.
+//  Probe:  pass 1 reads recL only, in partition order: each of the 8 XCDs takes tiles of its own
+//          contiguous slice from a queue, so an XCD works on about one partition at a time and
+//          that partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
+// HBM traffic per I record: 8 B written by the scatter + 4 B read by the probe.
 // Join semantics: every equal-key (D, I) pair (gpu/mitm_gpu2.py:107-119).
 #include "tree_dev.h"
 
@@ -32,6 +38,7 @@ constexpr int LT = PB * LPT;       // 2048 leaves per tile
 constexpr int PI = 16;             // records per thread in the probe
 constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
+constexpr int TSLOTS = 16;         // table slots per bucket
 
 struct LeafSrc {
     const SideDesc *descs;
@@ -43,84 +50,187 @@ struct LeafSrc {
     const u64 *tp;    // per tile of LT leaves: owners of its first and last leaf (2 entries per tile)
 };
 
-// Hashing of a key: h = key * 0x9E3779B97F4A7C15 (odd multiplier, so a bijection on u64).
-//   fine bucket    b = top lgNB bits of h          (lgNB <= 28: bits 36..63)
-//   partition      p = top lgP bits of h           (the first lgP bits of b)
-//   fingerprint    f = bits 20..31 of h            (independent of b)
-// An I record is 8 bytes: low word (b mod 2^(lgNB - lgP)) << 12 | f, high word the leaf's index
-// in its chunk; the exact key and id of a leaf are recomputed from the resident node tree only
-// when its fingerprint hits the D table. Table slots hold 0x8000 | f (0 = empty).
-struct KeyHash {
-    int lgNB, lgP;
+struct DRec {
+    u64 key, id;
 };
-__device__ __forceinline__ u64 khash(u64 key) { return key * 0x9E3779B97F4A7C15ULL; }
-__device__ __forceinline__ u64 hbucket(u64 h, int lgNB) { return h >> (64 - lgNB); }
-__device__ __forceinline__ u32 hfp(u64 h) { return (u32)(h >> 20) & 0xFFFu; }
-constexpr int TSLOTS = 8;
 
-// Tile set-up: the nodes owning leaves [j0, j1) are p0 .. p0 + np - 1 (every last-level node owns at
-// least one leaf, so np <= LT); their leaf offsets go to LDS: s_off[q] = off[p0 + q], q <= np.
-__device__ __forceinline__ void tile_setup(const LeafSrc &L, size_t t, u64 *s_off, size_t &p0, u32 &np) {
-    p0 = L.tp[2 * t];
-    np = (u32)(L.tp[2 * t + 1] - p0 + 1);
-    const u32 nc = np <= (u32)LT ? np : (u32)LT;
-    for (u32 i = threadIdx.x; i <= nc; i += PB) s_off[i] = L.off[p0 + i];
-    __syncthreads();
+// residue state of the leaf choosing index x below node nd
+__device__ __forceinline__ void leaf_residue(const SideDesc &s, const Node &nd, int x, u64 &xm, u64 &x2) {
+    if (s.leaf32) {
+        const u32 p = (u32)s.p[x];
+        xm = cf_redc32_mul(nd.xm, p, s.mc.m, (u32)s.mc.mneg);
+        x2 = nd.x2 * (u64)p;
+    } else {
+        xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
+        x2 = nd.x2 * s.p[x];
+    }
+}
+__device__ __forceinline__ u64 leaf_hash(u64 xm, u64 x2, u64 mask2, u64 koff) {
+    const u64 x2m = x2 & mask2;
+    return (xm + ((x2m << 32) | (x2m >> 32)) + koff) * 0x9E3779B97F4A7C15ULL;
 }
 
-// The leaves [jt, jt + LPT) of one thread, walked in order: the owning node is found once (binary
-// search of the tile's offsets in LDS), then advanced as the leaf index crosses node boundaries.
-struct LeafWalk {
-    const LeafSrc &L;
-    const u64 *s_off;
+// ---- leaf tiles ------------------------------------------------------------------------------
+// A tile is LT consecutive leaves [j0, j0 + cnt). Lane i of the block takes leaves j0 + i + PB * k,
+// so the 64 lanes of a wave work on 64 neighbouring leaves. The tile's parent nodes are
+// p0 .. p0 + np - 1 (every last-level node owns at least one leaf, so np <= LT). Set-up builds the
+// owner map own[i] = parent of leaf j0 + i (each node's first leaf is marked with its number, a
+// running maximum fills the rest) and, for a tile whose parents all belong to one instance with
+// primes below 2^32 and number at most NCAP (nearly every tile), stages in LDS what a leaf needs:
+// the parents' residue state, next index and first child number, the instance's primes and its
+// modulus constants. A leaf is then a few LDS reads and one 32-bit Montgomery step; other tiles
+// read nodes and descriptors from global memory.
+constexpr int NCAP = 1024;         // parents staged per tile
+constexpr int PCAP = 1024;         // primes staged (pools hold at most 1024)
+
+struct TileLds {
+    u64 xm[NCAP];                  // xm and x2 (16 KB) double as the scatter's staging area
+    u64 x2[NCAP];
+    int kb[NCAP];                  // child number of leaf j0 under parent q, i.e. j0 - off[p0 + q]
+    int nx[NCAP];                  // parent's next index
+    u32 prime[PCAP];
+    unsigned short own[LT];
+    u32 w[PB / 64];
+    u64 m, mask2, koff, minv;
+    u32 mneg32;
+    int sideI, inst;
+};
+
+struct Tile {
+    u64 j0;
+    u32 cnt;
     size_t p0;
-    u32 np, q;
-    u64 lo_off, hi_off;
+    u32 np;
+    bool fast;
+};
+
+__device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) {
+    Tile T;
+    T.j0 = L.lo + t * LT;
+    const u64 jend = L.lo + L.n;
+    T.cnt = (u32)((T.j0 + LT < jend ? T.j0 + LT : jend) - T.j0);
+    T.p0 = L.tp[2 * t];
+    T.np = (u32)(L.tp[2 * t + 1] - T.p0 + 1);
+    const u32 i0 = L.nodes[T.p0].meta >> 8, i1 = L.nodes[T.p0 + T.np - 1].meta >> 8;
+    const SideDesc &sd = L.descs[i0];
+    T.fast = i0 == i1 && T.np <= (u32)NCAP && sd.leaf32 && sd.n <= PCAP;
+    for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
+    if (T.fast) {
+        for (u32 q = threadIdx.x; q < T.np; q += PB) {
+            const Node &nd = L.nodes[T.p0 + q];
+            S.xm[q] = nd.xm;
+            S.x2[q] = nd.x2;
+            S.nx[q] = node_nxt(nd);
+            S.kb[q] = (int)(long long)(T.j0 - L.off[T.p0 + q]);
+        }
+        for (int x = threadIdx.x; x < sd.n; x += PB) S.prime[x] = (u32)sd.p[x];
+        if (threadIdx.x == 0) {
+            S.m = sd.mc.m;
+            S.mask2 = sd.mc.mask2;
+            S.koff = sd.koff;
+            S.minv = sd.mc.minv;
+            S.mneg32 = (u32)sd.mc.mneg;
+            S.sideI = sd.side == SIDE_I;
+            S.inst = (int)i0;
+        }
+    }
+    __syncthreads();
+    for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
+    __syncthreads();
+    // running maximum over own: LPT entries per thread, then a block scan of the thread maxima
+    u32 run = 0;
+    unsigned short v[LPT];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        v[k] = S.own[threadIdx.x * LPT + k];
+        run = run > v[k] ? run : v[k];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    u32 inc = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc = inc > o ? inc : o;
+    }
+    if (lane == 63) S.w[wv] = inc;
+    __syncthreads();
+    u32 pre = 0;
+    for (int q = 0; q < wv; ++q) pre = pre > S.w[q] ? pre : S.w[q];
+    u32 ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 0;
+    u32 cur = pre > ex ? pre : ex;
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        cur = cur > v[k] ? cur : v[k];
+        S.own[threadIdx.x * LPT + k] = (unsigned short)cur;
+    }
+    __syncthreads();
+    return T;
+}
+
+// Modulus constants of a fast tile, read once from LDS.
+struct FastCtx {
+    u64 m, mask2, koff;
+    u32 mneg32;
+    bool sideI;
+    __device__ __forceinline__ explicit FastCtx(const TileLds &S)
+        : m(S.m), mask2(S.mask2), koff(S.koff), mneg32(S.mneg32), sideI(S.sideI != 0) {}
+};
+
+// Fast tile: residue state of leaf j0 + i (any i < LT whose owner entry is valid) and its hash.
+__device__ __forceinline__ u64 leaf_fast(const TileLds &S, const FastCtx &c, u32 i, u64 &xm, u64 &x2, int &x,
+                                         u32 &q) {
+    q = S.own[i];
+    const int k = (int)i + S.kb[q];
+    x = c.sideI ? S.nx[q] + k : S.nx[q] - k;
+    const u32 p = S.prime[x];
+    xm = cf_redc32_mul(S.xm[q], p, c.m, c.mneg32);
+    x2 = S.x2[q] * (u64)p;
+    return leaf_hash(xm, x2, c.mask2, c.koff);
+}
+
+// Generic tile: the same from global memory.
+struct LeafRef {
     Node nd;
     const SideDesc *s;
-
-    __device__ __forceinline__ LeafWalk(const LeafSrc &L_, const u64 *s_off_, size_t p0_, u32 np_, u64 jt)
-        : L(L_), s_off(s_off_), p0(p0_), np(np_) {
-        u32 a = 0, b = np - 1;
-        while (a < b) {
-            const u32 mid = (a + b + 1) >> 1;
-            if (off_at(mid) <= jt) a = mid; else b = mid - 1;
-        }
-        q = a;
-        load();
-    }
-    __device__ __forceinline__ u64 off_at(u32 i) const { return np <= (u32)LT ? s_off[i] : L.off[p0 + i]; }
-    __device__ __forceinline__ void load() {
-        lo_off = off_at(q);
-        hi_off = off_at(q + 1);
-        nd = L.nodes[p0 + q];
-        s = &L.descs[nd.meta >> 8];
-    }
-    // key of leaf j (j >= every earlier j of this walk)
-    __device__ __forceinline__ void eval_key(u64 j, u64 &key) {
-        while (j >= hi_off) {
-            ++q;
-            load();
-        }
-        const int x = child_index(*s, nd, (int)(j - lo_off));
-        const u64 xm = cf_mulmod_m(nd.xm, s->pR[x], s->mc);
-        const u64 x2 = nd.x2 * s->p[x];
-        key = cf_key(xm, x2, s->mc) + s->koff;
-    }
-    // key, id and instance of leaf j (j >= every earlier j of this walk)
-    __device__ __forceinline__ void eval(u64 j, u64 &key, u64 &id, u32 &inst) {
-        while (j >= hi_off) {
-            ++q;
-            load();
-        }
-        inst = nd.meta >> 8;
-        const int x = child_index(*s, nd, (int)(j - lo_off));
-        const u64 xm = cf_mulmod_m(nd.xm, s->pR[x], s->mc);
-        const u64 x2 = nd.x2 * s->p[x];
-        key = cf_key(xm, x2, s->mc) + s->koff;
-        id = id_update(*s, nd.id, x, L.depth);
-    }
+    int x;
 };
+__device__ __forceinline__ LeafRef leaf_ref(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i) {
+    LeafRef r;
+    const u32 q = S.own[i];
+    r.nd = L.nodes[T.p0 + q];
+    r.s = &L.descs[r.nd.meta >> 8];
+    r.x = child_index(*r.s, r.nd, (int)(T.j0 + i - L.off[T.p0 + q]));
+    return r;
+}
+__device__ __forceinline__ u64 leaf_generic(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &xm,
+                                            u64 &x2, LeafRef &r) {
+    r = leaf_ref(L, T, S, i);
+    leaf_residue(*r.s, r.nd, r.x, xm, x2);
+    return leaf_hash(xm, x2, r.s->mc.mask2, r.s->koff);
+}
+
+// Hashes of the LPT leaves of this lane (lanes past the tile's end repeat its last leaf).
+__device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT]) {
+    if (T.fast) {
+        const FastCtx c(S);
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            u64 xm, x2;
+            int x;
+            u32 q;
+            h[k] = leaf_fast(S, c, i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
+        }
+    } else {
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            u64 xm, x2;
+            LeafRef r;
+            h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
+        }
+    }
+}
 
 // Owners of the first and the last leaf of every LT-leaf tile of [lo, lo + n): one binary search
 // per lane, all tiles at once, so no leaf kernel starts a tile with a serial search.
@@ -135,134 +245,246 @@ __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off
     }
 }
 
-// ---- D side: counting sort by fine bucket ---------------------------------------------------
-__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, int lgNB, u32 *__restrict__ bcount) {
-    __shared__ u64 s_off[LT + 1];
-    const size_t ntiles = (L.n + LT - 1) / LT;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
-        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
-        size_t p0;
-        u32 np;
-        tile_setup(L, t, s_off, p0, np);
-        const u64 jt = j0 + (u64)threadIdx.x * LPT;
-        if (jt < j1) {
-            LeafWalk w(L, s_off, p0, np, jt);
-            for (int k = 0; k < LPT && jt + k < j1; ++k) {
-                u64 key, id;
-                u32 inst;
-                w.eval(jt + k, key, id, inst);
-                atomicAdd(&bcount[hbucket(khash(key), lgNB)], 1u);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, int lgNB, const u64 *__restrict__ bstart,
-                                                       u32 *__restrict__ bcur, u64 *__restrict__ Dk,
-                                                       u64 *__restrict__ Did) {
-    __shared__ u64 s_off[LT + 1];
-    const size_t ntiles = (L.n + LT - 1) / LT;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
-        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
-        size_t p0;
-        u32 np;
-        tile_setup(L, t, s_off, p0, np);
-        const u64 jt = j0 + (u64)threadIdx.x * LPT;
-        if (jt < j1) {
-            LeafWalk w(L, s_off, p0, np, jt);
-            for (int k = 0; k < LPT && jt + k < j1; ++k) {
-                u64 key, id;
-                u32 inst;
-                w.eval(jt + k, key, id, inst);
-                const u64 b = hbucket(khash(key), lgNB);
-                const u64 at = bstart[b] + atomicAdd(&bcur[b], 1u);
-                Dk[at] = key;
-                Did[at] = id;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// ---- I side: partition by the top bits of the key hash --------------------------------------
-__global__ void __launch_bounds__(PB) k_part_count(LeafSrc L, int lgP, u32 *__restrict__ pcount) {
-    __shared__ u64 s_off[LT + 1];
-    __shared__ u32 h[256];
-    const int P = 1 << lgP;
-    const size_t ntiles = (L.n + LT - 1) / LT;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
-        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
-        h[threadIdx.x] = 0;
-        size_t p0;
-        u32 np;
-        tile_setup(L, t, s_off, p0, np);
-        const u64 jt = j0 + (u64)threadIdx.x * LPT;
-        if (jt < j1) {
-            LeafWalk w(L, s_off, p0, np, jt);
-            for (int k = 0; k < LPT && jt + k < j1; ++k) {
-                u64 key, id;
-                u32 inst;
-                w.eval(jt + k, key, id, inst);
-                atomicAdd(&h[(u32)(khash(key) >> (64 - lgP))], 1u);
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x < P && h[threadIdx.x]) atomicAdd(&pcount[threadIdx.x], h[threadIdx.x]);
-        __syncthreads();
-    }
-}
-
-// Partition p owns record slots [pbase[p], pbase[p + 1]); pcur[p] counts the slots taken. With a
-// capacity layout a partition that would run past its end sets *ovf and writes nothing for that
-// tile; the caller then redoes the chunk with exact counts as the layout.
-__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ pbase,
-                                                     u32 *__restrict__ pcur, u32 *__restrict__ ovf,
-                                                     u64 *__restrict__ rec) {
-    __shared__ u64 s_off[LT + 1];
-    __shared__ u32 h[256];
-    __shared__ u32 loc[256];
-    __shared__ u64 gbase[256];
-    __shared__ u32 s_w[PB / 64];
-    __shared__ u64 s_r[LT];
-    __shared__ u8_t s_p[LT];
-    constexpr u64 NONE = ~0ULL;
-    const int P = 1 << kh.lgP;
-    const u64 bomask = (1ULL << (kh.lgNB - kh.lgP)) - 1;
-    const size_t ntiles = (L.n + LT - 1) / LT;
-    for (size_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const u64 j0 = L.lo + t * LT, jend = L.lo + L.n;
-        const u64 j1 = j0 + LT < jend ? j0 + LT : jend;
-        const int cnt = (int)(j1 - j0);
-        h[threadIdx.x] = 0;
-        size_t p0;
-        u32 np;
-        tile_setup(L, t, s_off, p0, np);
-        u64 r[LPT];
-        u32 part[LPT], rank[LPT];
-        const u64 jt = j0 + (u64)threadIdx.x * LPT;
-        if (jt < j1) {
-            LeafWalk w(L, s_off, p0, np, jt);
+// ---- D side: counting sort by bucket --------------------------------------------------------
+__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 *__restrict__ bcount) {
+    __shared__ TileLds S;
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile T = tile_setup(L, t, S);
+        u64 h[LPT];
+        tile_hashes(L, T, S, h);
 #pragma unroll
-            for (int k = 0; k < LPT; ++k) {
-                if (jt + k < j1) {
-                    u64 key, id;
-                    u32 inst;
-                    w.eval_key(jt + k, key);
-                    const u64 hh = khash(key);
-                    part[k] = (u32)(hh >> (64 - kh.lgP));
-                    const u32 lo = (u32)((hbucket(hh, kh.lgNB) & bomask) << 12) | hfp(hh);
-                    r[k] = ((u64)(jt + k - L.lo) << 32) | lo;
-                    rank[k] = atomicAdd(&h[part[k]], 1u);
-                }
+        for (int k = 0; k < LPT; ++k)
+            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&bcount[hbucket(h[k], kh)], 1u);
+        __syncthreads();
+    }
+}
+
+// Scatter into bucket order; the rank a record takes in its bucket is also its table slot.
+__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
+                                                       u32 *__restrict__ bcur, DRec *__restrict__ DR,
+                                                       unsigned short *__restrict__ T) {
+    __shared__ TileLds S;
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tl = tile_setup(L, t, S);
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            if (i >= tl.cnt) continue;
+            const LeafRef r = leaf_ref(L, tl, S, i);
+            u64 xm, x2;
+            leaf_residue(*r.s, r.nd, r.x, xm, x2);
+            const u64 h = leaf_hash(xm, x2, r.s->mc.mask2, r.s->koff);
+            const u64 b = hbucket(h, kh);
+            const u32 slot = atomicAdd(&bcur[b], 1u);
+            const u64 b0 = bstart[b];
+            DR[b0 + slot] = DRec{cf_key(xm, x2, r.s->mc) + r.s->koff, id_update(*r.s, r.nd.id, r.x, L.depth)};
+            if (slot < (u32)TSLOTS) {
+                const u32 more = (slot == 7 && bstart[b + 1] - b0 > 8) ? 0x4000u : 0u;
+                T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));
             }
         }
         __syncthreads();
-        {   // local exclusive offsets of the partitions within the tile; one reservation per partition
-            const u32 v = h[threadIdx.x];
+    }
+}
+
+// ---- D side in LDS: hash range partition, then a counting sort per range -----------------
+// The D records are first placed in their 2^lgR hash ranges (range d owns buckets
+// [d * KR, (d + 1) * KR)), two passes per block range as on the I side; then one block per range
+// counts its records per bucket in LDS, writes the bucket starts, the records in bucket order and
+// the table slots. No global atomics.
+constexpr int PDMAX = 1 << CF_RANGE_BITS_LDS;          // hash ranges
+constexpr int WMAX = (int)CF_RANGE_BUCKETS_LDS;         // buckets per range
+
+struct DRec3 {
+    u64 key, id, h;
+};
+
+// D leaf of a tile: hash, exact key and id
+__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key,
+                                       u64 &id) {
+    u64 xm, x2;
+    if (T.fast) {
+        const FastCtx c(S);
+        int x;
+        u32 q;
+        h = leaf_fast(S, c, i, xm, x2, x, q);
+        const u64 t = ((x2 - xm) * S.minv) & c.mask2;
+        key = xm + c.m * t + c.koff;
+        id = id_update(L.descs[S.inst], L.nodes[T.p0 + q].id, x, L.depth);
+    } else {
+        LeafRef r;
+        h = leaf_generic(L, T, S, i, xm, x2, r);
+        key = cf_key(xm, x2, r.s->mc) + r.s->koff;
+        id = id_update(*r.s, r.nd.id, r.x, L.depth);
+    }
+}
+
+__global__ void __launch_bounds__(PB) k_drange_hist(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, u32 *__restrict__ H) {
+    __shared__ TileLds S;
+    __shared__ u32 hist[PDMAX];
+    const int PD = 1 << kh.lgR;
+    for (int d = threadIdx.x; d < PD; d += PB) hist[d] = 0;
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    const u64 t0 = (u64)blockIdx.x * tpb;
+    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    for (u64 t = t0; t < t1; ++t) {
+        const Tile T = tile_setup(L, t, S);
+        u64 h[LPT];
+        tile_hashes(L, T, S, h);
+#pragma unroll
+        for (int k = 0; k < LPT; ++k)
+            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&hist[hrange(h[k], kh)], 1u);
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < PD; d += PB) H[(u64)d * nblk + blockIdx.x] = hist[d];
+}
+
+__global__ void __launch_bounds__(PB) k_drange_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
+                                                       const u64 *__restrict__ O, DRec3 *__restrict__ out) {
+    __shared__ TileLds S;
+    __shared__ u32 cnt[PDMAX];
+    const int PD = 1 << kh.lgR;
+    for (int d = threadIdx.x; d < PD; d += PB) cnt[d] = 0;
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    const u64 t0 = (u64)blockIdx.x * tpb;
+    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    for (u64 t = t0; t < t1; ++t) {
+        const Tile T = tile_setup(L, t, S);
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            if (i >= T.cnt) continue;
+            u64 h, key, id;
+            d_leaf(L, T, S, i, h, key, id);
+            const u32 d = hrange(h, kh);
+            const u32 r = atomicAdd(&cnt[d], 1u);
+            out[O[(u64)d * nblk + blockIdx.x] + r] = DRec3{key, id, h};
+        }
+        __syncthreads();
+    }
+}
+
+// One block per hash range d: records in[dstart[d], dstart[d + 1]), whose buckets are
+// [d * KR, (d + 1) * KR), go to bucket order. A record outside those buckets (which the geometry
+// rules out) sets *kerr and is dropped rather than written out of bounds.
+__global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const DRec3 *__restrict__ in,
+                                                       const u64 *__restrict__ dstart, u64 *__restrict__ bstart,
+                                                       DRec *__restrict__ DR, unsigned short *__restrict__ T,
+                                                       u32 *__restrict__ kerr) {
+    __shared__ u32 cnt[WMAX];
+    __shared__ u32 pos[WMAX + 1];
+    __shared__ u32 s_w[PB / 64];
+    const u64 d = blockIdx.x;
+    const u32 w = (u32)kh.KR;
+    const u64 b_lo = d * kh.KR;
+    const u64 r0 = dstart[d], r1 = dstart[d + 1];
+    for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
+    __syncthreads();
+    for (u64 e = r0 + threadIdx.x; e < r1; e += PB) {
+        const u64 b = hbucket(in[e].h, kh) - b_lo;
+        if (b < w) atomicAdd(&cnt[b], 1u);
+        else atomicOr(kerr, 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0, w) into pos (PB threads, WMAX / PB consecutive entries each)
+    constexpr int PER = WMAX / PB;
+    u32 sum = 0;
+    const u32 base = threadIdx.x * PER;
+    for (int k = 0; k < PER; ++k) sum += base + k < w ? cnt[base + k] : 0u;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    u32 inc = sum;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const u32 o = __shfl_up(inc, dd, 64);
+        if (lane >= dd) inc += o;
+    }
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    u32 run = inc - sum;
+    for (int q = 0; q < wv; ++q) run += s_w[q];
+    for (int k = 0; k < PER; ++k) {
+        const u32 i = base + k;
+        if (i < w) {
+            pos[i] = run;
+            bstart[b_lo + i] = r0 + run;
+            run += cnt[i];
+        }
+    }
+    if (threadIdx.x == PB - 1) pos[w] = run;
+    if (d + 1 == gridDim.x && threadIdx.x == PB - 1) bstart[b_lo + w] = r0 + run;
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
+    __syncthreads();
+    for (u64 e = r0 + threadIdx.x; e < r1; e += PB) {
+        const DRec3 rec = in[e];
+        const u64 bb = hbucket(rec.h, kh) - b_lo;
+        if (bb >= w) continue;
+        const u32 b = (u32)bb;
+        const u32 slot = atomicAdd(&cnt[b], 1u);
+        DR[r0 + pos[b] + slot] = DRec{rec.key, rec.id};
+        if (slot < (u32)TSLOTS) {
+            const u32 more = (slot == 7 && pos[b + 1] - pos[b] > 8) ? 0x4000u : 0u;
+            T[(b_lo + b) * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(rec.h));
+        }
+    }
+}
+
+// ---- I side: partition by the top bits of the hash -------------------------------------------
+// Two passes over the same leaves and no global atomics: block k owns tiles [k * tpb, (k+1) * tpb)
+// and first counts its leaves per partition into H[p * nblk + k]; an exclusive scan of H in that
+// order gives O[p * nblk + k], where block k's run of partition p starts, so the second pass writes
+// every record to its final slot, one contiguous run per partition and block.
+__global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u32 nblk, u32 *__restrict__ H) {
+    __shared__ TileLds S;
+    __shared__ u32 hist[256];
+    const int P = 1 << lgP;
+    hist[threadIdx.x] = 0;
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    const u64 t0 = (u64)blockIdx.x * tpb;
+    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    for (u64 t = t0; t < t1; ++t) {
+        const Tile T = tile_setup(L, t, S);
+        u64 h[LPT];
+        tile_hashes(L, T, S, h);
+#pragma unroll
+        for (int k = 0; k < LPT; ++k)
+            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&hist[hpart(h[k], lgP)], 1u);
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)P) H[(u64)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
+                                                     const u64 *__restrict__ O, u32 *__restrict__ recL,
+                                                     u32 *__restrict__ recI) {
+    __shared__ TileLds S;
+    __shared__ u32 hist[256];
+    __shared__ u32 loc[256];
+    __shared__ u64 cur[256];
+    __shared__ u8_t s_p[LT];
+    uint2 *s_r = (uint2 *)S.xm;        // LT records staged over the parents' residue state (16 KB)
+    static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
+    const int P = 1 << kh.lgP;
+    const u32 fmask = (1u << kh.fpb) - 1u;
+    if (threadIdx.x < (unsigned)P) cur[threadIdx.x] = O[(u64)threadIdx.x * nblk + blockIdx.x];
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    const u64 t0 = (u64)blockIdx.x * tpb;
+    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    for (u64 t = t0; t < t1; ++t) {
+        hist[threadIdx.x] = 0;
+        const Tile T = tile_setup(L, t, S);
+        u64 h[LPT];
+        tile_hashes(L, T, S, h);
+        u32 rank[LPT];
+#pragma unroll
+        for (int k = 0; k < LPT; ++k)
+            if (threadIdx.x + PB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
+        __syncthreads();   // also: every lane is done with the staged parents
+        // tile-local exclusive offsets of the partitions (block scan of hist)
+        const u32 v = hist[threadIdx.x];
+        {
             const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
             u32 inc = v;
 #pragma unroll
@@ -270,52 +492,39 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, cons
                 const u32 o = __shfl_up(inc, d, 64);
                 if (lane >= d) inc += o;
             }
-            if (lane == 63) s_w[wv] = inc;
+            if (lane == 63) S.w[wv] = inc;
             __syncthreads();
             u32 pre = 0;
-            for (int q = 0; q < wv; ++q) pre += s_w[q];
+            for (int q = 0; q < wv; ++q) pre += S.w[q];
             loc[threadIdx.x] = pre + inc - v;
-            u64 gb = NONE;
-            if (threadIdx.x < (unsigned)P && v) {
-                const u64 r0 = atomicAdd(&pcur[threadIdx.x], v);
-                const u64 b0 = pbase[threadIdx.x] + r0;
-                if (b0 + v <= pbase[threadIdx.x + 1]) gb = b0;
-                else atomicOr(ovf, 1u);
-            }
-            gbase[threadIdx.x] = gb;
         }
         __syncthreads();
-        if (jt < j1) {
 #pragma unroll
-            for (int k = 0; k < LPT; ++k) {
-                if (jt + k < j1) {
-                    const u32 at = loc[part[k]] + rank[k];
-                    s_r[at] = r[k];
-                    s_p[at] = (u8_t)part[k];
-                }
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            if (i < T.cnt) {
+                const u32 part = hpart(h[k], kh.lgP);
+                const u32 at = loc[part] + rank[k];
+                s_r[at] = make_uint2((hoff(h[k], kh) << kh.fpb) | (hfp(h[k]) & fmask), (u32)(T.j0 + i - L.lo));
+                s_p[at] = (u8_t)part;
             }
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < cnt; q += PB) {
+        for (u32 q = threadIdx.x; q < T.cnt; q += PB) {
             const u32 pi = s_p[q];
-            if (gbase[pi] == NONE) continue;
-            rec[gbase[pi] + (u64)(q - (int)loc[pi])] = s_r[q];
+            const u64 g = cur[pi] + (u64)(q - loc[pi]);
+            const uint2 r = s_r[q];
+            recL[g] = r.x;
+            recI[g] = r.y;
         }
         __syncthreads();
+        if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
     }
 }
 
-// ---- probe table: 8 fingerprint slots of 16 bits per bucket ---------------------------------
-__global__ void __launch_bounds__(PB) k_table_build(const u64 *__restrict__ bstart, const u64 *__restrict__ Dk, u64 NB,
-                                                   unsigned short *__restrict__ T) {
-    for (u64 b = (u64)blockIdx.x * PB + threadIdx.x; b < NB; b += (u64)gridDim.x * PB) {
-        const u64 lo = bstart[b], cnt = bstart[b + 1] - lo;
-        u32 f[TSLOTS];
-#pragma unroll
-        for (int k = 0; k < TSLOTS; ++k) f[k] = (u64)k < cnt ? (0x8000u | hfp(khash(Dk[lo + k]))) : 0u;
-        *(uint4 *)(T + b * TSLOTS) = make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16),
-                                                f[6] | (f[7] << 16));
-    }
+// pbase[p] = O[p * nblk] for p <= P (O[P * nblk] is the total): where each partition starts
+__global__ void __launch_bounds__(PB) k_part_bases(const u64 *__restrict__ O, u32 nblk, int P, u64 *__restrict__ pbase) {
+    for (int p = threadIdx.x; p <= P; p += PB) pbase[p] = O[(u64)p * nblk];
 }
 
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).
@@ -329,49 +538,64 @@ __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ni
 }
 
 // Exact key and id of leaf lo + leaf (rare: only on a fingerprint hit).
-constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
-
 __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key, u64 &id) {
     const u64 j = L.lo + leaf;
     const size_t p = find_parent(L.off, 0, L.nn - 1, j);
     const Node nd = L.nodes[p];
     const SideDesc &s = L.descs[nd.meta >> 8];
     const int x = child_index(s, nd, (int)(j - L.off[p]));
-    const u64 xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
-    const u64 x2 = nd.x2 * s.p[x];
+    u64 xm, x2;
+    leaf_residue(s, nd, x, xm, x2);
     key = cf_key(xm, x2, s.mc) + s.koff;
     id = id_update(s, nd.id, x, L.depth);
 }
 
-// Probe, pass 1: each record reads its bucket's 16 bytes; a record whose fingerprint is among the
-// bucket's slots, or whose bucket holds more than 8 D records, is a candidate. Candidates of a
-// 4096-record tile are placed with one global reservation per tile (a block scan of per-thread
-// hit counts), into cand (capacity = nslots, so the list cannot overflow).
-// Does nothing if the scatter overflowed (*ovf).
-__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u64 *__restrict__ rec, const u64 *__restrict__ pbase,
-                                                   const u32 *__restrict__ pfill, const u32 *__restrict__ ovf,
-                                                   const unsigned short *__restrict__ T,
-                                                   unsigned long long *__restrict__ ncand, u64 *__restrict__ cand) {
+// Is the slot pattern fs2 (0x8000 | f in both halves of a word, compared under mask mm) among the
+// 4 words of a half bucket?
+__device__ __forceinline__ bool half_hit(const uint4 &a, u32 fs2, u32 mm) {
+    const u32 w[4] = {a.x, a.y, a.z, a.w};
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const u32 x = (w[k] & mm) ^ fs2;
+        hit |= ((x & 0xFFFFu) == 0) | ((x >> 16) == 0);
+    }
+    return hit;
+}
+
+constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
+
+// Probe, pass 1: each record reads the first half of its bucket (the second too when slot 7 says
+// the bucket holds more than 8); a record whose fingerprint is among the bucket's slots, or whose
+// bucket is full, is a candidate. Candidates of a 4096-record tile are
+// placed with one global reservation per tile (a block scan of per-thread hit counts) into
+// cand[0, ccap); *ncand counts them all, so the caller sees an overflow and repeats with room.
+// Tiles come from a queue per XCD (xq[8], zero at launch) over that XCD's contiguous slice; every
+// block leaves once its XCD's queue is past the slice.
+__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u32 *__restrict__ recL,
+                                                   const u64 *__restrict__ pbase, const unsigned short *__restrict__ T,
+                                                   u32 *__restrict__ xq, unsigned long long *__restrict__ ncand,
+                                                   u64 ccap, u64 *__restrict__ cand) {
     __shared__ u64 s_base[257];
-    __shared__ u64 s_end[256];
     __shared__ u32 s_w[PB / 64];
     __shared__ u64 s_gbase;
-    if (*ovf) return;
+    __shared__ u64 s_t;
     const int P = 1 << kh.lgP;
-    const int boshift = kh.lgNB - kh.lgP;
     for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
-    for (int i = threadIdx.x; i < P; i += PB) s_end[i] = pbase[i] + pfill[i];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const u64 nslots = s_base[P];
+    const u64 nslots = pbase[P];
     const u64 ntiles = (nslots + PT - 1) / PT;
-    // XCD-aware order: block b runs on XCD b % 8; give each XCD a contiguous slice of tiles
-    const u32 G = gridDim.x;
     const u32 xcd = blockIdx.x % NXCD;
-    const u32 nb_x = G / NXCD + (xcd < G % NXCD ? 1 : 0);     // blocks on this XCD
-    const u32 kb = blockIdx.x / NXCD;                         // rank of this block on its XCD
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
-    for (u64 t = t_lo + kb; t < t_hi; t += nb_x) {
+    const u32 fmask = (1u << kh.fpb) - 1u;
+    const u32 mm = (0x8000u | fmask) * 0x10001u;
+    const u64 K = kh_k(kh);
+    for (;;) {
+        // the barrier below also orders this write after every lane's read of the previous tile
+        if (threadIdx.x == 0) s_t = t_lo + atomicAdd(&xq[xcd], 1u);
+        __syncthreads();
+        const u64 t = s_t;
+        if (t >= t_hi) break;
         const u64 e0 = t * PT;
         int p = 0;
         {
@@ -387,27 +611,28 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u64 *__rest
             u32 lo[PROBE_BATCH];
             u64 b[PROBE_BATCH];
             bool ok[PROBE_BATCH];
-            uint4 tv[PROBE_BATCH];
+            uint4 ta[PROBE_BATCH];
 #pragma unroll
             for (int q = 0; q < PROBE_BATCH; ++q) {
                 const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
                 int pp = p;
                 while (pp + 1 < P && e >= s_base[pp + 1]) ++pp;
-                ok[q] = e < nslots && e < s_end[pp];
-                lo[q] = ok[q] ? (u32)rec[e] : 0u;
-                b[q] = ((u64)pp << boshift) | (lo[q] >> 12);
+                ok[q] = e < nslots;
+                lo[q] = ok[q] ? recL[e] : 0u;
+                b[q] = (u64)pp * K + (lo[q] >> kh.fpb);
             }
 #pragma unroll
-            for (int q = 0; q < PROBE_BATCH; ++q) tv[q] = *(const uint4 *)(T + b[q] * TSLOTS);
+            for (int q = 0; q < PROBE_BATCH; ++q) ta[q] = *(const uint4 *)(T + b[q] * TSLOTS);
 #pragma unroll
             for (int q = 0; q < PROBE_BATCH; ++q) {
-                const u32 fs = 0x8000u | (lo[q] & 0xFFFu);
-                const uint4 v = tv[q];
-                const u32 w[4] = {v.x, v.y, v.z, v.w};
-                bool hit = (v.w >> 16) != 0;     // more than 8 D records in the bucket
-#pragma unroll
-                for (int k = 0; k < 4; ++k) hit |= ((w[k] & 0xFFFFu) == fs) | ((w[k] >> 16) == fs);
-                if (hit && ok[q]) hmask |= 1u << (it0 + q);
+                const u32 fs2 = (0x8000u | (lo[q] & fmask)) * 0x10001u;
+                bool hit = half_hit(ta[q], fs2, mm);
+                if (ta[q].w & 0x40000000u) {
+                    // more than 8 D records: the second half, and slot 15 taken means full
+                    const uint4 tb = *(const uint4 *)(T + b[q] * TSLOTS + 8);
+                    hit |= half_hit(tb, fs2, mm) | ((tb.w >> 16) != 0);
+                }
+                if (ok[q] && hit) hmask |= 1u << (it0 + q);
             }
         }
         // place the tile's candidates: block exclusive scan of the per-thread counts
@@ -432,7 +657,8 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u64 *__rest
         while (hmask) {
             const int it = __ffs(hmask) - 1;
             hmask &= hmask - 1;
-            cand[at++] = e0 + (u64)it * PB + threadIdx.x;
+            if (at < ccap) cand[at] = e0 + (u64)it * PB + threadIdx.x;
+            ++at;
         }
         __syncthreads();
     }
@@ -440,22 +666,20 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u64 *__rest
 
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
 // with the D records of its bucket; every equal key emits a pair.
-__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const u64 *__restrict__ rec,
-                                                         const u64 *__restrict__ pbase,
-                                                         const unsigned long long *__restrict__ ncand,
+__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const u32 *__restrict__ recL,
+                                                         const u32 *__restrict__ recI, const u64 *__restrict__ pbase,
+                                                         const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
-                                                         int ninst, const unsigned short *__restrict__ T,
-                                                         const u64 *__restrict__ bstart, const u64 *__restrict__ Dk,
-                                                         const u64 *__restrict__ Did,
+                                                         int ninst, const u64 *__restrict__ bstart,
+                                                         const DRec *__restrict__ DR,
                                                          unsigned long long *__restrict__ npairs, u64 cap,
                                                          u64 *__restrict__ out_d, u64 *__restrict__ out_i,
                                                          u64 *__restrict__ out_inst) {
     __shared__ u64 s_base[257];
     const int P = 1 << kh.lgP;
-    const int boshift = kh.lgNB - kh.lgP;
     for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
     __syncthreads();
-    const u64 nc = *ncand;
+    const u64 nc = *ncand < ccap ? *ncand : ccap;
     for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
         const u64 e = cand[c];
         int a = 0, bb = P - 1;
@@ -463,20 +687,16 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, 
             const int mid = (a + bb + 1) >> 1;
             if (s_base[mid] <= e) a = mid; else bb = mid - 1;
         }
-        const u64 r = rec[e];
-        const u64 b = ((u64)a << boshift) | ((u32)r >> 12);
-        const u32 fs = 0x8000u | ((u32)r & 0xFFFu);
-        const uint4 v = *(const uint4 *)(T + b * TSLOTS);
-        const u32 w[4] = {v.x, v.y, v.z, v.w};
+        const u64 b = (u64)a * kh_k(kh) + (recL[e] >> kh.fpb);
         u64 key, iid;
-        leaf_exact(L, r >> 32, key, iid);
+        leaf_exact(L, recI[e], key, iid);
         const u64 lo = bstart[b], hi = bstart[b + 1];
         for (u64 q = lo; q < hi; ++q) {
-            if (q < lo + TSLOTS && ((w[(q - lo) >> 1] >> (16 * ((q - lo) & 1))) & 0xFFFFu) != fs) continue;
-            if (Dk[q] != key) continue;
+            const DRec d = DR[q];
+            if (d.key != key) continue;
             const unsigned long long at = atomicAdd(npairs, 1ULL);
             if (at < cap) {
-                out_d[at] = Did[q];
+                out_d[at] = d.id;
                 out_i[at] = iid;
                 out_inst[at] = inst_of_key(koffs, ninst, key);
             }
@@ -533,62 +753,91 @@ static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     return LeafSrc{descs, nodes, nn, off, lo, n, depth, tp.p};
 }
 
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                       int depth, int lgNB, u32 *bcount) {
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+                       int depth, const KeyHash &kh, u32 *bcount) {
     if (!n) return;
     DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgNB, bcount);
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
+    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, kh, bcount);
 }
 
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                         u64 n, int depth, int lgNB, const u64 *bstart, u32 *bcur, u64 *Dk, u64 *Did) {
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+                         int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T) {
     if (!n) return;
     DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgNB, bstart, bcur, Dk,
-              Did);
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
+    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, kh, bstart, bcur,
+              (DRec *)DR, T);
 }
 
-void part_table_build(cf_ctx *ctx, const u64 *bstart, const u64 *Dk, u64 NB, unsigned short *T) {
-    CF_LAUNCH(ctx, "k_table_build", k_table_build, dim3(grid_for(NB, PB, 65536)), dim3(PB), 0, bstart, Dk, NB, T);
-}
-
-void part_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                int depth, int lgP, u32 *pcount) {
-    if (!n) return;
+// D side in LDS (when the geometry allows it, see d_ranges): records of D leaves [0, n) in bucket
+// order into DR (16-byte records), bucket starts into bstart (NB + 1), table slots into T (zeroed).
+bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
+                const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr) {
+    if (kh.lgR > CF_RANGE_BITS_LDS || kh.KR > CF_RANGE_BUCKETS_LDS) return false;
+    const int PD = 1 << kh.lgR;
     DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_part_count", k_part_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, lgP, pcount);
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
+    const u64 ntiles = (n + LT - 1) / LT;
+    const u64 tpb = (ntiles + 1023) / 1024;
+    const u32 nblk = (u32)((ntiles + tpb - 1) / tpb);
+    DBuf<u32> H(ctx, (size_t)PD * nblk);
+    DBuf<u64> O(ctx, (size_t)PD * nblk + 1);
+    DBuf<u64> dstart(ctx, PD + 1);
+    DBuf<DRec3> tmp(ctx, n);
+    CF_LAUNCH(ctx, "k_drange_hist", k_drange_hist, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, H.p);
+    scan_u32_to_u64_async(ctx, H.p, O.p, (size_t)PD * nblk);
+    CF_LAUNCH(ctx, "k_part_bases", k_part_bases, dim3(1), dim3(PB), 0, O.p, nblk, PD, dstart.p);
+    CF_LAUNCH(ctx, "k_drange_scatter", k_drange_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, tmp.p);
+    CF_LAUNCH(ctx, "k_drange_buckets", k_drange_buckets, dim3(PD), dim3(PB), 0, kh, tmp.p, dstart.p, bstart, (DRec *)DR,
+              T, kerr);
+    return true;
 }
 
+// Partition leaves [lo, lo + n) exactly: recL/recI get n records in partition order, pbase the
+// P + 1 partition starts.
 void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, int lgNB, int lgP, const u64 *pbase, u32 *pcur, u32 *ovf, u64 *rec) {
-    if (!n) return;
+                  int depth, const KeyHash &kh, u64 *pbase, u32 *recL, u32 *recI) {
+    const int P = 1 << kh.lgP;
+    if (!n) {
+        fill_u64(ctx, pbase, P + 1, 0);
+        return;
+    }
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, KeyHash{lgNB, lgP}, pbase,
-              pcur, ovf, rec);
+    const u64 ntiles = (n + LT - 1) / LT;
+    // about 8 resident blocks per CU over 256 CUs, each with a long contiguous leaf range
+    const u64 tpb = (ntiles + 16383) / 16384;
+    const u32 nblk = (u32)((ntiles + tpb - 1) / tpb);
+    DBuf<u32> H(ctx, (size_t)P * nblk);
+    DBuf<u64> O(ctx, (size_t)P * nblk + 1);
+    CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, H.p);
+    scan_u32_to_u64_async(ctx, H.p, O.p, (size_t)P * nblk);
+    CF_LAUNCH(ctx, "k_part_bases", k_part_bases, dim3(1), dim3(PB), 0, O.p, nblk, P, pbase);
+    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, recL, recI);
 }
 
-void part_probe(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, int depth,
-                int lgNB, int lgP, const u64 *rec, const u64 *pbase, const u32 *pfill, u64 nslots, const u32 *ovf,
-                const u64 *koffs, int ninst, const unsigned short *T, const u64 *bstart, const u64 *Dk, const u64 *Did,
-                unsigned long long *npairs, u64 cap, u64 *od, u64 *oi, u64 *oinst) {
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const u32 *recL, const u64 *pbase, u64 nslots,
+                      const unsigned short *T, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand) {
+    fill_u32(ctx, xq, NXCD, 0);
+    fill_u64(ctx, (u64 *)ncand, 1, 0);
     if (!nslots) return;
-    const KeyHash kh{lgNB, lgP};
-    DBuf<u64> ncand(ctx, 1), cand(ctx, nslots);
-    fill_u64(ctx, ncand.p, 1, 0);
-    // the tile order gives each XCD a slice of tiles, so every XCD needs at least one block:
-    // the grid is a non-zero multiple of the XCD count
+    // tiles are dealt per XCD, so every XCD needs at least one block: the grid is a non-zero
+    // multiple of the XCD count, about as many blocks as the chip holds at once
     const u64 ntiles = (nslots + PT - 1) / PT;
-    unsigned g = (unsigned)(ntiles < 8192 ? ntiles : 8192);
+    unsigned g = (unsigned)(ntiles < 2048 ? ntiles : 2048);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, rec, pbase, pfill, ovf, T,
-              (unsigned long long *)ncand.p, cand.p);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, recL, pbase, T, xq, ncand, ccap, cand);
+}
+
+void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                      int depth, const KeyHash &kh, const u32 *recL, const u32 *recI, const u64 *pbase,
+                      const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
+                      const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
+                      u64 *oinst) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, nullptr};
-    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, rec, pbase,
-              (const unsigned long long *)ncand.p, cand.p, koffs, ninst, T, bstart, Dk, Did, npairs, cap, od, oi, oinst);
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, recL, recI, pbase, ncand,
+              ccap, cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out) {

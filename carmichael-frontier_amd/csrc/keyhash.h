@@ -1,0 +1,67 @@
+// keyhash.h -- bucket geometry of the partitioned join, shared by host and device code.
+//
+// A leaf's 64-bit hash h decides where its record goes:
+//   partition p = top lgP bits of h        (P = 2^lgP probe partitions)
+//   range     d = top lgR bits of h        (R = 2^lgR hash ranges, lgR >= lgP; each partition is
+//                                           2^(lgR - lgP) consecutive ranges)
+//   bucket    b = d * KR + ((next 32 - lgR bits of h) * KR >> (32 - lgR))
+// so range d owns exactly the buckets [d * KR, (d + 1) * KR) and partition p the buckets
+// [p * K, (p + 1) * K), K = KR << (lgR - lgP): a bucket never straddles a range or a partition,
+// and the D side can be counting-sorted one range at a time. NB = KR << lgR buckets in all.
+//   fingerprint f = bits 17..30 of h (14 bits, below every bit that places the record)
+#pragma once
+#include "modarith.h"
+
+struct KeyHash {
+    int lgP;    // 4 <= lgP <= 8
+    int lgR;    // lgP <= lgR <= 24
+    int fpb;    // fingerprint bits kept in an I record (<= 14)
+    int pad;
+    u64 KR;     // buckets per range
+};
+
+constexpr u64 CF_RANGE_BUCKETS_LDS = 8192;   // buckets per range the LDS counting sort takes
+constexpr int CF_RANGE_BITS_LDS = 12;         // ranges the LDS partition pass takes (2^12)
+
+CF_HD u64 kh_nb(const KeyHash &kh) { return kh.KR << kh.lgR; }
+CF_HD u64 kh_k(const KeyHash &kh) { return kh.KR << (kh.lgR - kh.lgP); }
+CF_HD u32 hpart(u64 h, int lgP) { return (u32)(h >> (64 - lgP)); }
+CF_HD u32 hrange(u64 h, const KeyHash &kh) { return (u32)(h >> (64 - kh.lgR)); }
+CF_HD u64 hbucket(u64 h, const KeyHash &kh) {
+    const u32 r = (u32)(h >> 32) & ((1u << (32 - kh.lgR)) - 1u);
+    return (u64)hrange(h, kh) * kh.KR + (((u64)r * kh.KR) >> (32 - kh.lgR));
+}
+// bucket offset within the record's partition, < K
+CF_HD u32 hoff(u64 h, const KeyHash &kh) { return (u32)(hbucket(h, kh) - (u64)hpart(h, kh.lgP) * kh_k(kh)); }
+CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
+
+// Geometry for a radius group with nD D records: about 5 D records per bucket, so a 16-slot bucket
+// is rarely full (Poisson tail P(X >= 16) ~ 7e-5); P = 2^lgP probe partitions, as few as keep one
+// partition's slice of the 32-byte-per-bucket table within about 2 MB (an XCD's L2 holds it while
+// the partition is probed), 16 <= P <= 256; as few hash ranges as keep a range within the buckets
+// the LDS counting sort takes (at most 2^12 ranges; beyond that the D side is bucketed by global
+// atomics). An I record keeps the bucket offset within its partition and 32 - that many bits of
+// fingerprint, at most 14. Returns false if a partition would need more than 2^24 buckets.
+inline int kh_bitlen(u64 v) {
+    int b = 0;
+    while (v) {
+        ++b;
+        v >>= 1;
+    }
+    return b;
+}
+inline bool kh_choose(u64 nD, KeyHash &kh) {
+    const u64 want = nD / 5 > 16 ? nD / 5 : 16;
+    int lgP = 4;
+    while (lgP < 8 && want * 32 / ((u64)1 << lgP) > (2u << 20)) ++lgP;
+    int lgR = lgP;
+    while (lgR < CF_RANGE_BITS_LDS && ((want + ((u64)1 << lgR) - 1) >> lgR) > CF_RANGE_BUCKETS_LDS) ++lgR;
+    const u64 KR = (want + ((u64)1 << lgR) - 1) >> lgR;
+    const int obits = kh_bitlen((KR << (lgR - lgP)) - 1);
+    kh.lgP = lgP;
+    kh.lgR = lgR;
+    kh.KR = KR;
+    kh.fpb = 32 - obits < 14 ? 32 - obits : 14;
+    kh.pad = 0;
+    return obits <= 24;
+}

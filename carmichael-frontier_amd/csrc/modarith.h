@@ -11,15 +11,26 @@
 // in [0, M) is recovered by Garner's CRT step:
 //     key = xm + m * (((x2 - xm) * m^-1) mod 2^a).
 // Only 64-bit multiply-low and multiply-high are used on the device: no 128/64 division.
+//
+// The join's leaves take one cheaper step, cf_redc32_mul: for a prime p < 2^32 a 32-bit Montgomery
+// step gives x * p * 2^-32 mod m from five 32x32 multiplies. Every leaf of an instance, D side and
+// I side alike, is exactly one such step below a plain-form node, so all of them carry the same
+// unit factor 2^-32 and equal residues stay equal: the join compares these scaled residues.
 #pragma once
+#ifdef __HIPCC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 typedef uint64_t u64;
 typedef unsigned int u32;
 typedef unsigned char u8_t;
 
+#ifdef __HIPCC__
 #define CF_HD __host__ __device__ __forceinline__
+#else
+#define CF_HD inline
+#endif
 
 CF_HD u64 cf_mulhi(u64 a, u64 b) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -68,6 +79,19 @@ CF_HD u64 cf_redc(u64 lo, u64 hi, u64 m, u64 mneg) {
 // xm * p mod m for xm < m, given pR = p * 2^64 mod m.
 CF_HD u64 cf_mulmod_m(u64 xm, u64 pR, const ModCtx &c) {
     return cf_redc(xm * pR, cf_mulhi(xm, pR), c.m, c.mneg);
+}
+
+// x * p * 2^-32 mod m for x < m and p < 2^32, with mneg32 = -m^-1 mod 2^32 (m odd).
+// T = x * p < m * 2^32 is held as hi = T >> 32 and the low word of lo; q makes T + q * m divisible
+// by 2^32, and (T + q * m) / 2^32 < 2m, which can pass 2^64 when m > 2^63 (caught by s < hi).
+CF_HD u64 cf_redc32_mul(u64 x, u32 p, u64 m, u32 mneg32) {
+    const u64 lo = (u64)(u32)x * p;
+    const u64 hi = (u64)(u32)(x >> 32) * p + (lo >> 32);
+    const u32 q = (u32)lo * mneg32;
+    const u64 c = ((u64)q * (u32)m + (u32)lo) >> 32;
+    const u64 t = (u64)q * (u32)(m >> 32) + c;
+    const u64 s = hi + t;
+    return (s < hi || s >= m) ? s - m : s;
 }
 
 // Canonical residue in [0, M) from (value mod m, value mod 2^64).

@@ -157,29 +157,17 @@ struct PhaseClock {
     }
 };
 
-// Fine buckets over the key space of one radius group: NB = 2^lgNB >= nD / 2 (1-2 D records per
-// bucket on average, so a bucket rarely outgrows its 8 table slots), addressed by the top bits of a multiplicative key hash. The probe table
-// holds 16 bytes per bucket; P = 2^lgP coarse partitions (the top hash bits) are chosen so one
-// partition's slice of the table is about 2 MB and stays in an XCD's 4 MB L2 while its records are
-// probed (16 <= P <= 256), and so a record's bucket offset inside its partition fits 20 bits.
+// Bucket geometry of one radius group (keyhash.h: kh_choose).
 struct Partitioning {
-    u64 NB = 1;
-    int lgNB = 0, lgP = 0;
-    int P() const { return 1 << lgP; }
+    KeyHash kh{};
+    u64 NB = 0;
+    int P() const { return 1 << kh.lgP; }
 };
 
 Partitioning assign_buckets(u64 nD) {
     Partitioning pt;
-    int lg = 4;
-    while ((1ULL << lg) < nD / 2) ++lg;
-    CF_REQUIRE(lg <= 28, "more than 2^28 key buckets for one radius group: split the batch");
-    pt.lgNB = lg;
-    pt.NB = 1ULL << lg;
-    int lgP = 4;
-    while (lgP < 8 && ((pt.NB * 16) >> lgP) > (2u << 20)) ++lgP;
-    while (lg - lgP > 20) ++lgP;
-    if (lgP > lg) lgP = lg;
-    pt.lgP = lgP;
+    CF_REQUIRE(kh_choose(nD, pt.kh), "more than 2^24 buckets per partition for one radius group: split the batch");
+    pt.NB = kh_nb(pt.kh);
     return pt;
 }
 
@@ -223,6 +211,11 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
         SideSpec si = spec_of_i(insts[j], b->koff[j]);
         side_tables_host(sd, b->dd[j], buf, offd[j].data());
         side_tables_host(si, b->id[j], buf, offi[j].data());
+        // the 32-bit scaled leaf step needs every prime of the instance (both sides) below 2^32
+        bool small = true;
+        for (u64 q : sd.primes) small &= q < (1ULL << 32);
+        for (u64 q : si.primes) small &= q < (1ULL << 32);
+        b->dd[j].leaf32 = b->id[j].leaf32 = small ? 1 : 0;
         if (si.cap <= 1 || (int)si.primes.size() < si.r) b->i_live[j] = false;
         if ((int)sd.primes.size() < sd.r) b->d_live[j] = false;
         b->groups[insts[j].r].push_back(j);
@@ -242,7 +235,7 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
     CF_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
+void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_lds = true) {
     cf_ctx *ctx = b->ctx;
     const int n = b->n;
     const auto &dd = b->dd;
@@ -257,11 +250,21 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
     PhaseClock clk(ctx);
     clk.mark();  // 0
     DBuf<u64> dcount(ctx, n), icount(ctx, n);
-    const u64 chunk = max_chunk ? max_chunk : ((u64)1 << 27);
+    // I leaves per chunk: as many as fit comfortably (the record index in a chunk is 32 bits); one
+    // chunk lets each XCD's probe stay inside one partition for a long stretch
+    u64 chunk = max_chunk;
+    if (!chunk) {
+        size_t free_b = 0, total_b = 0;
+        CF_HIP(hipMemGetInfo(&free_b, &total_b));
+        chunk = std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 40, (u64)1 << 24));
+    }
+    CF_REQUIRE(chunk <= ((u64)1 << 32), "max_chunk above 2^32 leaves");
     u64 cap = 1 << 16;
     DBuf<u64> cnt_pairs(ctx, 1);
-    DBuf<u32> ovf(ctx, 1);
-    std::vector<u64> pbase_h;
+    DBuf<u32> xq(ctx, 8);
+    DBuf<u64> flags(ctx, 1);                // candidate count of the last probe
+    DBuf<u32> kerr(ctx, 1);                 // set by a kernel that met an inconsistent record
+    fill_u32(ctx, kerr.p, 1, 0);
     double ms_d = 0, ms_i = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         out.pd = DBuf<u64>(ctx, cap);
@@ -283,24 +286,25 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
             const u64 nD = fd.total;
             if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
             const Partitioning pt = assign_buckets(nD);
+            const KeyHash &kh = pt.kh;
             DBuf<u32> bcount(ctx, pt.NB);
             DBuf<u64> bstart(ctx, pt.NB + 1);
-            DBuf<u64> Dk(ctx, nD ? nD : 1), Did(ctx, nD ? nD : 1);
-            DBuf<unsigned short> T(ctx, pt.NB * 8);
-            if (nD) {
+            DBuf<u64> DR(ctx, 2 * (nD ? nD : 1));
+            DBuf<unsigned short> T(ctx, pt.NB * 16);
+            if (nD) fill_u32(ctx, (u32 *)T.p, pt.NB * 8, 0);
+            if (nD && !(d_lds && part_d_lds(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
+                                            DR.p, T.p, kerr.p))) {
                 fill_u32(ctx, bcount.p, pt.NB, 0);
-                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.lgNB, bcount.p);
+                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bcount.p);
                 scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
                 fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
-                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, pt.lgNB, bstart.p,
-                                    bcount.p, Dk.p, Did.p);
-                part_table_build(ctx, bstart.p, Dk.p, pt.NB, T.p);
+                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
+                                    bcount.p, DR.p, T.p);
             }
             fd = Frontier();
             gc.mark();
-            // ---- I side: expand, then per chunk of leaves: scatter by partition into slots sized
-            // from the expected share (hashing spreads keys evenly), probe; a chunk whose scatter
-            // overflowed a partition (heavily repeated keys) is redone with exact counts
+            // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
+            // scatter), probe the table, check the candidates exactly
             roots.clear();
             for (int j : g.second)
                 if (i_live[j]) roots.push_back(root_node(id[j], SIDE_I, j));
@@ -308,46 +312,36 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
             if (f.total && nD) {
                 const int P = pt.P();
-                DBuf<u32> pcount(ctx, P), pcur(ctx, P);
-                DBuf<u64> pstart(ctx, P + 1), pbase(ctx, P + 1);
-                pbase_h.assign(P + 1, 0);
-                DBuf<u64> rec;
-                u64 cap_m = 0;
+                DBuf<u64> pbase(ctx, P + 1);
+                DBuf<u32> recL, recI;
+                DBuf<u64> cand;
+                u64 cap_m = 0, ccap = 0;
                 for (u64 L = 0; L < f.total; L += chunk) {
                     const u64 m = std::min<u64>(chunk, f.total - L);
-                    const double mean = (double)m / P;
-                    u64 capP = (u64)(mean * 1.05 + 16.0 * std::sqrt(mean) + 64.0);
-                    capP = (capP + 63) / 64 * 64;
-                    const u64 nslots = capP * P;
-                    if (cap_m < std::max(nslots, m)) {
-                        cap_m = std::max(nslots, m);
-                        rec = DBuf<u64>(ctx, cap_m);
+                    if (cap_m < m) {
+                        cap_m = m;
+                        recL = DBuf<u32>(ctx, cap_m);
+                        recI = DBuf<u32>(ctx, cap_m);
                     }
-                    for (int p = 0; p <= P; ++p) pbase_h[p] = (u64)p * capP;
-                    h2d(ctx, pbase.p, pbase_h.data(), P + 1);
-                    fill_u32(ctx, pcur.p, P, 0);
-                    fill_u32(ctx, ovf.p, 1, 0);
-                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgNB, pt.lgP, pbase.p,
-                                 pcur.p, ovf.p, rec.p);
-                    u32 overflow = 0;
-                    d2h(ctx, &overflow, ovf.p, 1);
-                    const u64 *layout = pbase.p;
-                    u64 slots = nslots;
-                    if (overflow) {
-                        // exact layout: count per partition, scan, scatter again
-                        fill_u32(ctx, pcount.p, P, 0);
-                        part_count(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgP, pcount.p);
-                        scan_u32_to_u64_async(ctx, pcount.p, pstart.p, P);
-                        fill_u32(ctx, pcur.p, P, 0);
-                        fill_u32(ctx, ovf.p, 1, 0);
-                        part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, pt.lgNB, pt.lgP,
-                                     pstart.p, pcur.p, ovf.p, rec.p);
-                        layout = pstart.p;
-                        slots = m;
+                    if (ccap < std::max<u64>(m >> 10, 1 << 16)) {
+                        ccap = std::max<u64>(m >> 10, 1 << 16);
+                        cand = DBuf<u64>(ctx, ccap);
                     }
-                    part_probe(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, pt.lgNB, pt.lgP, rec.p, layout,
-                               pcur.p, slots, ovf.p, koffs.p, n, T.p, bstart.p, Dk.p, Did.p,
-                               (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
+                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, pbase.p, recL.p,
+                                 recI.p);
+                    part_probe_pass1(ctx, kh, recL.p, pbase.p, m, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                     cand.p);
+                    const u64 nc = d2h_u64(ctx, flags.p);
+                    if (nc > ccap) {
+                        // more candidates than room (heavily repeated keys): probe again with room for all
+                        ccap = nc;
+                        cand = DBuf<u64>(ctx, ccap);
+                        part_probe_pass1(ctx, kh, recL.p, pbase.p, m, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                         cand.p);
+                    }
+                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, recL.p, recI.p,
+                                     pbase.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
+                                     DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }
             gc.mark();
@@ -355,6 +349,9 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk) {
             ms_i += gc.ms(1, 2);
         }
         const u64 np = d2h_u64(ctx, cnt_pairs.p);
+        u32 ke = 0;
+        d2h(ctx, &ke, kerr.p, 1);
+        if (ke) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
         out.npairs = np;
         if (np <= cap) break;
         cap = np;
@@ -439,17 +436,18 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
              uint64_t *npairs, cf_stats *stats) {
     API_BEGIN
     CF_REQUIRE(ctx && inst && out_d && out_i && npairs, "NULL argument");
-    CF_REQUIRE(mode == CF_SOLVE_PARTITION || mode == CF_SOLVE_SORTMERGE, "unknown solve mode");
+    CF_REQUIRE(mode == CF_SOLVE_PARTITION || mode == CF_SOLVE_SORTMERGE || mode == CF_SOLVE_PARTITION_DGLOBAL,
+               "unknown solve mode");
     *out_d = nullptr;
     *out_i = nullptr;
     *npairs = 0;
     g_peak_live = 0;
-    if (mode == CF_SOLVE_PARTITION) {
+    if (mode != CF_SOLVE_SORTMERGE) {
         cf_batch b;
         b.ctx = ctx;
         batch_prepare(&b, inst, 1);
         BatchOut bo;
-        solve_part(&b, bo, stats, max_chunk);
+        solve_part(&b, bo, stats, max_chunk, mode == CF_SOLVE_PARTITION);
         if (bo.npairs) {
             *out_d = bo.pd.release();
             *out_i = bo.pi.release();

@@ -148,12 +148,15 @@ typedef struct cf_stats {
     double ms_gen_d, ms_sort_d, ms_gen_i, ms_sort_i, ms_join;
 } cf_stats;
 
-/* mode: CF_SOLVE_PARTITION -- D records counting-sorted into fine key buckets; I leaves, in
- *       chunks of at most max_chunk (0: 2^27), partitioned by key range and probed against the
- *       D buckets of their partition; CF_SOLVE_SORTMERGE -- I chunks radix sorted and joined as
- *       join_sorted(). Same pair set. */
+/* mode: CF_SOLVE_PARTITION -- D records counting-sorted into hash buckets behind a fingerprint
+ *       table; I leaves, in chunks of at most max_chunk (0: as many as fit in device memory, at
+ *       most 2^31), partitioned by hash range and probed against the table slice of their
+ *       partition; CF_SOLVE_PARTITION_DGLOBAL -- the same with the D side bucketed by per-record
+ *       global atomics (the path very large D sides take); CF_SOLVE_SORTMERGE -- I chunks radix
+ *       sorted and joined as join_sorted(). Same pair set. */
 #define CF_SOLVE_PARTITION 0
 #define CF_SOLVE_SORTMERGE 1
+#define CF_SOLVE_PARTITION_DGLOBAL 2
 int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk, uint64_t **out_d_dev,
              uint64_t **out_i_dev, uint64_t *npairs, cf_stats *stats);
 
@@ -168,7 +171,8 @@ int cf_solve_batch(cf_ctx *ctx, const cf_instance *insts, int n, uint64_t **out_
  * exact bound tables on the device (the host-side instance construction the reference's drivers
  * do before any GPU work, gpu/mitm_gpu_mstar_driver.py:184-208, gpu/igen_gpu.py:91-111);
  * cf_batch_solve is then device work only and may be called any number of times. Instances may be
- * freed after cf_batch_create. max_chunk bounds the I records partitioned at once (0: 2^27). */
+ * freed after cf_batch_create. max_chunk bounds the I records partitioned at once (0: as many as
+ * fit, at most 2^31). */
 int cf_batch_create(cf_ctx *ctx, const cf_instance *insts, int n, cf_batch **out);
 int cf_batch_solve(cf_batch *batch, uint64_t max_chunk, uint64_t **out_d_dev, uint64_t **out_i_dev,
                    uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);

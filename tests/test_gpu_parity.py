@@ -254,9 +254,11 @@ def test_partition_equals_sortmerge_and_logged(workloads):
         say("partition", fam, D, r)
         a = pipeline.solve(inst, mode="partition")
         b = pipeline.solve(inst, mode="sortmerge")
+        c = pipeline.solve(inst, mode="partition_dglobal")
         lg = logged[(fam, D, r)]
         assert (a.nD, a.nI, len(a.pairs_d)) == (b.nD, b.nI, len(b.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"])
         assert np.array_equal(a.pair_array(), b.pair_array())
+        assert np.array_equal(c.pair_array(), b.pair_array())
 
 
 def test_batch_equals_alone(workloads):
@@ -279,17 +281,43 @@ def test_batch_equals_alone(workloads):
 
 @pytest.mark.parametrize("M", [64, 1024, 2 ** 20 + 7])
 def test_solve_heavy_collisions(M):
-    """Tiny moduli: keys repeat thousands of times, so partitions overflow their expected share and
-    the chunk is redone with exact counts; the pair multiset must still equal the oracle join."""
+    """Tiny moduli: keys repeat thousands of times, so buckets overflow their 16 table slots and the
+    probe finds more candidates than its first allowance (it is repeated with room for all); the
+    pair multiset must still equal the oracle join, with the D side bucketed either way."""
     P = [p for p in range(3, 2000) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
     B0, INS = P[:20], P[20:80]
     inst = I.Instance(M=M, r=3, B0=B0, INS=INS, P0=prod(B0), Icap=prod(INS[-3:]) + 1, Dmin=1, split=None, hmax=3,
                       tag=f"collide{M}")
     say("collide", M)
+    Dk, Did, Ik, Iid = O.instance_records(inst)
+    o = np.argsort(Dk, kind="stable")
+    q = np.argsort(Ik, kind="stable")
+    ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
+    for mode in ("partition", "partition_dglobal"):
+        res = pipeline.solve(inst, mode=mode)
+        assert (res.nD, res.nI) == (len(Dk), len(Ik))
+        assert np.array_equal(res.pair_array(), O.pair_set(ed, ei)), mode
+
+
+def test_solve_primes_above_32_bits():
+    """Pools with primes above 2^32 take the 64-bit leaf step (no 32-bit scaled residues); a mixed
+    batch holds instances of both kinds, each in its own key range."""
+    from cfjoin import verify as V
+    big = [q for q in range((1 << 32) + 1, (1 << 32) + 4000, 2) if V.is_prime(q)][:12]
+    P = [p for p in range(3, 400) if V.is_prime(p)]
+    M = 2 ** 8 * 3 ** 3 * 5 * 7      # small, so residues collide and pairs exist
+    B0, INS = P[:24], P[24:60] + big
+    inst = I.Instance(M=M, r=3, B0=B0, INS=INS, P0=prod(B0), Icap=prod(INS[-3:]) + 1, Dmin=1, split=None, hmax=3,
+                      tag="wide")
+    small = I.c1_instance(0, 100)
+    say("wide", M, len(INS))
     res = pipeline.solve(inst)
     Dk, Did, Ik, Iid = O.instance_records(inst)
     o = np.argsort(Dk, kind="stable")
     q = np.argsort(Ik, kind="stable")
     ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
     assert (res.nD, res.nI) == (len(Dk), len(Ik))
-    assert np.array_equal(res.pair_array(), O.pair_set(ed, ei))
+    assert len(ed) > 0 and np.array_equal(res.pair_array(), O.pair_set(ed, ei))
+    both, _ = pipeline.solve_batch([inst, small])
+    assert np.array_equal(both[0].pair_array(), res.pair_array())
+    assert np.array_equal(both[1].pair_array(), pipeline.solve(small).pair_array())

@@ -189,3 +189,19 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+# ---- device-side arithmetic and bucket geometry, compiled for the host ----------------------
+@pytest.mark.parametrize("name", ["t_modarith", "t_keyhash"])
+def test_native_checks(tmp_path, name):
+    """t_modarith: csrc/modarith.h (plain Montgomery step, Garner's key, the 32-bit scaled leaf step)
+    against 128-bit integer arithmetic. t_keyhash: csrc/keyhash.h -- every hash's bucket lies in its
+    own hash range and partition for the geometries of 1 to 10^10 D records, and a planted
+    partition-scaled map is caught. Each program exits 0 when every check holds."""
+    exe = tmp_path / name
+    src = os.path.join(ROOT, "tests", "native", name + ".cpp")
+    inc = os.path.join(ROOT, "carmichael-frontier_amd", "csrc")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", inc, "-o", str(exe), src], check=True, timeout=300)
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout
+    assert "bad=0" in p.stdout

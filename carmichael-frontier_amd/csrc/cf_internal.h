@@ -233,15 +233,36 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
 // the record is then dropped instead of written out of bounds)
 bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
                 const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr);
-// I leaves [lo, lo + n) partitioned exactly: n records (recL, recI) in partition order; pbase gets
-// the P + 1 partition starts
-void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, const KeyHash &kh, u64 *pbase, u32 *recL, u32 *recI);
+// Where one chunk's I records live: nseg segments, partition-major (nblk per partition); segment g
+// holds fill[g] records from slot O[g] (exact layout) or g << capl (capacity layout, O null).
+struct SegLayout {
+    const u64 *O;
+    const u32 *fill;
+    u64 nseg;
+    u32 nblk;
+    int capl;
+};
+struct IChunkLayout {   // the buffers behind a SegLayout
+    DBuf<u64> O;
+    DBuf<u32> fill;
+    u64 nseg = 0;
+    u32 nblk = 0;
+    int capl = 0;
+    SegLayout view() const { return SegLayout{O.p, fill.p, nseg, nblk, capl}; }
+};
+// slots a chunk of n leaves needs in the capacity layout
+u64 part_cap_slots(u64 n, const KeyHash &kh);
+// one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout)
+void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                      int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI, u32 *ovf);
+// count, scan and scatter: n records exactly, in partition order
+void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI);
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
-void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const u32 *recL, const u64 *pbase, u64 nslots,
-                      const unsigned short *T, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand);
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u32 *recL, const unsigned short *T,
+                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const u32 *recL, const u32 *recI, const u64 *pbase,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u32 *recL, const u32 *recI,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst);

@@ -296,10 +296,6 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
 constexpr int PDMAX = 1 << CF_RANGE_BITS_LDS;          // hash ranges
 constexpr int WMAX = (int)CF_RANGE_BUCKETS_LDS;         // buckets per range
 
-struct DRec3 {
-    u64 key, id, h;
-};
-
 // D leaf of a tile: hash, exact key and id
 __device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key,
                                        u64 &id) {
@@ -341,12 +337,15 @@ __global__ void __launch_bounds__(PB) k_drange_hist(LeafSrc L, KeyHash kh, u64 t
     for (int d = threadIdx.x; d < PD; d += PB) H[(u64)d * nblk + blockIdx.x] = hist[d];
 }
 
+// Record slots of block k in range d start at O[d * nblk + k]; the block keeps its running slot per
+// range in LDS (the D side of a join taking this path has fewer than 2^32 records).
 __global__ void __launch_bounds__(PB) k_drange_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
-                                                       const u64 *__restrict__ O, DRec3 *__restrict__ out) {
+                                                       const u64 *__restrict__ O, u64 *__restrict__ th,
+                                                       DRec *__restrict__ tkv) {
     __shared__ TileLds S;
-    __shared__ u32 cnt[PDMAX];
+    __shared__ u32 slot[PDMAX];
     const int PD = 1 << kh.lgR;
-    for (int d = threadIdx.x; d < PD; d += PB) cnt[d] = 0;
+    for (int d = threadIdx.x; d < PD; d += PB) slot[d] = (u32)O[(u64)d * nblk + blockIdx.x];
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
@@ -357,21 +356,22 @@ __global__ void __launch_bounds__(PB) k_drange_scatter(LeafSrc L, KeyHash kh, u6
             if (i >= T.cnt) continue;
             u64 h, key, id;
             d_leaf(L, T, S, i, h, key, id);
-            const u32 d = hrange(h, kh);
-            const u32 r = atomicAdd(&cnt[d], 1u);
-            out[O[(u64)d * nblk + blockIdx.x] + r] = DRec3{key, id, h};
+            const u32 g = atomicAdd(&slot[hrange(h, kh)], 1u);
+            th[g] = h;
+            tkv[g] = DRec{key, id};
         }
         __syncthreads();
     }
 }
 
-// One block per hash range d: records in[dstart[d], dstart[d + 1]), whose buckets are
+// One block per hash range d: records [dstart[d], dstart[d + 1]) of (th, tkv), whose buckets are
 // [d * KR, (d + 1) * KR), go to bucket order. A record outside those buckets (which the geometry
 // rules out) sets *kerr and is dropped rather than written out of bounds.
-__global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const DRec3 *__restrict__ in,
-                                                       const u64 *__restrict__ dstart, u64 *__restrict__ bstart,
-                                                       DRec *__restrict__ DR, unsigned short *__restrict__ T,
-                                                       u32 *__restrict__ kerr) {
+constexpr int DB = 8;              // records per thread loaded before any is used
+__global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const u64 *__restrict__ th,
+                                                       const DRec *__restrict__ tkv, const u64 *__restrict__ dstart,
+                                                       u64 *__restrict__ bstart, DRec *__restrict__ DR,
+                                                       unsigned short *__restrict__ T, u32 *__restrict__ kerr) {
     __shared__ u32 cnt[WMAX];
     __shared__ u32 pos[WMAX + 1];
     __shared__ u32 s_w[PB / 64];
@@ -381,10 +381,20 @@ __global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const DRec3 *
     const u64 r0 = dstart[d], r1 = dstart[d + 1];
     for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
     __syncthreads();
-    for (u64 e = r0 + threadIdx.x; e < r1; e += PB) {
-        const u64 b = hbucket(in[e].h, kh) - b_lo;
-        if (b < w) atomicAdd(&cnt[b], 1u);
-        else atomicOr(kerr, 1u);
+    for (u64 e0 = r0; e0 < r1; e0 += (u64)PB * DB) {
+        u64 hv[DB];
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {
+            const u64 e = e0 + threadIdx.x + (u64)PB * k;
+            hv[k] = e < r1 ? th[e] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {
+            if (e0 + threadIdx.x + (u64)PB * k >= r1) continue;
+            const u64 b = hbucket(hv[k], kh) - b_lo;
+            if (b < w) atomicAdd(&cnt[b], 1u);
+            else atomicOr(kerr, 1u);
+        }
     }
     __syncthreads();
     // exclusive scan of cnt[0, w) into pos (PB threads, WMAX / PB consecutive entries each)
@@ -416,16 +426,29 @@ __global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const DRec3 *
     __syncthreads();
     for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
     __syncthreads();
-    for (u64 e = r0 + threadIdx.x; e < r1; e += PB) {
-        const DRec3 rec = in[e];
-        const u64 bb = hbucket(rec.h, kh) - b_lo;
-        if (bb >= w) continue;
-        const u32 b = (u32)bb;
-        const u32 slot = atomicAdd(&cnt[b], 1u);
-        DR[r0 + pos[b] + slot] = DRec{rec.key, rec.id};
-        if (slot < (u32)TSLOTS) {
-            const u32 more = (slot == 7 && pos[b + 1] - pos[b] > 8) ? 0x4000u : 0u;
-            T[(b_lo + b) * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(rec.h));
+    for (u64 e0 = r0; e0 < r1; e0 += (u64)PB * DB) {
+        u64 hv[DB];
+        DRec kv[DB];
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {
+            const u64 e = e0 + threadIdx.x + (u64)PB * k;
+            if (e < r1) {
+                hv[k] = th[e];
+                kv[k] = tkv[e];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DB; ++k) {
+            if (e0 + threadIdx.x + (u64)PB * k >= r1) continue;
+            const u64 bb = hbucket(hv[k], kh) - b_lo;
+            if (bb >= w) continue;
+            const u32 b = (u32)bb;
+            const u32 slot = atomicAdd(&cnt[b], 1u);
+            DR[r0 + pos[b] + slot] = kv[k];
+            if (slot < (u32)TSLOTS) {
+                const u32 more = (slot == 7 && pos[b + 1] - pos[b] > 8) ? 0x4000u : 0u;
+                T[(b_lo + b) * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(hv[k]));
+            }
         }
     }
 }
@@ -456,19 +479,30 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
     if (threadIdx.x < (unsigned)P) H[(u64)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Second pass. Exact layout (O non-null): block k's run of partition p starts at O[p * nblk + k].
+// Capacity layout (O null): block k owns the segment of 2^capl slots at (p * nblk + k) << capl for
+// each partition p and writes its fill into fill[p * nblk + k]; a segment that would overflow sets
+// *ovf (its excess records are not written), and the caller redoes the chunk with the exact layout.
+// The capacity layout needs no first pass: its segments are sized for the expected share
+// (hashing spreads records evenly) with a margin of about 8 standard deviations.
 __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
-                                                     const u64 *__restrict__ O, u32 *__restrict__ recL,
+                                                     const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
+                                                     u32 *__restrict__ ovf, u32 *__restrict__ recL,
                                                      u32 *__restrict__ recI) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
     __shared__ u64 cur[256];
     __shared__ u8_t s_p[LT];
+    __shared__ u32 s_ovf;
     uint2 *s_r = (uint2 *)S.xm;        // LT records staged over the parents' residue state (16 KB)
     static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
     const int P = 1 << kh.lgP;
     const u32 fmask = (1u << kh.fpb) - 1u;
-    if (threadIdx.x < (unsigned)P) cur[threadIdx.x] = O[(u64)threadIdx.x * nblk + blockIdx.x];
+    const bool exact = O != nullptr;
+    const u64 cap = (u64)1 << capl;
+    if (threadIdx.x < (unsigned)P) cur[threadIdx.x] = exact ? O[(u64)threadIdx.x * nblk + blockIdx.x] : 0;
+    if (threadIdx.x == 0) s_ovf = 0;
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
@@ -512,13 +546,31 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
         __syncthreads();
         for (u32 q = threadIdx.x; q < T.cnt; q += PB) {
             const u32 pi = s_p[q];
-            const u64 g = cur[pi] + (u64)(q - loc[pi]);
+            const u64 o = cur[pi] + (u64)(q - loc[pi]);
+            u64 g;
+            if (exact) {
+                g = o;
+            } else {
+                if (o >= cap) {
+                    s_ovf = 1;
+                    continue;
+                }
+                g = (((u64)pi * nblk + blockIdx.x) << capl) + o;
+            }
             const uint2 r = s_r[q];
             recL[g] = r.x;
             recI[g] = r.y;
         }
         __syncthreads();
         if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
+    }
+    __syncthreads();
+    if (!exact) {
+        if (threadIdx.x < (unsigned)P) {
+            const u64 c = cur[threadIdx.x];
+            fill[(u64)threadIdx.x * nblk + blockIdx.x] = (u32)(c < cap ? c : cap);
+        }
+        if (threadIdx.x == 0 && s_ovf) atomicOr(ovf, 1u);
     }
 }
 
@@ -567,98 +619,86 @@ constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued 
 
 // Probe, pass 1: each record reads the first half of its bucket (the second too when slot 7 says
 // the bucket holds more than 8); a record whose fingerprint is among the bucket's slots, or whose
-// bucket is full, is a candidate. Candidates of a 4096-record tile are
-// placed with one global reservation per tile (a block scan of per-thread hit counts) into
-// cand[0, ccap); *ncand counts them all, so the caller sees an overflow and repeats with room.
-// Tiles come from a queue per XCD (xq[8], zero at launch) over that XCD's contiguous slice; every
-// block leaves once its XCD's queue is past the slice.
-__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u32 *__restrict__ recL,
-                                                   const u64 *__restrict__ pbase, const unsigned short *__restrict__ T,
-                                                   u32 *__restrict__ xq, unsigned long long *__restrict__ ncand,
-                                                   u64 ccap, u64 *__restrict__ cand) {
-    __shared__ u64 s_base[257];
-    __shared__ u32 s_w[PB / 64];
-    __shared__ u64 s_gbase;
+// bucket is full, is a candidate. A tile is SPT consecutive segments of the chunk's layout; each
+// wave of the block takes whole segments, so a segment's partition and start are uniform across
+// the wave. Candidates (rare) are placed by the wave that finds them, one reservation per batch
+// that has any, into cand[0, ccap); *ncand counts them all, so the caller sees an overflow and
+// repeats with room. Tiles come from a queue per XCD (xq[8], zero at launch) over that XCD's
+// contiguous slice of tiles, so an XCD works through its partitions one after another and the
+// table slice of the partition in hand stays in its L2; every block leaves once its XCD's queue
+// is past the slice.
+constexpr int SPT = 8;             // segments per tile: two per wave
+
+__device__ __forceinline__ u64 seg_start(const SegLayout &S, u64 g) { return S.O ? S.O[g] : (g << S.capl); }
+
+__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, const u32 *__restrict__ recL,
+                                                   const unsigned short *__restrict__ T, u32 *__restrict__ xq,
+                                                   unsigned long long *__restrict__ ncand, u64 ccap,
+                                                   u64 *__restrict__ cand) {
     __shared__ u64 s_t;
-    const int P = 1 << kh.lgP;
-    for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const u64 nslots = pbase[P];
-    const u64 ntiles = (nslots + PT - 1) / PT;
+    const u64 ntiles = (S.nseg + SPT - 1) / SPT;
     const u32 xcd = blockIdx.x % NXCD;
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const u32 mm = (0x8000u | fmask) * 0x10001u;
     const u64 K = kh_k(kh);
     for (;;) {
-        // the barrier below also orders this write after every lane's read of the previous tile
+        // the barrier at the end of the loop orders this write after every wave's read of the last tile
         if (threadIdx.x == 0) s_t = t_lo + atomicAdd(&xq[xcd], 1u);
         __syncthreads();
         const u64 t = s_t;
         if (t >= t_hi) break;
-        const u64 e0 = t * PT;
-        int p = 0;
-        {
-            int a = 0, b = P - 1;
-            while (a < b) {
-                const int mid = (a + b + 1) >> 1;
-                if (s_base[mid] <= e0) a = mid; else b = mid - 1;
-            }
-            p = a;
-        }
-        u32 hmask = 0;
-        for (int it0 = 0; it0 < PI; it0 += PROBE_BATCH) {
-            u32 lo[PROBE_BATCH];
-            u64 b[PROBE_BATCH];
-            bool ok[PROBE_BATCH];
-            uint4 ta[PROBE_BATCH];
+        for (int sj = 0; sj < SPT / (PB / 64); ++sj) {
+            const u64 g = t * SPT + (u64)wv * (SPT / (PB / 64)) + sj;
+            if (g >= S.nseg) break;
+            const u32 fill = S.fill[g];
+            const u32 *rec = recL + seg_start(S, g);
+            const u64 bpart = (g / S.nblk) * K;
+            for (u32 v0 = 0; v0 < fill; v0 += 64 * PROBE_BATCH) {
+                u32 lo[PROBE_BATCH];
+                uint4 ta[PROBE_BATCH];
+                u32 hmask = 0;
 #pragma unroll
-            for (int q = 0; q < PROBE_BATCH; ++q) {
-                const u64 e = e0 + (u64)(it0 + q) * PB + threadIdx.x;
-                int pp = p;
-                while (pp + 1 < P && e >= s_base[pp + 1]) ++pp;
-                ok[q] = e < nslots;
-                lo[q] = ok[q] ? recL[e] : 0u;
-                b[q] = (u64)pp * K + (lo[q] >> kh.fpb);
-            }
-#pragma unroll
-            for (int q = 0; q < PROBE_BATCH; ++q) ta[q] = *(const uint4 *)(T + b[q] * TSLOTS);
-#pragma unroll
-            for (int q = 0; q < PROBE_BATCH; ++q) {
-                const u32 fs2 = (0x8000u | (lo[q] & fmask)) * 0x10001u;
-                bool hit = half_hit(ta[q], fs2, mm);
-                if (ta[q].w & 0x40000000u) {
-                    // more than 8 D records: the second half, and slot 15 taken means full
-                    const uint4 tb = *(const uint4 *)(T + b[q] * TSLOTS + 8);
-                    hit |= half_hit(tb, fs2, mm) | ((tb.w >> 16) != 0);
+                for (int q = 0; q < PROBE_BATCH; ++q) {
+                    const u32 v = v0 + (u32)q * 64 + lane;
+                    lo[q] = v < fill ? rec[v] : 0u;
                 }
-                if (ok[q] && hit) hmask |= 1u << (it0 + q);
+#pragma unroll
+                for (int q = 0; q < PROBE_BATCH; ++q)
+                    ta[q] = *(const uint4 *)(T + (bpart + (lo[q] >> kh.fpb)) * TSLOTS);
+#pragma unroll
+                for (int q = 0; q < PROBE_BATCH; ++q) {
+                    const u32 fs2 = (0x8000u | (lo[q] & fmask)) * 0x10001u;
+                    bool hit = half_hit(ta[q], fs2, mm);
+                    if (ta[q].w & 0x40000000u) {
+                        // more than 8 D records: the second half, and slot 15 taken means full
+                        const uint4 tb = *(const uint4 *)(T + (bpart + (lo[q] >> kh.fpb)) * TSLOTS + 8);
+                        hit |= half_hit(tb, fs2, mm) | ((tb.w >> 16) != 0);
+                    }
+                    if (hit && v0 + (u32)q * 64 + lane < fill) hmask |= 1u << q;
+                }
+                if (__any(hmask != 0)) {
+                    const u32 c = (u32)__popc(hmask);
+                    u32 inc = c;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const u32 o = __shfl_up(inc, d, 64);
+                        if (lane >= d) inc += o;
+                    }
+                    unsigned long long base = 0;
+                    if (lane == 63) base = atomicAdd(ncand, (unsigned long long)inc);
+                    base = __shfl(base, 63, 64);
+                    u64 at = base + inc - c;
+                    const u64 e0 = seg_start(S, g) + v0 + lane;
+                    while (hmask) {
+                        const int q = __ffs(hmask) - 1;
+                        hmask &= hmask - 1;
+                        if (at < ccap) cand[at] = e0 + (u64)q * 64;
+                        ++at;
+                    }
+                }
             }
-        }
-        // place the tile's candidates: block exclusive scan of the per-thread counts
-        const u32 c = (u32)__popc(hmask);
-        u32 inc = c;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const u32 o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += o;
-        }
-        if (lane == 63) s_w[wv] = inc;
-        __syncthreads();
-        u32 pre = 0, tot = 0;
-#pragma unroll
-        for (int q = 0; q < PB / 64; ++q) {
-            if (q < wv) pre += s_w[q];
-            tot += s_w[q];
-        }
-        if (threadIdx.x == 0) s_gbase = tot ? atomicAdd(ncand, (unsigned long long)tot) : 0;
-        __syncthreads();
-        u64 at = s_gbase + pre + inc - c;
-        while (hmask) {
-            const int it = __ffs(hmask) - 1;
-            hmask &= hmask - 1;
-            if (at < ccap) cand[at] = e0 + (u64)it * PB + threadIdx.x;
-            ++at;
         }
         __syncthreads();
     }
@@ -667,7 +707,7 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, const u32 *__rest
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
 // with the D records of its bucket; every equal key emits a pair.
 __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const u32 *__restrict__ recL,
-                                                         const u32 *__restrict__ recI, const u64 *__restrict__ pbase,
+                                                         const u32 *__restrict__ recI, SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
                                                          int ninst, const u64 *__restrict__ bstart,
@@ -675,19 +715,21 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, 
                                                          unsigned long long *__restrict__ npairs, u64 cap,
                                                          u64 *__restrict__ out_d, u64 *__restrict__ out_i,
                                                          u64 *__restrict__ out_inst) {
-    __shared__ u64 s_base[257];
-    const int P = 1 << kh.lgP;
-    for (int i = threadIdx.x; i <= P; i += PB) s_base[i] = pbase[i];
-    __syncthreads();
     const u64 nc = *ncand < ccap ? *ncand : ccap;
     for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
         const u64 e = cand[c];
-        int a = 0, bb = P - 1;
-        while (a < bb) {
-            const int mid = (a + bb + 1) >> 1;
-            if (s_base[mid] <= e) a = mid; else bb = mid - 1;
+        u64 g;
+        if (S.O) {   // the last segment starting at or before e (it is not empty)
+            u64 a = 0, bb = S.nseg - 1;
+            while (a < bb) {
+                const u64 mid = (a + bb + 1) >> 1;
+                if (S.O[mid] <= e) a = mid; else bb = mid - 1;
+            }
+            g = a;
+        } else {
+            g = e >> S.capl;
         }
-        const u64 b = (u64)a * kh_k(kh) + (recL[e] >> kh.fpb);
+        const u64 b = (g / S.nblk) * kh_k(kh) + (recL[e] >> kh.fpb);
         u64 key, iid;
         leaf_exact(L, recI[e], key, iid);
         const u64 lo = bstart[b], hi = bstart[b + 1];
@@ -774,7 +816,7 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
 // order into DR (16-byte records), bucket starts into bstart (NB + 1), table slots into T (zeroed).
 bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
                 const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr) {
-    if (kh.lgR > CF_RANGE_BITS_LDS || kh.KR > CF_RANGE_BUCKETS_LDS) return false;
+    if (kh.lgR > CF_RANGE_BITS_LDS || kh.KR > CF_RANGE_BUCKETS_LDS || n >= (1ULL << 32)) return false;
     const int PD = 1 << kh.lgR;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
@@ -784,60 +826,92 @@ bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn
     DBuf<u32> H(ctx, (size_t)PD * nblk);
     DBuf<u64> O(ctx, (size_t)PD * nblk + 1);
     DBuf<u64> dstart(ctx, PD + 1);
-    DBuf<DRec3> tmp(ctx, n);
+    DBuf<u64> th(ctx, n);
+    DBuf<DRec> tkv(ctx, n);
     CF_LAUNCH(ctx, "k_drange_hist", k_drange_hist, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, H.p);
     scan_u32_to_u64_async(ctx, H.p, O.p, (size_t)PD * nblk);
     CF_LAUNCH(ctx, "k_part_bases", k_part_bases, dim3(1), dim3(PB), 0, O.p, nblk, PD, dstart.p);
-    CF_LAUNCH(ctx, "k_drange_scatter", k_drange_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, tmp.p);
-    CF_LAUNCH(ctx, "k_drange_buckets", k_drange_buckets, dim3(PD), dim3(PB), 0, kh, tmp.p, dstart.p, bstart, (DRec *)DR,
-              T, kerr);
+    CF_LAUNCH(ctx, "k_drange_scatter", k_drange_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, th.p, tkv.p);
+    CF_LAUNCH(ctx, "k_drange_buckets", k_drange_buckets, dim3(PD), dim3(PB), 0, kh, th.p, tkv.p, dstart.p, bstart,
+              (DRec *)DR, T, kerr);
     return true;
 }
 
-// Partition leaves [lo, lo + n) exactly: recL/recI get n records in partition order, pbase the
-// P + 1 partition starts.
-void part_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                  int depth, const KeyHash &kh, u64 *pbase, u32 *recL, u32 *recI) {
+// Capacity layout of a chunk of n I leaves: blocks of tpb tiles whose expected share per partition
+// is about 0.7 of a 2^9-slot segment (mean ~350, segment ~ mean + 8 sd).
+constexpr int CAPL = 9;
+static void cap_geometry(u64 n, const KeyHash &kh, u64 &tpb, u32 &nblk) {
+    const u64 ntiles = (n + LT - 1) / LT;
+    tpb = ((u64)7 << CAPL << kh.lgP) / (10 * (u64)LT);
+    if (tpb < 1) tpb = 1;
+    nblk = (u32)((ntiles + tpb - 1) / tpb);
+}
+
+u64 part_cap_slots(u64 n, const KeyHash &kh) {
+    u64 tpb;
+    u32 nblk;
+    cap_geometry(n, kh, tpb, nblk);
+    return ((u64)nblk << kh.lgP) << CAPL;
+}
+
+void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                      int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI, u32 *ovf) {
+    u64 tpb;
+    u32 nblk;
+    cap_geometry(n, kh, tpb, nblk);
+    lay.O.reset();
+    lay.nblk = nblk;
+    lay.nseg = (u64)nblk << kh.lgP;
+    lay.capl = CAPL;
+    lay.fill = DBuf<u32>(ctx, lay.nseg);
+    fill_u32(ctx, ovf, 1, 0);
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, (const u64 *)nullptr,
+              CAPL, lay.fill.p, ovf, recL, recI);
+}
+
+void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI) {
     const int P = 1 << kh.lgP;
-    if (!n) {
-        fill_u64(ctx, pbase, P + 1, 0);
-        return;
-    }
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     const u64 ntiles = (n + LT - 1) / LT;
     // about 8 resident blocks per CU over 256 CUs, each with a long contiguous leaf range
     const u64 tpb = (ntiles + 16383) / 16384;
     const u32 nblk = (u32)((ntiles + tpb - 1) / tpb);
-    DBuf<u32> H(ctx, (size_t)P * nblk);
-    DBuf<u64> O(ctx, (size_t)P * nblk + 1);
-    CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, H.p);
-    scan_u32_to_u64_async(ctx, H.p, O.p, (size_t)P * nblk);
-    CF_LAUNCH(ctx, "k_part_bases", k_part_bases, dim3(1), dim3(PB), 0, O.p, nblk, P, pbase);
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, recL, recI);
+    lay.nblk = nblk;
+    lay.nseg = (u64)P * nblk;
+    lay.capl = 0;
+    lay.fill = DBuf<u32>(ctx, lay.nseg);
+    lay.O = DBuf<u64>(ctx, lay.nseg + 1);
+    CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
+    scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
+    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+              (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, recL, recI);
 }
 
-void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const u32 *recL, const u64 *pbase, u64 nslots,
-                      const unsigned short *T, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand) {
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u32 *recL, const unsigned short *T,
+                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand) {
     fill_u32(ctx, xq, NXCD, 0);
     fill_u64(ctx, (u64 *)ncand, 1, 0);
-    if (!nslots) return;
+    if (!S.nseg) return;
     // tiles are dealt per XCD, so every XCD needs at least one block: the grid is a non-zero
     // multiple of the XCD count, about as many blocks as the chip holds at once
-    const u64 ntiles = (nslots + PT - 1) / PT;
+    const u64 ntiles = (S.nseg + SPT - 1) / SPT;
     unsigned g = (unsigned)(ntiles < 2048 ? ntiles : 2048);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, recL, pbase, T, xq, ncand, ccap, cand);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, S, recL, T, xq, ncand, ccap, cand);
 }
 
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const u32 *recL, const u32 *recI, const u64 *pbase,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u32 *recL, const u32 *recI,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, nullptr};
-    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, recL, recI, pbase, ncand,
-              ccap, cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, recL, recI, S, ncand, ccap,
+              cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out) {

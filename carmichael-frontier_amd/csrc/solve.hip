@@ -256,12 +256,12 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
     if (!chunk) {
         size_t free_b = 0, total_b = 0;
         CF_HIP(hipMemGetInfo(&free_b, &total_b));
-        chunk = std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 40, (u64)1 << 24));
+        chunk = std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 48, (u64)1 << 24));
     }
     CF_REQUIRE(chunk <= ((u64)1 << 32), "max_chunk above 2^32 leaves");
     u64 cap = 1 << 16;
     DBuf<u64> cnt_pairs(ctx, 1);
-    DBuf<u32> xq(ctx, 8);
+    DBuf<u32> xq(ctx, 8), ovf(ctx, 1);
     DBuf<u64> flags(ctx, 1);                // candidate count of the last probe
     DBuf<u32> kerr(ctx, 1);                 // set by a kernel that met an inconsistent record
     fill_u32(ctx, kerr.p, 1, 0);
@@ -311,15 +311,15 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
             if (f.total && nD) {
-                const int P = pt.P();
-                DBuf<u64> pbase(ctx, P + 1);
                 DBuf<u32> recL, recI;
                 DBuf<u64> cand;
+                IChunkLayout lay;
                 u64 cap_m = 0, ccap = 0;
                 for (u64 L = 0; L < f.total; L += chunk) {
                     const u64 m = std::min<u64>(chunk, f.total - L);
-                    if (cap_m < m) {
-                        cap_m = m;
+                    const u64 need = std::max<u64>(part_cap_slots(m, kh), m);
+                    if (cap_m < need) {
+                        cap_m = need;
                         recL = DBuf<u32>(ctx, cap_m);
                         recI = DBuf<u32>(ctx, cap_m);
                     }
@@ -327,20 +327,32 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                         ccap = std::max<u64>(m >> 10, 1 << 16);
                         cand = DBuf<u64>(ctx, ccap);
                     }
-                    part_scatter(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, pbase.p, recL.p,
-                                 recI.p);
-                    part_probe_pass1(ctx, kh, recL.p, pbase.p, m, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                    // one pass into segments sized for the expected share; the probe that follows
+                    // is wasted only if a segment overflowed (heavily repeated keys), and then the
+                    // chunk is laid out exactly and probed again
+                    part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay, recL.p,
+                                     recI.p, ovf.p);
+                    part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                      cand.p);
-                    const u64 nc = d2h_u64(ctx, flags.p);
+                    u32 ov = 0;
+                    CF_HIP(hipMemcpyAsync(&ov, ovf.p, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
+                    u64 nc = d2h_u64(ctx, flags.p);
+                    if (ov) {
+                        part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
+                                           recL.p, recI.p);
+                        part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                         cand.p);
+                        nc = d2h_u64(ctx, flags.p);
+                    }
                     if (nc > ccap) {
                         // more candidates than room (heavily repeated keys): probe again with room for all
                         ccap = nc;
                         cand = DBuf<u64>(ctx, ccap);
-                        part_probe_pass1(ctx, kh, recL.p, pbase.p, m, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                        part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p);
                     }
-                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, recL.p, recI.p,
-                                     pbase.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
+                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), recL.p,
+                                     recI.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
                                      DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }

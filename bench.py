@@ -28,9 +28,9 @@ sys.path.insert(0, os.path.join(ROOT, "carmichael-frontier_amd"))
 PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per I record of the two kernels that dominate a step:
 #   k_part_scatter writes each I record (8 bytes: bucket offset, fingerprint, leaf index) once;
-#   k_part_probe reads each record's 4-byte bucket word once and the first 16-byte half of the
+#   k_part_probe reads each 8-byte record once and the first 16-byte half of the
 #   probe-table bucket it names (the second half only for the few buckets holding more than 8).
-KERNEL_BYTES_PER_I = {"k_part_scatter": 8, "k_part_probe": 4 + 16}
+KERNEL_BYTES_PER_I = {"k_part_scatter": 8, "k_part_probe": 8 + 16}
 
 
 def run_batched(batch):

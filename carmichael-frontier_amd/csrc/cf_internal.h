@@ -254,15 +254,16 @@ struct IChunkLayout {   // the buffers behind a SegLayout
 u64 part_cap_slots(u64 n, const KeyHash &kh);
 // one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout)
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI, u32 *ovf);
+                      int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, u32 *ovf);
 // count, scan and scatter: n records exactly, in partition order
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI);
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec);
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
-void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u32 *recL, const unsigned short *T,
+// rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
                       u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const SegLayout &S, const u32 *recL, const u32 *recI,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst);

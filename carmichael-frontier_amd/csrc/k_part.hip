@@ -20,10 +20,10 @@
 //
 //  D side: records counted per bucket and scattered into bucket order (DRec = exact key, id); the
 //          scatter writes the table slots as it goes.
-//  I side: leaves scattered by partition as two 4-byte words (recL = bucket offset << fpb | f,
-//          recI = leaf index in the chunk): each block ranks its 2048 leaves by partition in LDS
+//  I side: leaves scattered by partition as 8-byte records (low word: bucket offset << fpb | f,
+//          high word: leaf index in the chunk): each block ranks its 2048 leaves by partition in LDS
 //          and writes one run per partition into slots sized from the expected share.
-//  Probe:  pass 1 reads recL only, in partition order: each of the 8 XCDs takes tiles of its own
+//  Probe:  pass 1 reads the records in partition order: each of the 8 XCDs takes tiles of its own
 //          contiguous slice from a queue, so an XCD works on about one partition at a time and
 //          that partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
 // HBM traffic per I record: 8 B written by the scatter + 4 B read by the probe.
@@ -487,8 +487,7 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
 // (hashing spreads records evenly) with a margin of about 8 standard deviations.
 __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                      const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
-                                                     u32 *__restrict__ ovf, u32 *__restrict__ recL,
-                                                     u32 *__restrict__ recI) {
+                                                     u32 *__restrict__ ovf, uint2 *__restrict__ rec) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
@@ -557,9 +556,7 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
                 }
                 g = (((u64)pi * nblk + blockIdx.x) << capl) + o;
             }
-            const uint2 r = s_r[q];
-            recL[g] = r.x;
-            recI[g] = r.y;
+            rec[g] = s_r[q];
         }
         __syncthreads();
         if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
@@ -631,7 +628,7 @@ constexpr int SPT = 8;             // segments per tile: two per wave
 
 __device__ __forceinline__ u64 seg_start(const SegLayout &S, u64 g) { return S.O ? S.O[g] : (g << S.capl); }
 
-__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, const u32 *__restrict__ recL,
+__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, const uint2 *__restrict__ recs,
                                                    const unsigned short *__restrict__ T, u32 *__restrict__ xq,
                                                    unsigned long long *__restrict__ ncand, u64 ccap,
                                                    u64 *__restrict__ cand) {
@@ -653,7 +650,7 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, cons
             const u64 g = t * SPT + (u64)wv * (SPT / (PB / 64)) + sj;
             if (g >= S.nseg) break;
             const u32 fill = S.fill[g];
-            const u32 *rec = recL + seg_start(S, g);
+            const uint2 *rec = recs + seg_start(S, g);
             const u64 bpart = (g / S.nblk) * K;
             for (u32 v0 = 0; v0 < fill; v0 += 64 * PROBE_BATCH) {
                 u32 lo[PROBE_BATCH];
@@ -662,7 +659,7 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, cons
 #pragma unroll
                 for (int q = 0; q < PROBE_BATCH; ++q) {
                     const u32 v = v0 + (u32)q * 64 + lane;
-                    lo[q] = v < fill ? rec[v] : 0u;
+                    lo[q] = v < fill ? rec[v].x : 0u;
                 }
 #pragma unroll
                 for (int q = 0; q < PROBE_BATCH; ++q)
@@ -706,8 +703,8 @@ __global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, cons
 
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
 // with the D records of its bucket; every equal key emits a pair.
-__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const u32 *__restrict__ recL,
-                                                         const u32 *__restrict__ recI, SegLayout S,
+__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const uint2 *__restrict__ recs,
+                                                         SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
                                                          int ninst, const u64 *__restrict__ bstart,
@@ -729,9 +726,10 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, 
         } else {
             g = e >> S.capl;
         }
-        const u64 b = (g / S.nblk) * kh_k(kh) + (recL[e] >> kh.fpb);
+        const uint2 r = recs[e];
+        const u64 b = (g / S.nblk) * kh_k(kh) + (r.x >> kh.fpb);
         u64 key, iid;
-        leaf_exact(L, recI[e], key, iid);
+        leaf_exact(L, r.y, key, iid);
         const u64 lo = bstart[b], hi = bstart[b + 1];
         for (u64 q = lo; q < hi; ++q) {
             const DRec d = DR[q];
@@ -855,7 +853,7 @@ u64 part_cap_slots(u64 n, const KeyHash &kh) {
 }
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI, u32 *ovf) {
+                      int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, u32 *ovf) {
     u64 tpb;
     u32 nblk;
     cap_geometry(n, kh, tpb, nblk);
@@ -868,11 +866,11 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, (const u64 *)nullptr,
-              CAPL, lay.fill.p, ovf, recL, recI);
+              CAPL, lay.fill.p, ovf, (uint2 *)rec);
 }
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u32 *recL, u32 *recI) {
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec) {
     const int P = 1 << kh.lgP;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
@@ -888,10 +886,10 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
     CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
-              (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, recL, recI);
+              (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
 }
 
-void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u32 *recL, const unsigned short *T,
+void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
                       u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand) {
     fill_u32(ctx, xq, NXCD, 0);
     fill_u64(ctx, (u64 *)ncand, 1, 0);
@@ -901,16 +899,18 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
     const u64 ntiles = (S.nseg + SPT - 1) / SPT;
     unsigned g = (unsigned)(ntiles < 2048 ? ntiles : 2048);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, S, recL, T, xq, ncand, ccap, cand);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, S, (const uint2 *)rec, T, xq, ncand, ccap,
+              cand);
 }
 
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const SegLayout &S, const u32 *recL, const u32 *recI,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, nullptr};
-    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, recL, recI, S, ncand, ccap,
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, (const uint2 *)rec, S,
+              ncand, ccap,
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
 }
 

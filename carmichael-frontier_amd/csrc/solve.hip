@@ -311,7 +311,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
             if (f.total && nD) {
-                DBuf<u32> recL, recI;
+                DBuf<u64> rec;
                 DBuf<u64> cand;
                 IChunkLayout lay;
                 u64 cap_m = 0, ccap = 0;
@@ -320,8 +320,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     const u64 need = std::max<u64>(part_cap_slots(m, kh), m);
                     if (cap_m < need) {
                         cap_m = need;
-                        recL = DBuf<u32>(ctx, cap_m);
-                        recI = DBuf<u32>(ctx, cap_m);
+                        rec = DBuf<u64>(ctx, cap_m);
                     }
                     if (ccap < std::max<u64>(m >> 10, 1 << 16)) {
                         ccap = std::max<u64>(m >> 10, 1 << 16);
@@ -330,17 +329,17 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     // one pass into segments sized for the expected share; the probe that follows
                     // is wasted only if a segment overflowed (heavily repeated keys), and then the
                     // chunk is laid out exactly and probed again
-                    part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay, recL.p,
-                                     recI.p, ovf.p);
-                    part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                    part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay, rec.p,
+                                     ovf.p);
+                    part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                      cand.p);
                     u32 ov = 0;
                     CF_HIP(hipMemcpyAsync(&ov, ovf.p, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
                     u64 nc = d2h_u64(ctx, flags.p);
                     if (ov) {
                         part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                           recL.p, recI.p);
-                        part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                           rec.p);
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p);
                         nc = d2h_u64(ctx, flags.p);
                     }
@@ -348,11 +347,11 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                         // more candidates than room (heavily repeated keys): probe again with room for all
                         ccap = nc;
                         cand = DBuf<u64>(ctx, ccap);
-                        part_probe_pass1(ctx, kh, lay.view(), recL.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p);
                     }
-                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), recL.p,
-                                     recI.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
+                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
+                                     (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
                                      DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }

@@ -4,8 +4,9 @@
 Workload (default): the lambda-family batch -- every ksmall instance of the 118 divisor moduli of
 lambda(N) = 155 016 423 578 016 000 (bound U = N) at radii 4..7 with r <= rmax(D): 280 instances,
 1.60e9 keys (nD + nI), 79 pairs. One step = one pass of the join over the whole batch:
-  batched -- all 280 instances in one solve (cf_batch_solve), the timed value;
-  plain   -- one solve per instance, timed beside it.
+  batched -- all 280 instances in one solve, captured once into a HIP graph (cf_batch_capture)
+             and replayed as one graph launch per step, the timed value;
+  plain   -- one solve per instance, each an eager sequence of launches, timed beside it.
 Both legs prepare their instances first (descriptors and exact bound tables uploaded by
 cf_batch_create), so the timed region starts with every input resident in HBM.
 Every instance's (nD, nI, pairs) is checked against the reference's logged counts, and the pair sets of
@@ -90,6 +91,7 @@ def main():
     ap.add_argument("--family", default="lambdaN", choices=["lambdaN", "M0"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain leg (profiling the batched path alone)")
+    ap.add_argument("--no-graph", action="store_true", help="time the batched leg as eager launches, not a graph")
     ap.add_argument("--cpu-keys", type=float, default=8.0e8, help="CPU-baseline sample size in keys")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="after the run, one device-to-device copy of BYTES (calibrates byte counters)")
@@ -122,27 +124,35 @@ def main():
     print(f"[bench] warm-up done: {keys:.4g} keys, logged-count mismatches batched={len(bad_b)} plain={len(bad_p)}, "
           f"batched == plain pair sets: {same}", file=sys.stderr, flush=True)
 
-    # ---- kernel breakdown: one untimed step with every kernel bracketed by HIP events; it names
-    # the dominant kernel (the one of the two I-record kernels that takes longer)
+    # ---- kernel breakdown: one untimed eager step with every kernel bracketed by HIP events on the
+    # stream it runs on; it names the dominant kernel (the one of the two I-record kernels that
+    # takes longer) and gives its launch times (a graph replay carries no per-kernel events; the
+    # replayed launches are the same kernels on the same data, checked with rocprofv3)
     ctx.prof_enable(None)
     ctx.prof_reset()
     run_batched(batch)
     breakdown = ctx.prof_dump()
     ctx.prof_disable()
     dominant = max(KERNEL_BYTES_PER_I, key=lambda k: breakdown.get(k, (0.0, 0))[0])
+    k_ms, k_launches = breakdown.get(dominant, (0.0, 0))
 
-    # ---- timed: batched, with HIP events on the dominant kernel only (on the stream it runs on)
-    ctx.prof_enable(dominant)
-    ctx.prof_reset()
+    # ---- the persistent launch: the batch's solve captured into one HIP graph; one replay is
+    # checked against the eager result
+    graph_same = None
+    if not args.no_graph:
+        batch.capture()
+        res_g, _ = run_batched(batch)
+        graph_same = all((a.nD, a.nI) == (b.nD, b.nI) and (a.pair_array() == b.pair_array()).all()
+                         for a, b in zip(res_b, res_g))
+    print(f"[bench] graph replay == eager: {graph_same}", file=sys.stderr, flush=True)
+
+    # ---- timed: batched
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run_batched(batch)
     ctx.synchronize()
     t1 = time.perf_counter()
-    kern = ctx.prof_dump()
-    ctx.prof_disable()
-    k_ms, k_launches = kern.get(dominant, (0.0, 0))
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = keys / ((t1 - t0) / args.steps)
 
@@ -159,12 +169,13 @@ def main():
 
     per_rec = KERNEL_BYTES_PER_I[dominant]
     bytes_per_step = per_rec * nI_total
-    achieved = bytes_per_step * args.steps / (k_ms / 1e3) / 1e9 if k_ms else 0.0
+    achieved = bytes_per_step / (k_ms / 1e3) / 1e9 if k_ms else 0.0
     roofline = dict(kernel=dominant, bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                     frac=round(achieved / PEAK_HBM_GBS, 4), traffic=None,
                     algorithmic=f"{per_rec} B per I record x {nI_total} I records per step",
-                    avg_launch_ms=round(k_ms / max(k_launches, 1), 4), launches_per_step=k_launches // args.steps,
-                    kernel_ms_per_step=round(k_ms / args.steps, 3),
+                    avg_launch_ms=round(k_ms / max(k_launches, 1), 4), launches_per_step=k_launches,
+                    kernel_ms_per_step=round(k_ms, 3),
+                    kernel_timing="HIP events around each launch on its stream, eager profiled step",
                     breakdown_ms_per_step={k: round(v[0], 3) for k, v in
                                            sorted(breakdown.items(), key=lambda kv: -kv[1][0])})
 
@@ -193,7 +204,9 @@ def main():
         "batched_vs_plain": value / plain_value if plain_value else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "graph": not args.no_graph,
         "verify": {"logged_count_mismatches_batched": len(bad_b),
+                   "graph_replay_equals_eager": graph_same,
                    "logged_count_mismatches_plain": None if args.no_plain else len(bad_p),
                    "batched_equals_plain": None if args.no_plain else bool(same),
                    "pairs": sum(len(r.pairs_d) for r in res_b)},

@@ -88,6 +88,7 @@ SIGNATURES = {
     "cf_batch_solve": (c_int, [c_void_p, c_u64, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
                                ctypes.POINTER(c_void_p), P_u64, P_u64, ctypes.POINTER(cf_stats)]),
     "cf_batch_destroy": (c_int, [c_void_p]),
+    "cf_batch_capture": (c_int, [c_void_p, c_u64]),
 }
 
 

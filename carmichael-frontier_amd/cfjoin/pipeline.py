@@ -98,6 +98,12 @@ class Batch:
         L.check(L.load().cf_batch_create(self.ctx.handle, arr, n, ctypes.byref(h)), "cf_batch_create")
         self.handle = h
 
+    def capture(self, max_chunk: int = 0) -> None:
+        """Capture the batch's solve into a HIP graph (cf_batch_capture): later solve() calls with
+        max_chunk 0 (or the captured one) are one graph launch each."""
+        L.check(L.load().cf_batch_capture(self.handle, max_chunk), "cf_batch_capture")
+        self.captured = True
+
     def solve(self, max_chunk: int = 0):
         """Returns (list of SolveResult in input order, stats dict)."""
         ctx = self.ctx

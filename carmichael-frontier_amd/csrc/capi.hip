@@ -32,6 +32,15 @@ size_t g_peak_live = 0;
 void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     const size_t sz = pool_round(bytes);
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->arena) {
+        auto a = ctx->arena->free.lower_bound(sz);
+        if (a != ctx->arena->free.end() && a->first <= sz + sz / 2) {
+            void *p = a->second;
+            ctx->live[p] = a->first;
+            ctx->arena->free.erase(a);
+            return p;
+        }
+    }
     auto it = ctx->free_blocks.lower_bound(sz);
     if (it != ctx->free_blocks.end() && it->first <= sz + sz / 2) {
         void *p = it->second;
@@ -39,10 +48,15 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
         ctx->free_blocks.erase(it);
         ctx->cached_bytes -= bs;
         ctx->live[p] = bs;
+        if (ctx->arena) ctx->arena->owned[p] = bs;
         return p;
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess && ctx->arena) {
+        (void)hipGetLastError();
+        throw CfError{CF_ENOMEM, "device out of memory allocating " + std::to_string(sz) + " bytes during capture"};
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         // give the cache back to the device and retry once
@@ -56,6 +70,7 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
         }
     }
     ctx->live[p] = sz;
+    if (ctx->arena) ctx->arena->owned[p] = sz;
     size_t live = 0;
     for (auto &kv : ctx->live) live += kv.second;
     g_peak_live = std::max(g_peak_live, live);
@@ -67,9 +82,26 @@ void ctx_free(cf_ctx *ctx, void *p) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->live.find(p);
     if (it == ctx->live.end()) return;
+    if (ctx->arena) {
+        ctx->arena->free.emplace(it->second, p);
+        ctx->arena->owned[p] = it->second;
+        ctx->live.erase(it);
+        return;
+    }
     ctx->free_blocks.emplace(it->second, p);
     ctx->cached_bytes += it->second;
     ctx->live.erase(it);
+}
+
+void arena_release(cf_ctx *ctx, CaptureArena *a) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto &kv : a->owned) {
+        ctx->live.erase(kv.first);
+        ctx->free_blocks.emplace(kv.second, kv.first);
+        ctx->cached_bytes += kv.second;
+    }
+    a->owned.clear();
+    a->free.clear();
 }
 
 void ctx_release(cf_ctx *ctx) {
@@ -92,7 +124,8 @@ int bitlen(u64 v) { return v ? 64 - __builtin_clzll(v) : 0; }
 
 // ---- kernel timing ---------------------------------------------------------------------------
 static bool prof_selected(cf_ctx *ctx, const char *name) {
-    return ctx->prof_on && (ctx->prof_filter.empty() || ctx->prof_filter == name);
+    // no events inside a captured graph: replays do not time them reliably
+    return ctx->prof_on && !ctx->arena && (ctx->prof_filter.empty() || ctx->prof_filter == name);
 }
 static hipEvent_t prof_event(cf_ctx *ctx) {
     if (!ctx->prof_free.empty()) {

@@ -65,7 +65,18 @@ struct cf_ctx {
     std::vector<ProfRec> prof_pending;
     std::vector<hipEvent_t> prof_free;
     std::map<std::string, std::pair<double, uint64_t>> prof_totals;   // name -> (ms, launches)
+    // stream capture into a HIP graph: while set, blocks freed go to the arena (reused only by
+    // later allocations of the same capture) and every block handed out is recorded in it, so
+    // nothing a graph node touches returns to the shared pool while the graph lives
+    struct CaptureArena *arena = nullptr;
 };
+
+struct CaptureArena {
+    std::multimap<size_t, void *> free;   // freed during the capture
+    std::map<void *, size_t> owned;       // every block the captured work uses
+};
+// return an arena's blocks to its context's pool (the graph that used them is gone)
+void arena_release(cf_ctx *ctx, CaptureArena *a);
 
 void cf_prof_begin(cf_ctx *ctx, const char *name, hipEvent_t *ev);
 void cf_prof_end(cf_ctx *ctx, const char *name, hipEvent_t ev);

@@ -30,28 +30,63 @@ struct Frontier {
     u64 total = 0;
 };
 
-// Expand the given depth-0 roots (one radius r) to the last level.
-Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const std::vector<Node> &roots, int r) {
+// Values the host reads back from the device during a solve (level sizes, overflow flags,
+// candidate and pair counts) decide allocation sizes and control flow. A tape records them in an
+// eager solve and plays them back while the same solve is captured into a HIP graph, where nothing
+// may be read back; the solve is deterministic, so the captured work is the recorded work.
+struct Tape {
+    enum Mode { OFF, RECORD, REPLAY } mode = OFF;
+    std::vector<u64> v;
+    size_t pos = 0;
+    bool replay() const { return mode == REPLAY; }
+};
+
+u64 host_value(cf_ctx *ctx, Tape *t, const u64 *dev) {
+    if (t && t->replay()) {
+        CF_REQUIRE(t->pos < t->v.size(), "captured solve asked for more values than were recorded");
+        return t->v[t->pos++];
+    }
+    const u64 x = d2h_u64(ctx, dev);
+    if (t && t->mode == Tape::RECORD) t->v.push_back(x);
+    return x;
+}
+u64 host_value_u32(cf_ctx *ctx, Tape *t, const u32 *dev) {
+    if (t && t->replay()) return host_value(ctx, t, nullptr);
+    u32 x = 0;
+    d2h(ctx, &x, dev, 1);
+    if (t && t->mode == Tape::RECORD) t->v.push_back(x);
+    return x;
+}
+
+// Expand the given depth-0 roots (one radius r, on the device) to the last level.
+Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const Node *roots, size_t nroots, int r, Tape *tape) {
     Frontier f;
-    if (roots.empty()) return f;
-    DBuf<Node> cur(ctx, roots.size());
-    h2d(ctx, cur.p, roots.data(), roots.size());
-    size_t ncur = roots.size();
+    if (!nroots) return f;
+    DBuf<Node> cur;
+    const Node *curp = roots;
+    size_t ncur = nroots;
     for (int depth = 0; depth + 1 < r; ++depth) {
         DBuf<u32> c(ctx, ncur);
-        launch_expand_count(ctx, d_desc, cur.p, ncur, depth, c.p);
+        launch_expand_count(ctx, d_desc, curp, ncur, depth, c.p);
         DBuf<u64> o(ctx, ncur + 1);
-        const u64 tot = scan_u32_to_u64(ctx, c.p, o.p, ncur);
+        scan_u32_to_u64_async(ctx, c.p, o.p, ncur);
+        const u64 tot = host_value(ctx, tape, o.p + ncur);
         if (tot == 0) return f;
         DBuf<Node> nx(ctx, tot);
-        launch_expand_write(ctx, d_desc, cur.p, ncur, o.p, tot, depth, nx.p);
+        launch_expand_write(ctx, d_desc, curp, ncur, o.p, tot, depth, nx.p);
         cur = std::move(nx);
+        curp = cur.p;
         ncur = tot;
+    }
+    if (!cur.p) {   // r == 1: the roots are the last level
+        cur = DBuf<Node>(ctx, ncur);
+        CF_HIP(hipMemcpyAsync(cur.p, roots, ncur * sizeof(Node), hipMemcpyDeviceToDevice, ctx->stream));
     }
     f.cnt = DBuf<u32>(ctx, ncur);
     launch_expand_count(ctx, d_desc, cur.p, ncur, r - 1, f.cnt.p);
     f.off = DBuf<u64>(ctx, ncur + 1);
-    f.total = scan_u32_to_u64(ctx, f.cnt.p, f.off.p, ncur);
+    scan_u32_to_u64_async(ctx, f.cnt.p, f.off.p, ncur);
+    f.total = host_value(ctx, tape, f.off.p + ncur);
     f.nodes = std::move(cur);
     f.nn = ncur;
     return f;
@@ -131,7 +166,10 @@ SideSpec spec_of_i(const cf_instance &in, u64 koff) {
 namespace {
 
 struct BatchOut {
-    DBuf<u64> pd, pi, pinst;
+    DBuf<u64> pd, pi, pinst;          // pairs: D id, I id, instance
+    DBuf<u64> dcount, icount;         // records per instance
+    DBuf<u64> cnt_pairs;              // pairs found (may exceed the capacity of pd)
+    DBuf<u32> kerr;
     u64 npairs = 0;
     std::vector<u64> nD, nI;
 };
@@ -175,6 +213,7 @@ Partitioning assign_buckets(u64 nD) {
 
 // A prepared batch: every instance validated, its exact bound tables computed on the host and
 // uploaded in one allocation; solving it is device work only.
+struct BatchGraph;
 struct cf_batch {
     cf_ctx *ctx = nullptr;
     int n = 0;
@@ -185,6 +224,11 @@ struct cf_batch {
     DBuf<u64> store;
     DBuf<SideDesc> d_desc, i_desc;
     DBuf<u64> koffs;
+    std::map<int, DBuf<Node>> droots, iroots;    // per radius: the enumeration roots of live sides
+    std::map<int, size_t> nd_roots, ni_roots;
+    u64 pair_cap = 1 << 16;                      // pair capacity of the next solve (grows to fit)
+    BatchGraph *graph = nullptr;                 // the captured solve, once cf_batch_capture ran
+    ~cf_batch();
 };
 
 namespace {
@@ -232,38 +276,56 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
     h2d(ctx, b->d_desc.p, b->dd.data(), n);
     h2d(ctx, b->i_desc.p, b->id.data(), n);
     h2d(ctx, b->koffs.p, b->koff.data(), n);
+    // enumeration roots, resident for every solve of the batch
+    for (auto &g : b->groups) {
+        std::vector<Node> rd, ri;
+        for (int j : g.second) {
+            if (b->d_live[j]) rd.push_back(root_node(b->dd[j], SIDE_D, j));
+            if (b->i_live[j]) ri.push_back(root_node(b->id[j], SIDE_I, j));
+        }
+        b->nd_roots[g.first] = rd.size();
+        b->ni_roots[g.first] = ri.size();
+        b->droots[g.first] = DBuf<Node>(ctx, rd.size() ? rd.size() : 1);
+        b->iroots[g.first] = DBuf<Node>(ctx, ri.size() ? ri.size() : 1);
+        h2d(ctx, b->droots[g.first].p, rd.data(), rd.size());
+        h2d(ctx, b->iroots[g.first].p, ri.data(), ri.size());
+    }
     CF_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_lds = true) {
+// I leaves per chunk: as many as fit comfortably (the record index in a chunk is 32 bits); one
+// chunk lets each XCD's probe stay inside one partition for a long stretch
+u64 default_chunk() {
+    size_t free_b = 0, total_b = 0;
+    CF_HIP(hipMemGetInfo(&free_b, &total_b));
+    return std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 48, (u64)1 << 24));
+}
+
+// tape: null for a plain eager solve; RECORD to note every value read back; REPLAY while the
+// stream is being captured (no value is read back and no timing is taken)
+void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_lds = true, Tape *tape = nullptr) {
     cf_ctx *ctx = b->ctx;
     const int n = b->n;
-    const auto &dd = b->dd;
-    const auto &id = b->id;
-    const auto &d_live = b->d_live;
-    const auto &i_live = b->i_live;
     const auto &groups = b->groups;
     const DBuf<SideDesc> &d_desc = b->d_desc;
     const DBuf<SideDesc> &i_desc = b->i_desc;
     const DBuf<u64> &koffs = b->koffs;
+    const bool capturing = tape && tape->replay();
 
     PhaseClock clk(ctx);
-    clk.mark();  // 0
-    DBuf<u64> dcount(ctx, n), icount(ctx, n);
-    // I leaves per chunk: as many as fit comfortably (the record index in a chunk is 32 bits); one
-    // chunk lets each XCD's probe stay inside one partition for a long stretch
-    u64 chunk = max_chunk;
-    if (!chunk) {
-        size_t free_b = 0, total_b = 0;
-        CF_HIP(hipMemGetInfo(&free_b, &total_b));
-        chunk = std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 48, (u64)1 << 24));
-    }
+    if (!capturing) clk.mark();  // 0
+    out.dcount = DBuf<u64>(ctx, n);
+    out.icount = DBuf<u64>(ctx, n);
+    DBuf<u64> &dcount = out.dcount, &icount = out.icount;
+    const u64 chunk = max_chunk ? max_chunk : default_chunk();
     CF_REQUIRE(chunk <= ((u64)1 << 32), "max_chunk above 2^32 leaves");
-    u64 cap = 1 << 16;
-    DBuf<u64> cnt_pairs(ctx, 1);
+    u64 cap = b->pair_cap;
+    out.cnt_pairs = DBuf<u64>(ctx, 1);
+    out.kerr = DBuf<u32>(ctx, 1);           // set by a kernel that met an inconsistent record
+    DBuf<u64> &cnt_pairs = out.cnt_pairs;
+    DBuf<u32> &kerr = out.kerr;
     DBuf<u32> xq(ctx, 8), ovf(ctx, 1);
     DBuf<u64> flags(ctx, 1);                // candidate count of the last probe
-    DBuf<u32> kerr(ctx, 1);                 // set by a kernel that met an inconsistent record
     fill_u32(ctx, kerr.p, 1, 0);
     double ms_d = 0, ms_i = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -278,11 +340,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         // side is generated, partitioned and probed against it
         for (auto &g : groups) {
             PhaseClock gc(ctx);
-            gc.mark();
-            std::vector<Node> roots;
-            for (int j : g.second)
-                if (d_live[j]) roots.push_back(root_node(dd[j], SIDE_D, j));
-            Frontier fd = expand_to_last(ctx, d_desc.p, roots, g.first);
+            if (!capturing) gc.mark();
+            Frontier fd = expand_to_last(ctx, d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first, tape);
             const u64 nD = fd.total;
             if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
             const Partitioning pt = assign_buckets(nD);
@@ -302,13 +361,10 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                                     bcount.p, DR.p, T.p);
             }
             fd = Frontier();
-            gc.mark();
+            if (!capturing) gc.mark();
             // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
             // scatter), probe the table, check the candidates exactly
-            roots.clear();
-            for (int j : g.second)
-                if (i_live[j]) roots.push_back(root_node(id[j], SIDE_I, j));
-            Frontier f = expand_to_last(ctx, i_desc.p, roots, g.first);
+            Frontier f = expand_to_last(ctx, i_desc.p, b->iroots.at(g.first).p, b->ni_roots.at(g.first), g.first, tape);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
             if (f.total && nD) {
                 DBuf<u64> rec;
@@ -333,15 +389,14 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                                      ovf.p);
                     part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                      cand.p);
-                    u32 ov = 0;
-                    CF_HIP(hipMemcpyAsync(&ov, ovf.p, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
-                    u64 nc = d2h_u64(ctx, flags.p);
+                    const u64 ov = host_value_u32(ctx, tape, ovf.p);
+                    u64 nc = host_value(ctx, tape, flags.p);
                     if (ov) {
                         part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
                                            rec.p);
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p);
-                        nc = d2h_u64(ctx, flags.p);
+                        nc = host_value(ctx, tape, flags.p);
                     }
                     if (nc > ccap) {
                         // more candidates than room (heavily repeated keys): probe again with room for all
@@ -355,18 +410,21 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                                      DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }
-            gc.mark();
-            ms_d += gc.ms(0, 1);
-            ms_i += gc.ms(1, 2);
+            if (!capturing) {
+                gc.mark();
+                ms_d += gc.ms(0, 1);
+                ms_i += gc.ms(1, 2);
+            }
         }
-        const u64 np = d2h_u64(ctx, cnt_pairs.p);
-        u32 ke = 0;
-        d2h(ctx, &ke, kerr.p, 1);
-        if (ke) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        const u64 np = host_value(ctx, tape, cnt_pairs.p);
+        if (host_value_u32(ctx, tape, kerr.p))
+            throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
         out.npairs = np;
         if (np <= cap) break;
         cap = np;
     }
+    b->pair_cap = std::max<u64>(b->pair_cap, out.npairs);
+    if (capturing) return;
     clk.mark();  // 1
     out.nD.assign(n, 0);
     out.nI.assign(n, 0);
@@ -392,7 +450,88 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
 
 }  // namespace
 
+// A batch's solve captured into a HIP graph: every buffer its nodes touch lives in the arena
+// until the batch is destroyed; out holds the pair and count buffers each replay refills.
+struct BatchGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CaptureArena arena;
+    BatchOut out;
+    u64 npairs = 0;
+    u64 chunk = 0;
+};
+
+static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
+    (void)hipStreamSynchronize(ctx->stream);
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+    // the output buffers are arena blocks: hand them back with the rest of the arena
+    G->out.pd.release();
+    G->out.pi.release();
+    G->out.pinst.release();
+    G->out.dcount.release();
+    G->out.icount.release();
+    G->out.cnt_pairs.release();
+    G->out.kerr.release();
+    arena_release(ctx, &G->arena);
+    delete G;
+}
+
+cf_batch::~cf_batch() {
+    if (graph) graph_destroy(ctx, graph);
+}
+
 extern "C" {
+
+int cf_batch_capture(cf_batch *b, uint64_t max_chunk) {
+    API_BEGIN
+    CF_REQUIRE(b, "NULL argument");
+    CF_REQUIRE(!b->graph, "the batch is already captured");
+    cf_ctx *ctx = b->ctx;
+    const u64 chunk = max_chunk ? max_chunk : default_chunk();
+    {   // settles the pair capacity, so the recorded solve makes a single attempt
+        BatchOut warm;
+        solve_part(b, warm, nullptr, chunk);
+    }
+    Tape tape;
+    tape.mode = Tape::RECORD;
+    {
+        BatchOut rec;
+        solve_part(b, rec, nullptr, chunk, true, &tape);
+    }
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    std::unique_ptr<BatchGraph> G(new BatchGraph());
+    G->chunk = chunk;
+    tape.mode = Tape::REPLAY;
+    tape.pos = 0;
+    ctx->arena = &G->arena;
+    CF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    try {
+        solve_part(b, G->out, nullptr, chunk, true, &tape);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        ctx->arena = nullptr;
+        graph_destroy(ctx, G.release());
+        throw;
+    }
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &G->graph);
+    ctx->arena = nullptr;
+    if (ec != hipSuccess || tape.pos != tape.v.size()) {
+        graph_destroy(ctx, G.release());
+        CF_HIP(ec);
+        throw CfError{CF_EINTERNAL, "the captured solve read back fewer values than the recorded one"};
+    }
+    G->npairs = G->out.npairs;
+    const hipError_t ei = hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        graph_destroy(ctx, G.release());
+        CF_HIP(ei);
+    }
+    b->graph = G.release();
+    API_END
+}
 
 int cf_batch_create(cf_ctx *ctx, const cf_instance *insts, int n, cf_batch **out) {
     API_BEGIN
@@ -411,6 +550,55 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
     *out_d = *out_i = *out_inst = nullptr;
     *npairs = 0;
     g_peak_live = 0;
+    BatchGraph *G = b->graph;
+    if (G && (!max_chunk || max_chunk == G->chunk)) {
+        // replay the captured solve: one graph launch, then the few results are read back
+        cf_ctx *ctx = b->ctx;
+        hipEvent_t e0, e1;
+        CF_HIP(hipEventCreate(&e0));
+        CF_HIP(hipEventCreate(&e1));
+        CF_HIP(hipEventRecord(e0, ctx->stream));
+        CF_HIP(hipGraphLaunch(G->exec, ctx->stream));
+        CF_HIP(hipEventRecord(e1, ctx->stream));
+        const u64 np = d2h_u64(ctx, G->out.cnt_pairs.p);
+        u32 ke = 0;
+        d2h(ctx, &ke, G->out.kerr.p, 1);
+        float ms = 0;
+        CF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ke) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        if (np != G->npairs) throw CfError{CF_EINTERNAL, "a replay of the captured solve found a different pair count"};
+        std::vector<u64> nD(b->n), nI(b->n);
+        d2h(ctx, nD.data(), G->out.dcount.p, b->n);
+        d2h(ctx, nI.data(), G->out.icount.p, b->n);
+        if (np) {
+            DBuf<u64> pd(ctx, np), pi(ctx, np), pj(ctx, np);
+            CF_HIP(hipMemcpyAsync(pd.p, G->out.pd.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(pi.p, G->out.pi.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(pj.p, G->out.pinst.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipStreamSynchronize(ctx->stream));
+            *out_d = pd.release();
+            *out_i = pi.release();
+            *out_inst = pj.release();
+        }
+        *npairs = np;
+        if (inst_counts)
+            for (int j = 0; j < b->n; ++j) {
+                inst_counts[2 * j] = nD[j];
+                inst_counts[2 * j + 1] = nI[j];
+            }
+        if (stats) {
+            *stats = cf_stats{};
+            for (int j = 0; j < b->n; ++j) {
+                stats->nD += nD[j];
+                stats->nI += nI[j];
+            }
+            stats->npairs = np;
+            stats->ms_total = ms;
+        }
+        return CF_OK;
+    }
     BatchOut bo;
     solve_part(b, bo, stats, max_chunk);
     if (bo.npairs) {

@@ -178,6 +178,15 @@ int cf_batch_solve(cf_batch *batch, uint64_t max_chunk, uint64_t **out_d_dev, ui
                    uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);
 int cf_batch_destroy(cf_batch *batch);
 
+/* The batch's persistent launch: one eager solve notes every value the host reads back during a
+ * solve (level sizes, candidate and pair counts), then the whole solve is captured into a HIP
+ * graph with those values, and every later cf_batch_solve (with max_chunk 0 or the captured one)
+ * is a single graph launch followed by the read-back of the pairs -- no host round trip inside
+ * the solve. The graph keeps its device buffers until cf_batch_destroy. The reference launches
+ * every kernel of every instance from Python (gpu/mitm_gpu_mstar_driver.py:209-243); this has
+ * no counterpart there. */
+int cf_batch_capture(cf_batch *batch, uint64_t max_chunk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -321,3 +321,32 @@ def test_solve_primes_above_32_bits():
     both, _ = pipeline.solve_batch([inst, small])
     assert np.array_equal(both[0].pair_array(), res.pair_array())
     assert np.array_equal(both[1].pair_array(), pipeline.solve(small).pair_array())
+
+
+def test_batch_graph_replays_match_eager(workloads):
+    """The captured batch (one HIP graph launch per solve) returns, on every replay, the pair sets and
+    record counts of the eager solve -- with eager work and synchronizations between the replays.
+    The batch mixes ordinary instances with one whose keys repeat (its chunk overflows the capacity
+    layout and its probe outgrows the first candidate allowance, so those paths are captured too)."""
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    P = [p for p in range(3, 2000) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
+    collide = I.Instance(M=1024, r=3, B0=P[:20], INS=P[20:80], P0=prod(P[:20]), Icap=prod(P[77:80]) + 1, Dmin=1,
+                         split=None, hmax=3, tag="collide1024")
+    insts = W.batch_instances("lambdaN", radii=(4, 5, 6), logged=logged, limit=30)
+    insts += [I.c1_instance(0, 100), collide]
+    say("graph", len(insts))
+    eager, _ = pipeline.solve_batch(insts)
+    other = I.c1_instance(1, None)
+    for max_chunk in (0, 50_000):
+        b = pipeline.Batch(insts)
+        b.capture(max_chunk)
+        for rep in range(3):
+            res, st = b.solve(max_chunk)
+            assert len(res) == len(insts)
+            for inst, ra, rb in zip(insts, eager, res):
+                assert (rb.nD, rb.nI) == (ra.nD, ra.nI), (max_chunk, rep, inst.tag)
+                assert np.array_equal(rb.pair_array(), ra.pair_array()), (max_chunk, rep, inst.tag)
+            pipeline.solve(other)            # eager work between replays
+            L.default_context().synchronize()
+        b.close()

@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain leg (profiling the batched path alone)")
     ap.add_argument("--no-graph", action="store_true", help="time the batched leg as eager launches, not a graph")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (skip the autotuner)")
     ap.add_argument("--cpu-keys", type=float, default=8.0e8, help="CPU-baseline sample size in keys")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="after the run, one device-to-device copy of BYTES (calibrates byte counters)")
@@ -123,6 +124,16 @@ def main():
     nI_total = sum(r.nI for r in res_b)
     print(f"[bench] warm-up done: {keys:.4g} keys, logged-count mismatches batched={len(bad_b)} plain={len(bad_p)}, "
           f"batched == plain pair sets: {same}", file=sys.stderr, flush=True)
+
+    # ---- autotuner (untimed): slice per partition, D records per bucket, segment fill, probe grid
+    tuning = None
+    if not args.no_tune:
+        from cfjoin import autotune
+        tr = autotune.tune(batch, reps=1)
+        tuning = {"setting": tr["best"], "ms": round(tr["ms"], 3), "default_ms": round(tr["default_ms"], 3),
+                  "trials": len(tr["trials"])}
+        print(f"[bench] tuned: {tr['best']} {tr['ms']:.2f} ms (default {tr['default_ms']:.2f} ms)", file=sys.stderr,
+              flush=True)
 
     # ---- kernel breakdown: one untimed eager step with every kernel bracketed by HIP events on the
     # stream it runs on; it names the dominant kernel (the one of the two I-record kernels that
@@ -199,7 +210,7 @@ def main():
         "data": "synthetic: instances rebuilt from the reference's census rows; no record data is loaded",
         "config": {"workload": f"{args.family}-family batch: {len(insts)} instances (moduli x radii 4..7 <= rmax), "
                                f"{keys} keys (nD+nI)", "instances": len(insts), "keys_per_step": keys,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "tuning": tuning},
         "plain": {"value": plain_value, "ms_per_step": plain_ms},
         "batched_vs_plain": value / plain_value if plain_value else None,
         "roofline": roofline,

@@ -1,0 +1,69 @@
+"""Autotuner for batched solves: picks the probe-table slice per partition, the D records per
+bucket, the fill of the I capacity segments and the probe grid (cf_tuning) by timing candidate
+settings on the prepared batch itself.
+
+Search: coordinate descent from the defaults -- one knob at a time over its candidate values, the
+others held at their best so far, each candidate timed as the best of `reps` solves. Every candidate's
+result is compared with the default setting's (per-instance record counts and pair sets): a knob
+may only change speed, so a difference is an error, not a choice.
+"""
+from __future__ import annotations
+
+import time
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import lib as L
+
+DEFAULTS = dict(slice_kib=2048, d_per_bucket=5, seg_fill_pct=70, probe_blocks=2048)
+CANDIDATES = {
+    "slice_kib": [1024, 2048, 4096],
+    "d_per_bucket": [4, 5, 6, 8],
+    "seg_fill_pct": [60, 70, 80],
+    "probe_blocks": [1024, 2048, 4096],
+}
+
+
+def _signature(results) -> List[Tuple[int, int, bytes]]:
+    return [(r.nD, r.nI, r.pair_array().tobytes()) for r in results]
+
+
+def _time(batch, setting: Dict[str, int], reps: int, ctx: L.Context):
+    batch.set_tuning(**setting)
+    res, _ = batch.solve()                 # warm: allocations settle in the pool
+    best = float("inf")
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        batch.solve()
+        ctx.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best * 1e3, res
+
+
+def tune(batch, reps: int = 2, candidates: Optional[Dict[str, List[int]]] = None, log=None) -> Dict:
+    """Tune a prepared, not yet captured pipeline.Batch in place; returns
+    {"best": setting, "ms": best time, "default_ms": ..., "trials": [(setting, ms), ...]}."""
+    cands = candidates or CANDIDATES
+    ctx = batch.ctx
+    best = dict(DEFAULTS)
+    best_ms, ref = _time(batch, best, reps, ctx)
+    ref_sig = _signature(ref)
+    default_ms = best_ms
+    trials = [(dict(best), best_ms)]
+    for knob, values in cands.items():
+        for v in values:
+            if v == best[knob]:
+                continue
+            setting = dict(best, **{knob: v})
+            ms, res = _time(batch, setting, reps, ctx)
+            if _signature(res) != ref_sig:
+                raise AssertionError(f"tuning {setting} changed the result of the solve")
+            trials.append((setting, ms))
+            if log:
+                log(f"[tune] {setting} {ms:.2f} ms")
+            if ms < best_ms:
+                best, best_ms = setting, ms
+    batch.set_tuning(**best)
+    return {"best": best, "ms": best_ms, "default_ms": default_ms, "trials": trials}

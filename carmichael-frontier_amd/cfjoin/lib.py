@@ -47,6 +47,11 @@ class cf_stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class cf_tuning(ctypes.Structure):
+    _fields_ = [("slice_kib", ctypes.c_int32), ("d_per_bucket", ctypes.c_int32), ("seg_fill_pct", ctypes.c_int32),
+                ("probe_blocks", ctypes.c_int32)]
+
+
 # name -> (restype, argtypes); every name here is declared in include/cfjoin.h
 SIGNATURES = {
     "cf_version": (c_int, []),
@@ -89,6 +94,7 @@ SIGNATURES = {
                                ctypes.POINTER(c_void_p), P_u64, P_u64, ctypes.POINTER(cf_stats)]),
     "cf_batch_destroy": (c_int, [c_void_p]),
     "cf_batch_capture": (c_int, [c_void_p, c_u64]),
+    "cf_batch_set_tuning": (c_int, [c_void_p, ctypes.POINTER(cf_tuning)]),
 }
 
 

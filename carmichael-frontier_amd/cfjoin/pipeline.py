@@ -98,6 +98,14 @@ class Batch:
         L.check(L.load().cf_batch_create(self.ctx.handle, arr, n, ctypes.byref(h)), "cf_batch_create")
         self.handle = h
 
+    def set_tuning(self, slice_kib: int = 0, d_per_bucket: int = 0, seg_fill_pct: int = 0,
+                   probe_blocks: int = 0) -> None:
+        """Speed knobs of later solves (cf_batch_set_tuning); 0 keeps a default. Results never change."""
+        t = L.cf_tuning(slice_kib, d_per_bucket, seg_fill_pct, probe_blocks)
+        L.check(L.load().cf_batch_set_tuning(self.handle, ctypes.byref(t)), "cf_batch_set_tuning")
+        self.tuning = dict(slice_kib=slice_kib, d_per_bucket=d_per_bucket, seg_fill_pct=seg_fill_pct,
+                           probe_blocks=probe_blocks)
+
     def capture(self, max_chunk: int = 0) -> None:
         """Capture the batch's solve into a HIP graph (cf_batch_capture): later solve() calls with
         max_chunk 0 (or the captured one) are one graph launch each."""

@@ -262,17 +262,24 @@ struct IChunkLayout {   // the buffers behind a SegLayout
     SegLayout view() const { return SegLayout{O.p, fill.p, nseg, nblk, capl}; }
 };
 // slots a chunk of n leaves needs in the capacity layout
-u64 part_cap_slots(u64 n, const KeyHash &kh);
+// launch shapes and geometry targets that change speed, never results (cf_tuning, 0 = default)
+struct Tuning {
+    int slice_kib = 0;       // probe-table slice per partition (default 2048)
+    int d_per_bucket = 0;    // D records per bucket (default 5)
+    int seg_fill_pct = 0;    // expected fill of a capacity segment, percent (default 70)
+    int probe_blocks = 0;    // probe grid (default 2048)
+};
+u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout)
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, u32 *ovf);
+                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf);
 // count, scan and scatter: n records exactly, in partition order
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec);
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
 // rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
-                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand);
+                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand, const Tuning &tn);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,

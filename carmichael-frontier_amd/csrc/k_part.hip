@@ -836,27 +836,28 @@ bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn
 }
 
 // Capacity layout of a chunk of n I leaves: blocks of tpb tiles whose expected share per partition
-// is about 0.7 of a 2^9-slot segment (mean ~350, segment ~ mean + 8 sd).
+// is a set fraction of a 2^9-slot segment (default 70%: mean ~350, segment ~ mean + 8 sd).
 constexpr int CAPL = 9;
-static void cap_geometry(u64 n, const KeyHash &kh, u64 &tpb, u32 &nblk) {
+static void cap_geometry(u64 n, const KeyHash &kh, const Tuning &tn, u64 &tpb, u32 &nblk) {
     const u64 ntiles = (n + LT - 1) / LT;
-    tpb = ((u64)7 << CAPL << kh.lgP) / (10 * (u64)LT);
+    const u64 pct = tn.seg_fill_pct > 0 && tn.seg_fill_pct <= 100 ? (u64)tn.seg_fill_pct : 70;
+    tpb = (pct << CAPL << kh.lgP) / (100 * (u64)LT);
     if (tpb < 1) tpb = 1;
     nblk = (u32)((ntiles + tpb - 1) / tpb);
 }
 
-u64 part_cap_slots(u64 n, const KeyHash &kh) {
+u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) {
     u64 tpb;
     u32 nblk;
-    cap_geometry(n, kh, tpb, nblk);
+    cap_geometry(n, kh, tn, tpb, nblk);
     return ((u64)nblk << kh.lgP) << CAPL;
 }
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, u32 *ovf) {
+                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf) {
     u64 tpb;
     u32 nblk;
-    cap_geometry(n, kh, tpb, nblk);
+    cap_geometry(n, kh, tn, tpb, nblk);
     lay.O.reset();
     lay.nblk = nblk;
     lay.nseg = (u64)nblk << kh.lgP;
@@ -890,14 +891,15 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
 }
 
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
-                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand) {
+                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand, const Tuning &tn) {
     fill_u32(ctx, xq, NXCD, 0);
     fill_u64(ctx, (u64 *)ncand, 1, 0);
     if (!S.nseg) return;
     // tiles are dealt per XCD, so every XCD needs at least one block: the grid is a non-zero
     // multiple of the XCD count, about as many blocks as the chip holds at once
     const u64 ntiles = (S.nseg + SPT - 1) / SPT;
-    unsigned g = (unsigned)(ntiles < 2048 ? ntiles : 2048);
+    const u64 gmax = tn.probe_blocks > 0 ? (u64)tn.probe_blocks : 2048;
+    unsigned g = (unsigned)(ntiles < gmax ? ntiles : gmax);
     g = g < NXCD ? NXCD : g - g % NXCD;
     CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, S, (const uint2 *)rec, T, xq, ncand, ccap,
               cand);

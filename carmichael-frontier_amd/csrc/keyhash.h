@@ -35,10 +35,10 @@ CF_HD u64 hbucket(u64 h, const KeyHash &kh) {
 CF_HD u32 hoff(u64 h, const KeyHash &kh) { return (u32)(hbucket(h, kh) - (u64)hpart(h, kh.lgP) * kh_k(kh)); }
 CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
 
-// Geometry for a radius group with nD D records: about 5 D records per bucket, so a 16-slot bucket
-// is rarely full (Poisson tail P(X >= 16) ~ 7e-5); P = 2^lgP probe partitions, as few as keep one
-// partition's slice of the 32-byte-per-bucket table within about 2 MB (an XCD's L2 holds it while
-// the partition is probed), 16 <= P <= 256; as few hash ranges as keep a range within the buckets
+// Geometry for a radius group with nD D records: about 5 D records per bucket by default, so a
+// 16-slot bucket is rarely full (Poisson tail P(X >= 16) ~ 7e-5); P = 2^lgP probe partitions, as few
+// as keep one partition's slice of the 32-byte-per-bucket table within about 2 MB by default (an
+// XCD's L2 holds it while the partition is probed), 16 <= P <= 256; as few hash ranges as keep a range within the buckets
 // the LDS counting sort takes (at most 2^12 ranges; beyond that the D side is bucketed by global
 // atomics). An I record keeps the bucket offset within its partition and 32 - that many bits of
 // fingerprint, at most 14. Returns false if a partition would need more than 2^24 buckets.
@@ -50,10 +50,12 @@ inline int kh_bitlen(u64 v) {
     }
     return b;
 }
-inline bool kh_choose(u64 nD, KeyHash &kh) {
-    const u64 want = nD / 5 > 16 ? nD / 5 : 16;
+// d_per_bucket and slice_bytes are the tunable targets (defaults 5 and 2 MiB).
+inline bool kh_choose(u64 nD, KeyHash &kh, int d_per_bucket = 5, u64 slice_bytes = 2u << 20) {
+    const u64 dpb = d_per_bucket > 0 ? (u64)d_per_bucket : 5;
+    const u64 want = nD / dpb > 16 ? nD / dpb : 16;
     int lgP = 4;
-    while (lgP < 8 && want * 32 / ((u64)1 << lgP) > (2u << 20)) ++lgP;
+    while (lgP < 8 && want * 32 / ((u64)1 << lgP) > slice_bytes) ++lgP;
     int lgR = lgP;
     while (lgR < CF_RANGE_BITS_LDS && ((want + ((u64)1 << lgR) - 1) >> lgR) > CF_RANGE_BUCKETS_LDS) ++lgR;
     const u64 KR = (want + ((u64)1 << lgR) - 1) >> lgR;

@@ -202,9 +202,11 @@ struct Partitioning {
     int P() const { return 1 << kh.lgP; }
 };
 
-Partitioning assign_buckets(u64 nD) {
+Partitioning assign_buckets(u64 nD, const Tuning &tn) {
     Partitioning pt;
-    CF_REQUIRE(kh_choose(nD, pt.kh), "more than 2^24 buckets per partition for one radius group: split the batch");
+    const u64 slice = tn.slice_kib > 0 ? (u64)tn.slice_kib << 10 : (u64)2 << 20;
+    CF_REQUIRE(kh_choose(nD, pt.kh, tn.d_per_bucket, slice),
+               "more than 2^24 buckets per partition for one radius group: split the batch");
     pt.NB = kh_nb(pt.kh);
     return pt;
 }
@@ -227,6 +229,7 @@ struct cf_batch {
     std::map<int, DBuf<Node>> droots, iroots;    // per radius: the enumeration roots of live sides
     std::map<int, size_t> nd_roots, ni_roots;
     u64 pair_cap = 1 << 16;                      // pair capacity of the next solve (grows to fit)
+    Tuning tune;                                 // cf_batch_set_tuning
     BatchGraph *graph = nullptr;                 // the captured solve, once cf_batch_capture ran
     ~cf_batch();
 };
@@ -344,7 +347,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             Frontier fd = expand_to_last(ctx, d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first, tape);
             const u64 nD = fd.total;
             if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
-            const Partitioning pt = assign_buckets(nD);
+            const Partitioning pt = assign_buckets(nD, b->tune);
             const KeyHash &kh = pt.kh;
             DBuf<u32> bcount(ctx, pt.NB);
             DBuf<u64> bstart(ctx, pt.NB + 1);
@@ -373,7 +376,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                 u64 cap_m = 0, ccap = 0;
                 for (u64 L = 0; L < f.total; L += chunk) {
                     const u64 m = std::min<u64>(chunk, f.total - L);
-                    const u64 need = std::max<u64>(part_cap_slots(m, kh), m);
+                    const u64 need = std::max<u64>(part_cap_slots(m, kh, b->tune), m);
                     if (cap_m < need) {
                         cap_m = need;
                         rec = DBuf<u64>(ctx, cap_m);
@@ -385,17 +388,17 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     // one pass into segments sized for the expected share; the probe that follows
                     // is wasted only if a segment overflowed (heavily repeated keys), and then the
                     // chunk is laid out exactly and probed again
-                    part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay, rec.p,
+                    part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
                                      ovf.p);
                     part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
-                                     cand.p);
+                                     cand.p, b->tune);
                     const u64 ov = host_value_u32(ctx, tape, ovf.p);
                     u64 nc = host_value(ctx, tape, flags.p);
                     if (ov) {
                         part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
                                            rec.p);
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
-                                         cand.p);
+                                         cand.p, b->tune);
                         nc = host_value(ctx, tape, flags.p);
                     }
                     if (nc > ccap) {
@@ -403,7 +406,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                         ccap = nc;
                         cand = DBuf<u64>(ctx, ccap);
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
-                                         cand.p);
+                                         cand.p, b->tune);
                     }
                     part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
                                      (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
@@ -482,6 +485,20 @@ cf_batch::~cf_batch() {
 }
 
 extern "C" {
+
+int cf_batch_set_tuning(cf_batch *b, const cf_tuning *t) {
+    API_BEGIN
+    CF_REQUIRE(b && t, "NULL argument");
+    CF_REQUIRE(!b->graph, "the batch is already captured: tune before cf_batch_capture");
+    CF_REQUIRE(t->slice_kib >= 0 && t->slice_kib <= (1 << 20) && t->d_per_bucket >= 0 && t->d_per_bucket <= 12 &&
+                   t->seg_fill_pct >= 0 && t->seg_fill_pct <= 90 && t->probe_blocks >= 0 && t->probe_blocks <= 65536,
+               "tuning values out of range");
+    b->tune.slice_kib = t->slice_kib;
+    b->tune.d_per_bucket = t->d_per_bucket;
+    b->tune.seg_fill_pct = t->seg_fill_pct;
+    b->tune.probe_blocks = t->probe_blocks;
+    API_END
+}
 
 int cf_batch_capture(cf_batch *b, uint64_t max_chunk) {
     API_BEGIN

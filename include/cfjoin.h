@@ -178,6 +178,17 @@ int cf_batch_solve(cf_batch *batch, uint64_t max_chunk, uint64_t **out_d_dev, ui
                    uint64_t **out_inst_dev, uint64_t *npairs, uint64_t *inst_counts, cf_stats *stats);
 int cf_batch_destroy(cf_batch *batch);
 
+/* Speed knobs of a batch's solves; results never depend on them (0 keeps the default). Chosen by
+ * the autotuner (cfjoin/autotune.py), which times candidates on the batch itself. No counterpart
+ * in the reference, whose kernels run with fixed launch shapes (gpu/mitm_gpu2.py:93-97). */
+typedef struct cf_tuning {
+    int32_t slice_kib;        /* probe-table slice per partition, KiB (default 2048) */
+    int32_t d_per_bucket;     /* D records per table bucket (default 5, at most 12) */
+    int32_t seg_fill_pct;     /* expected fill of an I capacity segment, percent (default 70, at most 90) */
+    int32_t probe_blocks;     /* probe grid, blocks (default 2048) */
+} cf_tuning;
+int cf_batch_set_tuning(cf_batch *batch, const cf_tuning *tuning);
+
 /* The batch's persistent launch: one eager solve notes every value the host reads back during a
  * solve (level sizes, candidate and pair counts), then the whole solve is captured into a HIP
  * graph with those values, and every later cf_batch_solve (with max_chunk 0 or the captured one)

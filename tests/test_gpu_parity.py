@@ -350,3 +350,23 @@ def test_batch_graph_replays_match_eager(workloads):
             pipeline.solve(other)            # eager work between replays
             L.default_context().synchronize()
         b.close()
+
+
+def test_autotune_only_changes_speed(workloads):
+    """Every tuning candidate solves a mixed batch to the same counts and pair sets (the tuner checks
+    each one against the default setting and raises otherwise); the chosen setting is then captured."""
+    from cfjoin import autotune
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    insts = W.batch_instances("lambdaN", radii=(4, 5), logged=logged, limit=20) + [I.c1_instance(2, None)]
+    say("tune", len(insts))
+    eager, _ = pipeline.solve_batch(insts)
+    b = pipeline.Batch(insts)
+    tr = autotune.tune(b, reps=1, candidates={"slice_kib": [512, 4096], "d_per_bucket": [3, 8],
+                                              "seg_fill_pct": [50, 90], "probe_blocks": [64, 4096]})
+    assert len(tr["trials"]) == 9
+    b.capture()
+    res, _ = b.solve()
+    for ra, rb in zip(eager, res):
+        assert (ra.nD, ra.nI) == (rb.nD, rb.nI) and np.array_equal(ra.pair_array(), rb.pair_array())
+    b.close()

@@ -29,9 +29,10 @@ sys.path.insert(0, os.path.join(ROOT, "carmichael-frontier_amd"))
 PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per I record of the two kernels that dominate a step:
 #   k_part_scatter writes each I record (8 bytes: bucket offset, fingerprint, leaf index) once;
-#   k_part_probe reads each 8-byte record once and the first 16-byte half of the
-#   probe-table bucket it names (the second half only for the few buckets holding more than 8).
-KERNEL_BYTES_PER_I = {"k_part_scatter": 8, "k_part_probe": 8 + 16}
+#   k_part_probe reads each 8-byte record once. Its 16-byte probe-table reads hit L2 by design (an
+#   XCD probes one partition at a time and the partition's table slice fits its L2), so they are
+#   not HBM bytes; the measured traffic (FETCH_SIZE) shows what they cost when they do miss.
+KERNEL_BYTES_PER_I = {"k_part_scatter": 8, "k_part_probe": 8}
 
 
 def run_batched(batch):
@@ -83,6 +84,66 @@ def cpu_baseline(insts, res_batched, threads, budget_keys):
                 seconds=round(t_sum, 2), pair_set_mismatches=mismatches)
 
 
+def copy_probe(ctx, nbytes):
+    """A plain device-to-device copy of nbytes: the calibration point for FETCH_SIZE / WRITE_SIZE."""
+    from cfjoin import xp
+    n = nbytes // 8
+    src = xp.empty(n)
+    dst = xp.empty(n)
+    ctx.d2d(dst.ptr, src.ptr, n * 8)
+    ctx.synchronize()
+
+
+def kernel_name(s):
+    s = s.replace("(anonymous namespace)::", "").replace("void ", "")
+    return s.split("(")[0].split("<")[0]
+
+
+def measure_traffic(kernel, family, setting):
+    """HBM bytes per launch of `kernel`: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE,
+    KiB per dispatch) over a child run of this script (one eager solve of the batch with the same
+    tuning, then a 1 GiB device copy); each counter is scaled by what it reads for that copy (the
+    copy reads and writes 1 GiB). Returns (bytes per launch, details) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    tune_arg = ",".join(str(setting.get(k, 0)) for k in ("slice_kib", "d_per_bucket", "seg_fill_pct", "probe_blocks"))
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="cf_pmc_", dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, ctr)
+            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+                   os.path.abspath(__file__), "--pmc-child", "--family", family, "--tuning", tune_arg,
+                   "--copy-probe", str(1 << 30)]
+            try:
+                subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180, check=True)
+            except (subprocess.SubprocessError, OSError) as e:
+                return None, f"{ctr} pass failed: {type(e).__name__}"
+            files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+            if not files:
+                return None, f"{ctr} pass wrote no counters"
+            vals, copy = [], 0.0
+            for r in csv.DictReader(open(files[0])):
+                k = kernel_name(r["Kernel_Name"])
+                v = float(r["Counter_Value"])
+                if k == kernel:
+                    vals.append(v)
+                elif "copyBuffer" in k:
+                    copy = max(copy, v)
+            if not vals or copy <= 0:
+                return None, f"{ctr}: no dispatches of {kernel} or of the copy"
+            scale = (1 << 20) / copy                # KiB the copy moved / KiB the counter read for it
+            out[ctr] = (sum(vals) / len(vals) * 1024 * scale, scale, len(vals))
+    total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
+    return total, {"fetch_bytes": round(out["FETCH_SIZE"][0]), "write_bytes": round(out["WRITE_SIZE"][0]),
+                   "fetch_scale": round(out["FETCH_SIZE"][1], 4), "write_scale": round(out["WRITE_SIZE"][1], 4),
+                   "launches": out["FETCH_SIZE"][2]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +154,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain leg (profiling the batched path alone)")
     ap.add_argument("--no-graph", action="store_true", help="time the batched leg as eager launches, not a graph")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (skip the autotuner)")
+    ap.add_argument("--tuning", default="", metavar="SLICE,DPB,FILL,GRID",
+                    help="pin the tuning (slice KiB, D per bucket, segment fill %%, probe blocks) instead of tuning")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes for traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-keys", type=float, default=8.0e8, help="CPU-baseline sample size in keys")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="after the run, one device-to-device copy of BYTES (calibrates byte counters)")
@@ -112,6 +177,14 @@ def main():
 
     # instance preparation (descriptors and bound tables onto the device) happens before timing
     batch = pipeline.Batch(insts)
+    pinned = [int(x) for x in args.tuning.split(",")] if args.tuning else None
+    if pinned:
+        batch.set_tuning(*pinned)
+    if args.pmc_child:
+        # one eager solve of the batch, then the calibration copy: what the counter passes read
+        run_batched(batch)
+        copy_probe(ctx, args.copy_probe or (1 << 30))
+        return
     singles = [] if args.no_plain else [pipeline.Batch([x]) for x in insts]
     # warm-up doubles as the bit-exactness check of both paths
     for _ in range(max(args.warmup, 1)):
@@ -126,8 +199,9 @@ def main():
           f"batched == plain pair sets: {same}", file=sys.stderr, flush=True)
 
     # ---- autotuner (untimed): slice per partition, D records per bucket, segment fill, probe grid
-    tuning = None
-    if not args.no_tune:
+    tuning = {"setting": dict(zip(("slice_kib", "d_per_bucket", "seg_fill_pct", "probe_blocks"), pinned)),
+              "pinned": True} if pinned else None
+    if not args.no_tune and not pinned:
         from cfjoin import autotune
         tr = autotune.tune(batch, reps=1)
         tuning = {"setting": tr["best"], "ms": round(tr["ms"], 3), "default_ms": round(tr["default_ms"], 3),
@@ -190,6 +264,19 @@ def main():
                     breakdown_ms_per_step={k: round(v[0], 3) for k, v in
                                            sorted(breakdown.items(), key=lambda kv: -kv[1][0])})
 
+    # the device work is done: give the memory back before the counter passes run their own solve
+    batch.close()
+    for x in singles:
+        x.close()
+    ctx.release_pool()
+    if not args.no_traffic:
+        setting = (tuning or {}).get("setting") or {}
+        traffic, detail = measure_traffic(dominant, args.family, setting)
+        roofline["traffic"] = round(traffic) if traffic else None
+        roofline["traffic_detail"] = detail
+        if traffic:
+            roofline["traffic_vs_algorithmic"] = round(traffic / (per_rec * nI_total / max(k_launches, 1)), 3)
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -227,13 +314,7 @@ def main():
     print(json.dumps(line), flush=True)
 
     if args.copy_probe:
-        # a plain device copy of known size: the calibration point for FETCH_SIZE / WRITE_SIZE
-        n = args.copy_probe // 8
-        from cfjoin import xp
-        src = xp.empty(n)
-        dst = xp.empty(n)
-        ctx.d2d(dst.ptr, src.ptr, n * 8)
-        ctx.synchronize()
+        copy_probe(ctx, args.copy_probe)
 
 
 if __name__ == "__main__":

@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "carmichael-frontier_amd"))
 
@@ -144,6 +146,65 @@ def measure_traffic(kernel, family, setting):
                    "launches": out["FETCH_SIZE"][2]}
 
 
+def serve_main(args):
+    """The serving measurement: one set of requests (both families, radii 4..8, up to 5e8 records
+    each), served (a) by the continuous-batching server and (b) one request at a time (one worker,
+    batches of one, FIFO): first all arriving at once (each mode's capacity), then as a Poisson stream
+    at 0.9 of the one-at-a-time capacity (latency at a fixed rate). Every result is checked against the
+    reference's logged counts and the runs' pair sets against each other."""
+    from cfjoin import serve
+    from cfjoin import workloads as W
+    logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))
+    insts = W.serving_requests(args.requests, seed=args.seed, logged=logged)
+    keys = sum(x.meta["logged"]["nD"] + x.meta["logged"]["nI"] for x in insts)
+    print(f"[serve] {len(insts)} requests, {keys:.4g} keys", file=sys.stderr, flush=True)
+    # warm-up: a few requests one at a time (fills the device pools)
+    serve.run_one_at_a_time(insts[:min(16, len(insts))])
+
+    def stream(workers, max_batch, arrivals):
+        srv = serve.Server(workers=workers, max_batch=max_batch)
+        t0 = time.perf_counter()
+        try:
+            out = serve.run_stream(srv, insts, arrivals)
+        finally:
+            srv.close()
+        return sorted(out, key=lambda r: r.tag), time.perf_counter() - t0
+
+    # capacity of each mode: every request arrives at once
+    burst = np.zeros(len(insts))
+    single_burst, wall_sb = stream(1, 1, burst)
+    batched_burst, wall_bb = stream(args.workers, args.max_batch, burst)
+    print(f"[serve] all at once: one at a time {wall_sb:.3f} s, continuous batching {wall_bb:.3f} s", file=sys.stderr,
+          flush=True)
+    # the fixed-rate stream: Poisson arrivals at 0.9 of what one-at-a-time service sustains
+    rate = args.rate or 0.9 * len(insts) / wall_sb
+    arrivals = serve.poisson_arrivals(len(insts), rate, seed=args.seed + 1)
+    batched, wall_b = stream(args.workers, args.max_batch, arrivals)
+    single, wall_s = stream(1, 1, arrivals)
+    print(f"[serve] Poisson at {rate:.1f}/s: continuous batching {wall_b:.3f} s, one at a time {wall_s:.3f} s",
+          file=sys.stderr, flush=True)
+    bad = [x.tag for x, r in zip(insts, batched)
+           if (r.nD, r.nI, len(r.pairs_d)) != (x.meta["logged"]["nD"], x.meta["logged"]["nI"], x.meta["logged"]["pairs"])]
+    same = all(np.array_equal(a.pair_array(), b.pair_array()) and (a.nD, a.nI) == (b.nD, b.nI)
+               for runs in ((batched, single), (batched, batched_burst), (batched, single_burst)) for a, b in zip(*runs))
+    sb, ss = serve.summary(batched, wall_b), serve.summary(single, wall_s)
+    sbb, ssb = serve.summary(batched_burst, wall_bb), serve.summary(single_burst, wall_sb)
+    line = {
+        "metric": "serving: residue keys joined/s and p50/p99 request latency at a fixed Poisson arrival rate",
+        "value": sb["keys_per_s"], "unit": "residue keys joined/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_ms": round(sb["p50_ms"], 3), "p99_ms": round(sb["p99_ms"], 3),
+        "dtype": "u64", "data": "synthetic: instances rebuilt from the reference's census rows",
+        "config": {"workload": f"{len(insts)} requests drawn (seed {args.seed}) from the logged instances of both "
+                               f"families, radii 4..8, <= 5e8 keys each; {keys} keys", "arrival_rate_per_s": round(rate, 2),
+                   "workers": args.workers, "max_batch": args.max_batch},
+        "continuous_batching": sb, "one_at_a_time": ss,
+        "all_at_once": {"continuous_batching": sbb, "one_at_a_time": ssb,
+                        "throughput_ratio": sbb["keys_per_s"] / ssb["keys_per_s"]},
+        "verify": {"logged_count_mismatches": len(bad), "same_results_in_mix_and_alone": bool(same)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +219,15 @@ def main():
                     help="pin the tuning (slice KiB, D per bucket, segment fill %%, probe blocks) instead of tuning")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes for traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--serve", action="store_true",
+                    help="serving mode: a Poisson stream of modulus solves through the continuous-batching server, "
+                         "beside the same stream served one request at a time")
+    ap.add_argument("--requests", type=int, default=240, help="serving mode: requests in the stream")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving mode: arrivals per second (0: 1.5x what one-at-a-time service sustains)")
+    ap.add_argument("--workers", type=int, default=4, help="serving mode: server workers (streams)")
+    ap.add_argument("--max-batch", type=int, default=64, help="serving mode: requests per batch at most")
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-keys", type=float, default=8.0e8, help="CPU-baseline sample size in keys")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="after the run, one device-to-device copy of BYTES (calibrates byte counters)")
@@ -169,6 +239,10 @@ def main():
 
     from cfjoin import lib, pipeline
     from cfjoin import workloads as W
+
+    if args.serve:
+        serve_main(args)
+        return
 
     logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))
     insts = W.batch_instances(args.family, radii=(4, 5, 6, 7), logged=logged)

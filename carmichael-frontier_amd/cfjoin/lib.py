@@ -52,6 +52,13 @@ class cf_tuning(ctypes.Structure):
                 ("probe_blocks", ctypes.c_int32)]
 
 
+class cf_served(ctypes.Structure):
+    _fields_ = [("tag", c_u64), ("status", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("nD", c_u64),
+                ("nI", c_u64), ("npairs", c_u64), ("pairs_d", ctypes.POINTER(c_u64)),
+                ("pairs_i", ctypes.POINTER(c_u64)), ("t_submit_us", ctypes.c_double),
+                ("t_start_us", ctypes.c_double), ("t_done_us", ctypes.c_double), ("error", ctypes.c_char * 160)]
+
+
 # name -> (restype, argtypes); every name here is declared in include/cfjoin.h
 SIGNATURES = {
     "cf_version": (c_int, []),
@@ -95,6 +102,13 @@ SIGNATURES = {
     "cf_batch_destroy": (c_int, [c_void_p]),
     "cf_batch_capture": (c_int, [c_void_p, c_u64]),
     "cf_batch_set_tuning": (c_int, [c_void_p, ctypes.POINTER(cf_tuning)]),
+    "cf_batch_footprint": (c_int, [c_void_p, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "cf_server_create": (c_int, [c_int, c_int, c_int, c_u64, ctypes.POINTER(c_void_p)]),
+    "cf_server_submit": (c_int, [c_void_p, c_void_p, c_u64]),
+    "cf_server_next": (c_int, [c_void_p, c_int, ctypes.POINTER(cf_served)]),
+    "cf_server_release": (c_int, [c_void_p, c_u64]),
+    "cf_server_pending": (c_int, [c_void_p, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "cf_server_destroy": (c_int, [c_void_p]),
 }
 
 

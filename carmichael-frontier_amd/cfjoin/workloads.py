@@ -53,3 +53,36 @@ def anchor(family: str, r: int = 6, logged: Optional[dict] = None) -> I.Instance
         w = I.load_workloads()
         inst.meta["logged"] = logged.get((family, int(w["families"][family]["anchor"]), r))
     return inst
+
+
+def serving_requests(n: int, seed: int = 0, logged: Optional[dict] = None, max_keys: float = 5e8,
+                     radii=(4, 5, 6, 7, 8)) -> List[I.Instance]:
+    """The serving workload: n modulus solves drawn uniformly (numpy default_rng(seed)) from the
+    logged instances of both families with radius in `radii` and at most max_keys records (nD + nI),
+    which mixes the batch class (divisor moduli, pools of 120-600 primes) with the larger anchors'
+    class. Requires the logged-counts table (it sizes the draw)."""
+    import numpy as np
+    assert logged is not None, "serving_requests draws from the logged instances"
+    w = I.load_workloads()
+    cands = []
+    for (fam, D, r), row in sorted(logged.items()):
+        if r not in radii or row.get("path") not in ("dfilter", "host-exact") or row["nD"] + row["nI"] > max_keys:
+            continue
+        if row["nD"] + row["nI"] == 0:
+            continue
+        cands.append((fam, D, r))
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(cands), n)
+    pools = {}
+    out = []
+    for k in picks:
+        fam, D, r = cands[int(k)]
+        f = w["families"][fam]
+        row = next(x for x in f["rows"] if int(x["D"]) == D)
+        if (fam, D) not in pools:
+            pools[(fam, D)] = I.pool(D, int(row["cap"]))
+        inst = I.family_instance(D, int(row["cap"]), int(f["U"]), r, P=pools[(fam, D)])
+        inst.tag = f"{fam}:D{D}:r{r}"
+        inst.meta["logged"] = logged.get((fam, D, r))
+        out.append(inst)
+    return out

@@ -27,7 +27,6 @@ static size_t pool_round(size_t bytes) {
     return (bytes + g - 1) / g * g;
 }
 
-size_t g_peak_live = 0;
 
 void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     const size_t sz = pool_round(bytes);
@@ -73,7 +72,7 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     if (ctx->arena) ctx->arena->owned[p] = sz;
     size_t live = 0;
     for (auto &kv : ctx->live) live += kv.second;
-    g_peak_live = std::max(g_peak_live, live);
+    ctx->peak_live = std::max(ctx->peak_live, live);
     return p;
 }
 

@@ -69,6 +69,7 @@ struct cf_ctx {
     // later allocations of the same capture) and every block handed out is recorded in it, so
     // nothing a graph node touches returns to the shared pool while the graph lives
     struct CaptureArena *arena = nullptr;
+    size_t peak_live = 0;                         // largest live footprint since the last reset
 };
 
 struct CaptureArena {
@@ -208,7 +209,6 @@ struct cf_plan {
 void plan_expand(cf_plan *pl);
 void plan_write_range(cf_plan *pl, u64 L0, u64 L1, u64 *keys, u64 *ids);
 int bitlen(u64 v);
-extern size_t g_peak_live;
 
 // ---- kernels (host launchers) -------------------------------------------------------------
 // exclusive scan of u32 or u64 counts into u64 offsets; returns the total (synchronous)
@@ -238,6 +238,10 @@ void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, si
 // DR: n records of 16 bytes (exact key, id) in bucket order; T: 16 u16 slots per bucket, zeroed
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
+// validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
+SideSpec spec_of_d(const cf_instance &in, u64 koff);
+SideSpec spec_of_i(const cf_instance &in, u64 koff);
+
 // D side bucketed through LDS (hash ranges, then a counting sort per range); false when the
 // bucket geometry is too large for it (the caller then uses the per-record atomic path)
 // kerr: set non-zero if a record falls outside its range's buckets (an internal inconsistency;

@@ -441,7 +441,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             st->nI += out.nI[j];
         }
         st->npairs = out.npairs;
-        st->peak_bytes = g_peak_live;
+        st->peak_bytes = ctx->peak_live;
         st->ms_total = clk.ms(0, 1);
         st->ms_gen_d = ms_d;            // D expansion, bucketing and probe table
         st->ms_sort_d = 0;
@@ -485,6 +485,42 @@ cf_batch::~cf_batch() {
 }
 
 extern "C" {
+
+int cf_batch_footprint(cf_batch *b, uint64_t *bytes, uint64_t *nD_total, uint64_t *nI_total) {
+    API_BEGIN
+    CF_REQUIRE(b && bytes, "NULL argument");
+    // Expand both trees of every radius group (counts only) and price the group's two phases: the D
+    // build (tables, its temporary records and the D parents) and the I phase (tables, the I records
+    // in their capacity layout and the I parents); groups run one after another.
+    cf_ctx *ctx = b->ctx;
+    u64 peak = 0, sd = 0, si = 0;
+    for (auto &g : b->groups) {
+        u64 nD = 0, nnD = 0, nI = 0, nnI = 0;
+        {
+            Frontier fd = expand_to_last(ctx, b->d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first,
+                                         nullptr);
+            nD = fd.total;
+            nnD = fd.nn;
+        }
+        {
+            Frontier fi = expand_to_last(ctx, b->i_desc.p, b->iroots.at(g.first).p, b->ni_roots.at(g.first), g.first,
+                                         nullptr);
+            nI = fi.total;
+            nnI = fi.nn;
+        }
+        sd += nD;
+        si += nI;
+        const Partitioning pt = assign_buckets(nD, b->tune);
+        const u64 tables = 16 * nD + 44 * pt.NB;            // DR, T, bstart, counters
+        const u64 d_build = tables + 24 * nD + 60 * nnD;    // + range-sorted temporaries + D parents
+        const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) + 60 * nnI + nI / 128;
+        peak = std::max(peak, std::max(d_build, i_phase));
+    }
+    *bytes = peak + (64u << 20);
+    if (nD_total) *nD_total = sd;
+    if (nI_total) *nI_total = si;
+    API_END
+}
 
 int cf_batch_set_tuning(cf_batch *b, const cf_tuning *t) {
     API_BEGIN
@@ -566,7 +602,7 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
     CF_REQUIRE(b && out_d && out_i && out_inst && npairs, "NULL argument");
     *out_d = *out_i = *out_inst = nullptr;
     *npairs = 0;
-    g_peak_live = 0;
+    b->ctx->peak_live = 0;
     BatchGraph *G = b->graph;
     if (G && (!max_chunk || max_chunk == G->chunk)) {
         // replay the captured solve: one graph launch, then the few results are read back
@@ -657,7 +693,7 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
     *out_d = nullptr;
     *out_i = nullptr;
     *npairs = 0;
-    g_peak_live = 0;
+    ctx->peak_live = 0;
     if (mode != CF_SOLVE_SORTMERGE) {
         cf_batch b;
         b.ctx = ctx;
@@ -759,7 +795,7 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
         stats->nD = nD;
         stats->nI = nI;
         stats->npairs = total_pairs;
-        stats->peak_bytes = g_peak_live;
+        stats->peak_bytes = ctx->peak_live;
         stats->ms_total = t;
         stats->ms_gen_d = a;
         stats->ms_sort_d = b;

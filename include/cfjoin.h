@@ -189,6 +189,11 @@ typedef struct cf_tuning {
 } cf_tuning;
 int cf_batch_set_tuning(cf_batch *batch, const cf_tuning *tuning);
 
+/* Device memory a solve of the batch needs at its peak (bytes, an estimate with headroom) and its
+ * exact record totals, from the enumeration trees' level counts (no leaf is generated). Used to
+ * admit work by memory footprint; no counterpart in the reference. */
+int cf_batch_footprint(cf_batch *batch, uint64_t *bytes, uint64_t *nD_total, uint64_t *nI_total);
+
 /* The batch's persistent launch: one eager solve notes every value the host reads back during a
  * solve (level sizes, candidate and pair counts), then the whole solve is captured into a HIP
  * graph with those values, and every later cf_batch_solve (with max_chunk 0 or the captured one)
@@ -197,6 +202,34 @@ int cf_batch_set_tuning(cf_batch *batch, const cf_tuning *tuning);
  * every kernel of every instance from Python (gpu/mitm_gpu_mstar_driver.py:209-243); this has
  * no counterpart there. */
 int cf_batch_capture(cf_batch *batch, uint64_t max_chunk);
+
+/* ---- serving loop (continuous batching) -------------------------------------------------------
+ * Requests (instances) are submitted as they arrive and solved by `workers` threads, each with its
+ * own stream on the device: a free worker takes everything that has arrived (up to max_batch) and
+ * solves it as one batch; a batch starts only while the priced footprints (cf_batch_footprint) of
+ * the batches on the device stay within mem_budget bytes (0: half the free device memory), or
+ * alone if it is larger. Each result equals the request's solve on its own. The reference runs
+ * instances strictly one after another (gpu/run_mstar_family.py); this has no counterpart there. */
+typedef struct cf_server cf_server;
+typedef struct cf_served {
+    uint64_t tag;
+    int32_t status;           /* 0, or the failed call's status (error holds the message) */
+    int32_t batch_size;       /* requests solved in the same batch */
+    uint64_t nD, nI, npairs;
+    const uint64_t *pairs_d;  /* host arrays, valid until cf_server_release(tag) */
+    const uint64_t *pairs_i;
+    double t_submit_us, t_start_us, t_done_us;   /* since cf_server_create */
+    char error[160];
+} cf_served;
+int cf_server_create(int device, int workers, int max_batch, uint64_t mem_budget, cf_server **out);
+/* the instance (and its prime lists) is copied; tags must be unique among unreleased results */
+int cf_server_submit(cf_server *server, const cf_instance *inst, uint64_t tag);
+/* 1: *out holds the next finished request; 0: none finished within timeout_ms */
+int cf_server_next(cf_server *server, int timeout_ms, cf_served *out);
+int cf_server_release(cf_server *server, uint64_t tag);
+int cf_server_pending(cf_server *server, uint64_t *queued, uint64_t *outstanding);
+/* finishes every submitted request, then stops the workers */
+int cf_server_destroy(cf_server *server);
 
 #ifdef __cplusplus
 }

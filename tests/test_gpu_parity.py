@@ -370,3 +370,32 @@ def test_autotune_only_changes_speed(workloads):
     for ra, rb in zip(eager, res):
         assert (ra.nD, ra.nI) == (rb.nD, rb.nI) and np.array_equal(ra.pair_array(), rb.pair_array())
     b.close()
+
+
+def test_server_results_equal_alone(workloads):
+    """Requests served together by the continuous-batching server (several workers, batches of
+    whatever has arrived) get exactly the counts and pair sets they get solved alone."""
+    from cfjoin import serve
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    insts = W.serving_requests(24, seed=3, logged=logged, max_keys=3e7) + [I.c1_instance(0, 100)]
+    say("serve", len(insts))
+    srv = serve.Server(workers=3, max_batch=8)
+    for k, x in enumerate(insts):
+        srv.submit(x, k)
+    got = {}
+    while len(got) < len(insts):
+        r = srv.next(30000)
+        assert r is not None, "server produced no result within 30 s"
+        assert r.status == 0, r.error
+        got[r.tag] = r
+    srv.close()
+    assert max(r.batch_size for r in got.values()) > 1          # requests were batched together
+    for k, x in enumerate(insts):
+        alone = pipeline.solve(x)
+        r = got[k]
+        assert (r.nD, r.nI) == (alone.nD, alone.nI), x.tag
+        assert np.array_equal(r.pair_array(), alone.pair_array()), x.tag
+        lg = x.meta.get("logged")
+        if lg:
+            assert (r.nD, r.nI, len(r.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]), x.tag

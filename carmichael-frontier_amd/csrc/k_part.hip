@@ -261,6 +261,8 @@ __global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 
 }
 
 // Scatter into bucket order; the rank a record takes in its bucket is also its table slot.
+__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key,
+                                       u64 &id);
 __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
                                                        u32 *__restrict__ bcur, DRec *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
@@ -271,14 +273,12 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + PB * k;
             if (i >= tl.cnt) continue;
-            const LeafRef r = leaf_ref(L, tl, S, i);
-            u64 xm, x2;
-            leaf_residue(*r.s, r.nd, r.x, xm, x2);
-            const u64 h = leaf_hash(xm, x2, r.s->mc.mask2, r.s->koff);
+            u64 h, key, id;
+            d_leaf(L, tl, S, i, h, key, id);
             const u64 b = hbucket(h, kh);
             const u32 slot = atomicAdd(&bcur[b], 1u);
             const u64 b0 = bstart[b];
-            DR[b0 + slot] = DRec{cf_key(xm, x2, r.s->mc) + r.s->koff, id_update(*r.s, r.nd.id, r.x, L.depth)};
+            DR[b0 + slot] = DRec{key, id};
             if (slot < (u32)TSLOTS) {
                 const u32 more = (slot == 7 && bstart[b + 1] - b0 > 8) ? 0x4000u : 0u;
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));

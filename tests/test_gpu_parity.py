@@ -399,3 +399,15 @@ def test_server_results_equal_alone(workloads):
         lg = x.meta.get("logged")
         if lg:
             assert (r.nD, r.nI, len(r.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]), x.tag
+
+
+def test_r8_anchor_d_filtered_matches_logged(workloads):
+    """The large-radius anchor (lambda(N), r = 8, 256-prime pool, h <= 1) with the reference's
+    D filter: 18 426 621 D and 471 754 092 I records, no pairs, as logged; I side in 8 chunks."""
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    inst = W.anchor("lambdaN", 8, logged)
+    say("r8", inst.M, len(inst.INS))
+    res = pipeline.solve(inst, max_chunk=1 << 26)
+    lg = inst.meta["logged"]
+    assert (res.nD, res.nI, len(res.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]) == (18426621, 471754092, 0)

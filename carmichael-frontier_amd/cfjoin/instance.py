@@ -16,7 +16,7 @@ import json
 import os
 from dataclasses import dataclass, field
 from math import comb, prod
-from typing import List, Optional
+from typing import List, Optional, Tuple
 
 from .verify import is_prime
 
@@ -185,3 +185,45 @@ def c1_instance(seed: int, s: Optional[int] = None, r: int = 3, k: int = 29, qma
         return Instance(M=M, r=r, B0=B0, INS=INS, P0=P0, Icap=1, Dmin=prod(B0[-r:]) + 1, split=None,
                         hmax=r, U=s * P0, n_ins_full=len(INS), tag=f"C1_seed{seed}_s{s}_empty")
     return inst
+
+
+def c2_modulus(seed: int, pool_lo: int = 200, pool_hi: int = 500) -> Tuple[int, int]:
+    """Plain k=64 workload modulus: M = 2^a 3^b 5^c 7^d times a seeded subset of the primes 11..61,
+    drawn with random.Random(seed) until M lies in [1.7e15, 1.6e17]; then the pool cap is raised from
+    2000 until Q(M) (primes q <= cap with (q-1) | M, q not dividing M) holds pool_lo..pool_hi primes.
+    Returns (M, cap)."""
+    import random
+    g = random.Random(seed)
+    small = [11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61]
+    for _ in range(100000):
+        M = 2 ** g.randint(5, 9) * 3 ** g.randint(2, 4) * 5 ** g.randint(1, 3) * 7 ** g.randint(1, 2)
+        for q in small:
+            if g.random() < 0.5:
+                M *= q
+        if not 1.7e15 <= M <= 1.6e17:
+            continue
+        lo, hi = 2000, 2000
+        while len(pool(M, hi)) < pool_lo and hi < (1 << 24):
+            lo, hi = hi, hi * 2
+        if len(pool(M, hi)) < pool_lo:
+            continue
+        while hi - lo > 1:                   # smallest cap reaching pool_lo primes
+            mid = (lo + hi) // 2
+            if len(pool(M, mid)) >= pool_lo:
+                hi = mid
+            else:
+                lo = mid
+        if len(pool(M, hi)) <= pool_hi:
+            return M, hi
+    raise ValueError(f"no k=64 modulus found for seed {seed}")
+
+
+def c2_instance(seed: int, s: int, r: int = 6, k: int = 64) -> Optional[Instance]:
+    """The k = 64, r = 6 single-modulus workload on a seeded modulus: Q(M) as in c2_modulus (200-500
+    primes), B0 = the 64 smallest, bound U = s * prod(B0) through the same recipe as the family
+    instances (the literal 1.1 * prod(B0) leaves both sides empty)."""
+    M, cap = c2_modulus(seed)
+    Q = pool(M, cap)
+    B0 = Q[:k]
+    INS = Q[k:]
+    return build_instance(M, B0, INS, r, s * prod(B0), split=None, hmax=r, tag=f"C2_seed{seed}_s{s}")

@@ -333,3 +333,116 @@ long long or_cpu_join(u64 M, int r, u64 P0modM, u64 icap_lo, u64 icap_hi, u64 dm
     if (times) { times[0] = t1 - t0; times[1] = t2 - t1; }
     return (long long)np;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* The CPU join with the roles swapped, for a D side far larger than the I side: the I records   */
+/* go into the hash (key -> every id with that key), then T threads sweep the D subsets, thread  */
+/* t taking the largest index i = nB-1-t, nB-1-t-T, ... and running the D enumeration below it   */
+/* (same order, prune, key and ids as or_ddump), probing each D key. Every (D id, I id) pair of  */
+/* equal keys is emitted. Same pair set as or_cpu_join (src/mitm64.cpp:51-121 hashes D instead). */
+typedef struct { dhash_t *h; } isink_t;
+static void isink_put(void *user, u64 key, u64 id) { dh_insert(((isink_t *)user)->h, key, id); }
+
+typedef struct {
+    u64 M; int R; u128 Dmin; const u64 *B; int nB; int bitmask;
+    const dhash_t *h; int tid, T;
+    u64 *pd, *pi; u64 npairs, cap; u64 leaves;
+} dsweep_t;
+
+static void dsweep_emit(dsweep_t *w, u64 key, u64 id) {
+    const dhash_t *h = w->h;
+    u64 s = hslot(key, h->mask);
+    while (h->head[s] >= 0) {
+        if (h->slotkey[s] == key) {
+            for (int64_t e = h->head[s]; e >= 0; e = h->ent[e].next) {
+                if (w->npairs == w->cap) {
+                    w->cap = w->cap ? 2 * w->cap : 64;
+                    w->pd = (u64 *)realloc(w->pd, w->cap * sizeof(u64));
+                    w->pi = (u64 *)realloc(w->pi, w->cap * sizeof(u64));
+                }
+                w->pd[w->npairs] = id;
+                w->pi[w->npairs] = h->ent[e].id;
+                w->npairs++;
+            }
+            return;
+        }
+        s = (s + 1) & h->mask;
+    }
+}
+
+static void dsweep_rec(dsweep_t *w, int hi, int depth, u128 prod, u64 res, int *idx) {
+    if (depth == w->R) {
+        u64 id = 0;
+        if (w->bitmask) { for (int j = 0; j < w->R; ++j) id |= 1ULL << idx[j]; }
+        else { for (int j = w->R - 1, pos = 0; j >= 0; --j, ++pos) id |= ((u64)idx[j]) << (6 * pos); }
+        w->leaves++;
+        dsweep_emit(w, res % w->M, id);
+        return;
+    }
+    const int t = w->R - depth - 1;
+    for (int i = hi; i >= t; --i) {
+        u128 np = prod * w->B[i];
+        u128 best = np;
+        for (int j = 0; j < t; ++j) best *= w->B[i - 1 - j];
+        if (best < w->Dmin) break;
+        idx[depth] = i;
+        dsweep_rec(w, i - 1, depth + 1, np, mulmod(res, w->B[i] % w->M, w->M), idx);
+        if (depth == 0) break;   /* depth 0: this thread's one largest index */
+    }
+}
+
+static void *dsweep_main(void *arg) {
+    dsweep_t *w = (dsweep_t *)arg;
+    int idx[16];
+    for (int top = w->nB - 1 - w->tid; top >= w->R - 1; top -= w->T) dsweep_rec(w, top, 0, (u128)1, 1 % w->M, idx);
+    return NULL;
+}
+
+/* stats[0]=D leaves, stats[1]=I leaves, stats[2]=pairs; times[0]=I hash s, times[1]=D sweep s. */
+long long or_join_dsweep(u64 M, int r, u64 P0modM, u64 icap_lo, u64 icap_hi, u64 dmin_lo, u64 dmin_hi,
+                         const u64 *B, int nB, const u64 *INS, int n, int split_idx, int hmax, int rank,
+                         int T, u64 *out_d, u64 *out_i, u64 cap, u64 *stats, double *times) {
+    if (T < 1) T = 1;
+    if (nB < 1 || nB > 64 || r < 1 || r > 16 || M == 0) return -1;
+    for (int i = 1; i < nB; ++i) if (B[i] <= B[i - 1]) return -4;
+    double t0 = now_s();
+    long long nI = or_idump(M, r, P0modM, icap_lo, icap_hi, INS, n, split_idx, hmax, rank, 0, n, NULL, NULL, 0);
+    if (nI < 0) return nI;
+    ictx_t c;
+    int rc = ictx_init(&c, M, r, P0modM, icap_lo, icap_hi, INS, n, split_idx, hmax, rank);
+    if (rc) return rc;
+    dhash_t h = {0};
+    u64 slots = 1024;
+    while (slots < 2 * (u64)nI) slots <<= 1;
+    h.mask = slots - 1;
+    h.head = (int64_t *)malloc(slots * sizeof(int64_t));
+    h.slotkey = (u64 *)malloc(slots * sizeof(u64));
+    h.capent = (u64)(nI ? nI : 1);
+    h.ent = (dent_t *)malloc(h.capent * sizeof(dent_t));
+    if (!h.head || !h.slotkey || !h.ent) { free(h.head); free(h.slotkey); free(h.ent); free(c.CB); return -2; }
+    for (u64 s = 0; s < slots; ++s) h.head[s] = -1;
+    isink_t is = {&h};
+    for (int i0 = 0; i0 < n; ++i0) itop(&c, i0, isink_put, &is);
+    free(c.CB);
+    double t1 = now_s();
+    dsweep_t *ws = (dsweep_t *)calloc(T, sizeof(dsweep_t));
+    pthread_t *th = (pthread_t *)calloc(T, sizeof(pthread_t));
+    for (int t = 0; t < T; ++t) {
+        ws[t] = (dsweep_t){M, r, mk128(dmin_lo, dmin_hi), B, nB, r > 10, &h, t, T, NULL, NULL, 0, 0, 0};
+        pthread_create(&th[t], NULL, dsweep_main, &ws[t]);
+    }
+    u64 leaves = 0, np = 0;
+    for (int t = 0; t < T; ++t) {
+        pthread_join(th[t], NULL);
+        leaves += ws[t].leaves;
+        for (u64 j = 0; j < ws[t].npairs; ++j, ++np)
+            if (np < cap) { out_d[np] = ws[t].pd[j]; out_i[np] = ws[t].pi[j]; }
+        free(ws[t].pd); free(ws[t].pi);
+    }
+    double t2 = now_s();
+    free(ws); free(th);
+    free(h.head); free(h.slotkey); free(h.ent);
+    if (stats) { stats[0] = leaves; stats[1] = h.nent; stats[2] = np; }
+    if (times) { times[0] = t1 - t0; times[1] = t2 - t1; }
+    return (long long)np;
+}

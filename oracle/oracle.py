@@ -45,6 +45,8 @@ def lib():
         L.or_cpu_join.restype = ctypes.c_longlong
         L.or_cpu_join.argtypes = [u64, ctypes.c_int, u64, u64, u64, u64, u64, P, ctypes.c_int, P, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, u64, P, P]
+        L.or_join_dsweep.restype = ctypes.c_longlong
+        L.or_join_dsweep.argtypes = L.or_cpu_join.argtypes
         _lib = L
     return _lib
 
@@ -140,6 +142,30 @@ def cpu_join(M: int, r: int, P0modM: int, Icap: int, Dmin: int, B, INS, split_id
         raise AssertionError("pair buffer too small")
     return dict(pairs_d=od[:n].copy(), pairs_i=oi[:n].copy(), nD=int(stats[0]), nI=int(stats[1]),
                 td=float(times[0]), ti=float(times[1]))
+
+
+def join_dsweep(M: int, r: int, P0modM: int, Icap: int, Dmin: int, B, INS, split_idx: Optional[int] = None,
+                hmax: Optional[int] = None, rank: bool = True, threads: int = 1, cap: int = 1 << 20):
+    """The CPU join with the roles swapped (hash the I side, sweep the D side on `threads` threads),
+    for D sides far larger than the I side. Same pair set as cpu_join. Returns dict(pairs_d, pairs_i,
+    nD, nI, ti (I hash s), td (D sweep s))."""
+    b = _arr(B)
+    ins = _arr(INS)
+    split_idx = 1 << 30 if split_idx is None else split_idx
+    hmax = 1 << 30 if hmax is None else hmax
+    il, ih = _split128(Icap)
+    dl, dh = _split128(Dmin)
+    od = np.empty(cap, np.uint64)
+    oi = np.empty(cap, np.uint64)
+    stats = np.zeros(3, np.uint64)
+    times = np.zeros(2, np.float64)
+    n = _check(lib().or_join_dsweep(M, r, P0modM, il, ih, dl, dh, _ptr(b), len(b), _ptr(ins), len(ins), split_idx,
+                                    hmax, int(rank), threads, _ptr(od), _ptr(oi), cap, _ptr(stats),
+                                    times.ctypes.data_as(ctypes.c_void_p)), "join_dsweep")
+    if n > cap:
+        raise AssertionError("pair buffer too small")
+    return dict(pairs_d=od[:n].copy(), pairs_i=oi[:n].copy(), nD=int(stats[0]), nI=int(stats[1]),
+                ti=float(times[0]), td=float(times[1]))
 
 
 def instance_records(inst, rank: Optional[bool] = None):

@@ -4,6 +4,7 @@ Run on an MI355X:  python -u -m pytest tests -m gpu -x -v --timeout 120 --timeou
 Each test prints the case it is about to run (instance, sizes) before touching the device.
 """
 import itertools
+import os
 from math import comb, prod
 
 import numpy as np
@@ -16,7 +17,8 @@ from cfjoin import pipeline
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN_LOGGED = __import__('os').path.join(__import__('os').path.dirname(__file__), 'golden', 'logged_instances.json')
+GOLDEN_DIR = os.path.join(os.path.dirname(__file__), 'golden')
+GOLDEN_LOGGED = os.path.join(GOLDEN_DIR, 'logged_instances.json')
 
 
 def say(*a):
@@ -411,3 +413,43 @@ def test_r8_anchor_d_filtered_matches_logged(workloads):
     res = pipeline.solve(inst, max_chunk=1 << 26)
     lg = inst.meta["logged"]
     assert (res.nD, res.nI, len(res.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]) == (18426621, 471754092, 0)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_c2_seeded_moduli(seed):
+    """Seeded k = 64, r = 6 moduli (pools of 200 primes after the cap is raised): bound s = 100 against
+    the CPU join here; s = 1000 (6e8-1.7e9 records) against the pair sets the CPU join wrote into
+    tests/golden/c2_pairs.json."""
+    import json
+    small = I.c2_instance(seed, 100)
+    say("c2", seed, small.M)
+    res = pipeline.solve(small)
+    ref = O.cpu_join(small.M, 6, small.P0 % small.M, small.Icap, small.Dmin, small.B0, small.INS, small.split_idx,
+                     small.hmax, small.rank_ids, threads=8)
+    assert (res.nD, res.nI) == (ref["nD"], ref["nI"])
+    assert np.array_equal(res.pair_array(), O.pair_set(ref["pairs_d"], ref["pairs_i"]))
+    fix = next(f for f in json.load(open(os.path.join(GOLDEN_DIR, "c2_pairs.json"))) if f["seed"] == seed)
+    big = I.c2_instance(seed, 1000)
+    assert big.M == fix["M"] and len(big.INS) == fix["nINS"]
+    res = pipeline.solve(big)
+    assert (res.nD, res.nI) == (fix["nD"], fix["nI"])
+    exp = np.array(fix["pairs"], dtype=np.uint64).reshape(-1, 2)
+    assert np.array_equal(res.pair_array(), exp)
+
+
+def test_r8_full_d_side_matches_oracle_fixture():
+    """The large-radius instance with the full D side (all C(64,8) = 4 426 165 368 D subsets, about
+    134 GiB of device memory) against the pair set the CPU oracle's swapped-role join wrote into
+    tests/golden/c4_full_pairs.json; I side in 2^26-record chunks."""
+    import copy
+    import json
+    from cfjoin import workloads as W
+    fix = json.load(open(os.path.join(GOLDEN_DIR, "c4_full_pairs.json")))
+    logged = W.logged_table(GOLDEN_LOGGED)
+    inst = copy.deepcopy(W.anchor("lambdaN", 8, logged))
+    inst.Dmin = 1
+    say("r8-full", inst.M)
+    res = pipeline.solve(inst, max_chunk=1 << 26)
+    assert (res.nD, res.nI) == (fix["nD"], fix["nI"]) == (4426165368, 471754092)
+    assert np.array_equal(res.pair_array(), np.array(fix["pairs"], dtype=np.uint64).reshape(-1, 2))
+    L.default_context().release_pool()

@@ -221,3 +221,60 @@ def test_anchor_pairs_contain_reference_matches(fam, fixtures, logged):
     anc = next(a for a in fixtures["anchors"] if a["family"] == fam)
     ref = {(tuple(a), tuple(b)) for a, b in anc["matches"]}
     assert ref and ref <= mine
+
+
+def test_dsweep_join_equals_cpu_join(logged, workloads):
+    """The swapped-role join (hash I, sweep D on several threads) returns the pair multiset, the
+    record counts and hence the logged counts of the D-hashing join."""
+    for r in _small_logged(logged)[::4] + [None]:
+        if r is None:
+            inst = I.c1_instance(0, 100)
+        else:
+            inst = fam_inst(workloads, r["family"], r["D"], r["r"])
+        args = (inst.M, inst.r, inst.P0 % inst.M, inst.Icap, inst.Dmin, inst.B0, inst.INS, inst.split_idx, inst.hmax,
+                inst.rank_ids)
+        a = O.cpu_join(*args, threads=2)
+        b = O.join_dsweep(*args, threads=3)
+        assert (a["nD"], a["nI"]) == (b["nD"], b["nI"])
+        assert np.array_equal(O.pair_set(a["pairs_d"], a["pairs_i"]), O.pair_set(b["pairs_d"], b["pairs_i"]))
+        if r is not None:
+            assert (b["nD"], b["nI"], len(b["pairs_d"])) == (r["nD"], r["nI"], r["pairs"])
+
+
+def _fixture_pairs_hold(inst, pairs):
+    """Every fixture pair decodes to r distinct D and I indices whose residues agree mod M
+    (P0 * prod(I) == prod(D)), with prod(D) >= Dmin and prod(I) < Icap, and the pair list has no repeats."""
+    from cfjoin import pipeline
+    d_dec, i_dec = pipeline.decoders(inst)
+    assert len({tuple(p) for p in pairs}) == len(pairs)
+    for dv, iv in pairs:
+        d, i = d_dec(int(dv)), i_dec(int(iv))
+        assert len(set(d)) == len(set(i)) == inst.r
+        pd = prod(inst.B0[x] for x in d)
+        pi = prod(inst.INS[x] for x in i)
+        assert (inst.P0 * pi - pd) % inst.M == 0
+        assert pd >= inst.Dmin and pi < inst.Icap
+
+
+def test_large_radius_fixture_pairs_are_exact(logged):
+    """The 78 pairs the swapped-role join found on the full-D r = 8 instance are real joins."""
+    import copy
+    with open(os.path.join(GOLDEN, "c4_full_pairs.json")) as f:
+        fix = json.load(f)
+    inst = copy.deepcopy(W.anchor("lambdaN", 8, logged))
+    inst.Dmin = 1
+    assert (fix["M"], fix["r"]) == (inst.M, 8) and len(fix["pairs"]) == 78
+    _fixture_pairs_hold(inst, fix["pairs"])
+
+
+def test_seeded_modulus_fixture_pairs_are_exact():
+    """The pairs the CPU join found on the seeded k = 64 moduli (bound s = 1000) are real joins, on
+    the moduli the generator still draws."""
+    with open(os.path.join(GOLDEN, "c2_pairs.json")) as f:
+        fix = json.load(f)
+    for f in fix:
+        M, cap = I.c2_modulus(f["seed"])
+        assert M == f["M"]
+        assert 200 <= len(I.pool(M, cap)) <= 500
+        inst = I.c2_instance(f["seed"], f["s"])
+        _fixture_pairs_hold(inst, f["pairs"])

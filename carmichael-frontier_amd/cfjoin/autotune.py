@@ -16,12 +16,12 @@ import numpy as np
 
 from . import lib as L
 
-DEFAULTS = dict(slice_kib=2048, d_per_bucket=5, seg_fill_pct=70, probe_blocks=2048)
+DEFAULTS = dict(slice_kib=2048, d_per_bucket=2, seg_fill_pct=70, probe_blocks=256)
 CANDIDATES = {
     "slice_kib": [1024, 2048, 4096],
-    "d_per_bucket": [4, 5, 6, 8],
+    "d_per_bucket": [1, 2, 3],
     "seg_fill_pct": [60, 70, 80],
-    "probe_blocks": [1024, 2048, 4096],
+    "probe_blocks": [],    # one 1024-thread probe block per CU: its filter slice fills the LDS
 }
 
 

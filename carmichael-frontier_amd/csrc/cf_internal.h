@@ -269,23 +269,34 @@ struct IChunkLayout {   // the buffers behind a SegLayout
 // launch shapes and geometry targets that change speed, never results (cf_tuning, 0 = default)
 struct Tuning {
     int slice_kib = 0;       // probe-table slice per partition (default 2048)
-    int d_per_bucket = 0;    // D records per bucket (default 5)
+    int d_per_bucket = 0;    // D records per bucket (default 2)
     int seg_fill_pct = 0;    // expected fill of a capacity segment, percent (default 70)
-    int probe_blocks = 0;    // probe grid (default 2048)
+    int probe_blocks = 0;    // probe grid, 1024-thread blocks (default 256: one per CU)
 };
 u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout)
+// tp receives the chunk's tile owner map (2 entries per 2048 leaves), which the exact layout and
+// pass 2 take as given
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf);
+                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
+                      DBuf<u64> &tp);
 // count, scan and scatter: n records exactly, in partition order
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec);
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp);
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
 // rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)
+// bucket filter of a D table: bits per bucket for the partition geometry (0: none fits in LDS), its
+// size in 32-bit words, and its construction from the table slots and bucket starts
+int filter_bits(const KeyHash &kh);
+// candidate slots the probe may leave unused (added to the candidate buffer)
+u64 cand_margin(const Tuning &tn);
+u64 filter_words(const KeyHash &kh, int FB);
+void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const KeyHash &kh, int FB, u32 *F);
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
-                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand, const Tuning &tn);
+                      const u32 *F, int FB, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand,
+                      const Tuning &tn);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst);

@@ -8,25 +8,25 @@
 //
 // Hash of a leaf: h = (xm + rot32(x2 mod 2^a) + koff) * 0x9E3779B97F4A7C15 -- no CRT step is needed
 // to place a record, only to compare one exactly. h gives the record's partition, hash range,
-// bucket and fingerprint (keyhash.h); NB is chosen for about 5 D records per bucket, and a
-// partition's buckets are contiguous, so its slice of the probe table is NB * 32 / P bytes (about
+// bucket and fingerprint (keyhash.h); NB is chosen for about 2 D records per bucket, and a
+// partition's buckets are contiguous, so its slice of the probe table is NB * 16 / P bytes (about
 // 2 MB).
 //
-// Probe table T: 16 slots of 16 bits per bucket (32 bytes), slot k holding 0x8000 | f of the
-// bucket's k-th D record; slot 7 also carries 0x4000 when the bucket holds more than 8 records. A
-// probe reads the first 16 bytes and the second only under that flag (about 7% of buckets at 5 per
-// bucket), and touches the D records only on a fingerprint hit or a full bucket (16 or more D
-// records), then compares exact keys.
+// Probe table T: 8 slots of 16 bits per bucket (16 bytes), slot k holding 0x8000 | f of the
+// bucket's k-th D record; slot 7 also carries 0x4000 when the bucket holds more than 8 records
+// (rare at 2 per bucket: such a bucket makes every probe of it a candidate). Bucket filter F: a
+// few bits per bucket, one per class of fingerprint low bits present; the probe keeps its
+// partition's slice of F in LDS and reads T only for records whose class is present.
 //
 //  D side: records counted per bucket and scattered into bucket order (DRec = exact key, id); the
-//          scatter writes the table slots as it goes.
+//          scatter writes the table slots as it goes; F is built from T.
 //  I side: leaves scattered by partition as 8-byte records (low word: bucket offset << fpb | f,
 //          high word: leaf index in the chunk): each block ranks its 2048 leaves by partition in LDS
 //          and writes one run per partition into slots sized from the expected share.
 //  Probe:  pass 1 reads the records in partition order: each of the 8 XCDs takes tiles of its own
-//          contiguous slice from a queue, so an XCD works on about one partition at a time and
-//          that partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
-// HBM traffic per I record: 8 B written by the scatter + 4 B read by the probe.
+//          contiguous slice from a queue, so an XCD works on one partition at a time and that
+//          partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
+// HBM traffic per I record: 8 B written by the scatter + 8 B read by the probe.
 // Join semantics: every equal-key (D, I) pair (gpu/mitm_gpu2.py:107-119).
 #include "tree_dev.h"
 
@@ -38,7 +38,7 @@ constexpr int LT = PB * LPT;       // 2048 leaves per tile
 constexpr int PI = 16;             // records per thread in the probe
 constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
-constexpr int TSLOTS = 16;         // table slots per bucket
+constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
 
 struct LeafSrc {
     const SideDesc *descs;
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
             const u64 b0 = bstart[b];
             DR[b0 + slot] = DRec{key, id};
             if (slot < (u32)TSLOTS) {
-                const u32 more = (slot == 7 && bstart[b + 1] - b0 > 8) ? 0x4000u : 0u;
+                const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));
             }
         }
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const u64 *__
             const u32 slot = atomicAdd(&cnt[b], 1u);
             DR[r0 + pos[b] + slot] = kv[k];
             if (slot < (u32)TSLOTS) {
-                const u32 more = (slot == 7 && pos[b + 1] - pos[b] > 8) ? 0x4000u : 0u;
+                const u32 more = (slot == TSLOTS - 1 && pos[b + 1] - pos[b] > (u32)TSLOTS) ? 0x4000u : 0u;
                 T[(b_lo + b) * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(hv[k]));
             }
         }
@@ -589,7 +589,9 @@ __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ni
 // Exact key and id of leaf lo + leaf (rare: only on a fingerprint hit).
 __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key, u64 &id) {
     const u64 j = L.lo + leaf;
-    const size_t p = find_parent(L.off, 0, L.nn - 1, j);
+    // the tile owner map (when given) brackets the parent to the tile's few nodes
+    const size_t p = L.tp ? find_parent(L.off, L.tp[2 * (leaf / LT)], L.tp[2 * (leaf / LT) + 1], j)
+                          : find_parent(L.off, 0, L.nn - 1, j);
     const Node nd = L.nodes[p];
     const SideDesc &s = L.descs[nd.meta >> 8];
     const int x = child_index(s, nd, (int)(j - L.off[p]));
@@ -600,105 +602,219 @@ __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key,
 }
 
 // Is the slot pattern fs2 (0x8000 | f in both halves of a word, compared under mask mm) among the
-// 4 words of a half bucket?
+// 4 words of a half bucket? A slot matches when its masked value XOR the pattern is zero, so the
+// 8 slots reduce to one packed 16-bit minimum.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bool half_hit(const uint4 &a, u32 fs2, u32 mm) {
-    const u32 w[4] = {a.x, a.y, a.z, a.w};
-    bool hit = false;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const u32 x = (w[k] & mm) ^ fs2;
-        hit |= ((x & 0xFFFFu) == 0) | ((x >> 16) == 0);
-    }
-    return hit;
+    const u16x2 x0 = __builtin_bit_cast(u16x2, (a.x & mm) ^ fs2);
+    const u16x2 x1 = __builtin_bit_cast(u16x2, (a.y & mm) ^ fs2);
+    const u16x2 x2 = __builtin_bit_cast(u16x2, (a.z & mm) ^ fs2);
+    const u16x2 x3 = __builtin_bit_cast(u16x2, (a.w & mm) ^ fs2);
+    const u16x2 m = __builtin_elementwise_min(__builtin_elementwise_min(x0, x1), __builtin_elementwise_min(x2, x3));
+    return (m.x == 0) | (m.y == 0);
 }
 
 constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
 
-// Probe, pass 1: each record reads the first half of its bucket (the second too when slot 7 says
-// the bucket holds more than 8); a record whose fingerprint is among the bucket's slots, or whose
-// bucket is full, is a candidate. A tile is SPT consecutive segments of the chunk's layout; each
-// wave of the block takes whole segments, so a segment's partition and start are uniform across
-// the wave. Candidates (rare) are placed by the wave that finds them, one reservation per batch
-// that has any, into cand[0, ccap); *ncand counts them all, so the caller sees an overflow and
-// repeats with room. Tiles come from a queue per XCD (xq[8], zero at launch) over that XCD's
-// contiguous slice of tiles, so an XCD works through its partitions one after another and the
-// table slice of the partition in hand stays in its L2; every block leaves once its XCD's queue
-// is past the slice.
-constexpr int SPT = 8;             // segments per tile: two per wave
+// Bucket filter: FB bits per bucket (FB = 4, 8 or 16), bucket b at bits [b * FB, (b + 1) * FB) of
+// an array of 32-bit words; bit f of a bucket is set when one of its D records has a fingerprint
+// whose low log2(FB) bits are f, and all FB bits are set for an overflowing bucket (more records
+// than slots).
+// An I record whose bit is clear has no fingerprint match and no full bucket, so the probe rejects
+// it without reading the table. One thread per word: it reads the slots of its 32 / FB buckets.
+__global__ void __launch_bounds__(PB) k_bucket_filter(const unsigned short *__restrict__ T,
+                                                      const u64 *__restrict__ bstart, u64 NB, int FB,
+                                                      u32 *__restrict__ F, u64 nwords) {
+    const u32 per = 32u / (u32)FB;
+    const u32 ones = FB == 32 ? ~0u : (1u << FB) - 1u;
+    for (u64 w = (u64)blockIdx.x * PB + threadIdx.x; w < nwords; w += (u64)gridDim.x * PB) {
+        u32 word = 0;
+        for (u32 k = 0; k < per; ++k) {
+            const u64 b = w * per + k;
+            if (b >= NB) break;
+            const u64 tot = bstart[b + 1] - bstart[b];
+            u32 bits = 0;
+            if (tot > (u64)TSLOTS) {
+                bits = ones;
+            } else {
+                for (u32 q = 0; q < (u32)tot; ++q) bits |= 1u << (T[b * TSLOTS + q] & (FB - 1));
+            }
+            word |= bits << (k * FB);
+        }
+        F[w] = word;
+    }
+}
+
+// Probe, pass 1: each record reads its bucket's 8 slots (16 bytes); a record whose fingerprint is
+// among them, or whose bucket overflows (slot 7 flagged: more D records than slots), is a
+// candidate. With a bucket filter (FB > 0) the block keeps its partition's slice of the filter in
+// LDS and a record reads the table only if its filter bit is set.
+// A tile is up to SPT consecutive segments of one partition (a partition's nblk segments are
+// consecutive in the layout); each wave takes whole segments, so a segment's partition and start
+// are uniform across the wave. Candidates are placed by the wave that finds them into slots it
+// reserves CAND_CHUNK at a time from *ncand (slots it leaves unused hold CAND_NONE), within
+// cand[0, ccap); *ncand counts every reserved slot, so the caller sees an overflow and repeats with
+// room. Tiles come from a queue per XCD (xq[8], zero at launch)
+// over that XCD's contiguous slice of tiles, so an XCD works through its partitions one after
+// another and the table slice of the partition in hand stays in its L2; every block leaves once its
+// XCD's queue is past the slice.
+constexpr int PBP = 1024;                       // threads per probe block (16 waves)
+constexpr u64 CAND_NONE = ~0ull;                // an unused candidate slot
+constexpr u32 CAND_CHUNK = 256;                 // candidate slots a wave reserves at a time
+constexpr int SPW = 2;                          // segments per wave per tile
+constexpr int SPT = SPW * (PBP / 64);           // segments per tile
+constexpr int FWMAX = 32768;                    // filter words a block holds (128 KB of LDS)
 
 __device__ __forceinline__ u64 seg_start(const SegLayout &S, u64 g) { return S.O ? S.O[g] : (g << S.capl); }
 
-__global__ void __launch_bounds__(PB) k_part_probe(KeyHash kh, SegLayout S, const uint2 *__restrict__ recs,
-                                                   const unsigned short *__restrict__ T, u32 *__restrict__ xq,
-                                                   unsigned long long *__restrict__ ncand, u64 ccap,
-                                                   u64 *__restrict__ cand) {
+// Raw buffer descriptor over [base, base + bytes): loads at byte offsets past the end return zero.
+// The third dword is the gfx9 descriptor's format word (32-bit data format).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, u64 bytes) {
+    const int n = bytes > 0x7FFFFFFFull ? 0x7FFFFFFF : (int)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, 0x00020000);
+}
+__device__ __forceinline__ u64 uniform_u64(u64 v) {
+    return ((u64)(u32)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+           (u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)v);
+}
+
+__global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, const uint2 *__restrict__ recs,
+                                                    const unsigned short *__restrict__ T,
+                                                    const u32 *__restrict__ F, int FB, u32 *__restrict__ xq,
+                                                    unsigned long long *__restrict__ ncand, u64 ccap,
+                                                    u64 *__restrict__ cand) {
+    __shared__ u32 sF[FWMAX];
     __shared__ u64 s_t;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const u64 ntiles = (S.nseg + SPT - 1) / SPT;
+    const int lane = threadIdx.x & 63;
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int P = 1 << kh.lgP;
+    const u64 tpp = (S.nblk + SPT - 1) / SPT;   // tiles per partition
+    const u64 ntiles = tpp * (u64)P;
     const u32 xcd = blockIdx.x % NXCD;
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const u32 mm = (0x8000u | fmask) * 0x10001u;
     const u64 K = kh_k(kh);
+    const u32 fbm = FB > 0 ? (u32)FB - 1u : 0u;
+    u64 cur_part = ~0ull;
+    u32 fbit0 = 0;                               // bit of the partition's first bucket within sF
+    // the wave's reserved candidate slots [cbase, cbase + cleft): one counter update per CAND_CHUNK
+    // candidates instead of one per round that has any
+    u64 cbase = 0;
+    u32 cleft = 0;
     for (;;) {
         // the barrier at the end of the loop orders this write after every wave's read of the last tile
         if (threadIdx.x == 0) s_t = t_lo + atomicAdd(&xq[xcd], 1u);
         __syncthreads();
-        const u64 t = s_t;
+        const u64 t = uniform_u64(s_t);
         if (t >= t_hi) break;
-        for (int sj = 0; sj < SPT / (PB / 64); ++sj) {
-            const u64 g = t * SPT + (u64)wv * (SPT / (PB / 64)) + sj;
-            if (g >= S.nseg) break;
-            const u32 fill = S.fill[g];
-            const uint2 *rec = recs + seg_start(S, g);
-            const u64 bpart = (g / S.nblk) * K;
-            for (u32 v0 = 0; v0 < fill; v0 += 64 * PROBE_BATCH) {
-                u32 lo[PROBE_BATCH];
-                uint4 ta[PROBE_BATCH];
-                u32 hmask = 0;
+        const u64 part = t / tpp;
+        if (FB > 0 && part != cur_part) {
+            // this block's first tile of a partition: its filter slice into LDS (the barrier above
+            // ordered every read of the previous slice before this)
+            const u64 bit0 = part * K * (u64)FB;
+            const u64 w0 = bit0 >> 5;
+            const u64 nw = ((bit0 + K * (u64)FB + 31) >> 5) - w0;
+            for (u64 i = threadIdx.x; i < nw; i += PBP) sF[i] = F[w0 + i];
+            fbit0 = (u32)(bit0 & 31);
+            cur_part = part;
+            __syncthreads();
+        }
+        const u64 gp = part * S.nblk;            // the partition's first segment
+        const u64 gend = gp + S.nblk;
+        // the partition's table slice: byte offset hoff * 16 of bucket hoff
+        const __amdgpu_buffer_rsrc_t tr = buf_rsrc(T + part * K * TSLOTS, K * TSLOTS * 2);
+        // the wave's segments: fills and starts (wave-uniform) read before the first is probed;
+        // a segment's records through a descriptor that ends at its fill
+        u32 fills[SPW];
+        u64 starts[SPW];
+        __amdgpu_buffer_rsrc_t rr[SPW];
+        u32 rounds_end = 0;
 #pragma unroll
-                for (int q = 0; q < PROBE_BATCH; ++q) {
-                    const u32 v = v0 + (u32)q * 64 + lane;
-                    lo[q] = v < fill ? rec[v].x : 0u;
+        for (int sj = 0; sj < SPW; ++sj) {
+            const u64 g = gp + (t - part * tpp) * SPT + (u64)wv * SPW + sj;
+            fills[sj] = g < gend ? S.fill[g] : 0u;
+            starts[sj] = g < gend ? seg_start(S, g) : 0;
+            rr[sj] = buf_rsrc(recs + starts[sj], (u64)fills[sj] * sizeof(uint2));
+            rounds_end = fills[sj] > rounds_end ? fills[sj] : rounds_end;
+        }
+        // the wave's SPW segments are probed together, 64 * PROBE_BATCH slots of each per round
+        // (one round for a capacity segment), so one round trip serves all of them
+        for (u32 v0 = 0; v0 < rounds_end; v0 += 64 * PROBE_BATCH) {
+            constexpr int NB2 = SPW * PROBE_BATCH;
+            static_assert(NB2 <= 32, "one mask bit per record of a round");
+            u32 lo[NB2];
+            uint4 ta[NB2];
+            u32 valid = 0;
+#pragma unroll
+            for (int e = 0; e < NB2; ++e) {
+                const int sj = e / PROBE_BATCH, q = e % PROBE_BATCH;
+                const u32 v = v0 + (u32)q * 64 + lane;
+                // past the fill the descriptor returns 0: bucket 0 of the partition, masked below
+                lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 0);
+                valid |= (u32)(v < fills[sj]) << e;
+            }
+            u32 look = valid;
+            if (FB > 0) {
+                u32 pass = 0;
+#pragma unroll
+                for (int e = 0; e < NB2; ++e) {
+                    const u32 bit = fbit0 + (lo[e] >> kh.fpb) * (u32)FB + (lo[e] & fbm);
+                    pass |= ((sF[bit >> 5] >> (bit & 31)) & 1u) << e;
                 }
+                look &= pass;
+            }
+            // first halves; a record not looked up takes an offset past the slice (the load returns
+            // zeros without touching memory and cannot match)
 #pragma unroll
-                for (int q = 0; q < PROBE_BATCH; ++q)
-                    ta[q] = *(const uint4 *)(T + (bpart + (lo[q] >> kh.fpb)) * TSLOTS);
+            for (int e = 0; e < NB2; ++e) {
+                const u32 off = (look >> e) & 1u ? (lo[e] >> kh.fpb) * (2u * TSLOTS) : 0x7FFFFFF0u;
+                const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(tr, (int)off, 0, 0);
+                ta[e] = make_uint4(x[0], x[1], x[2], x[3]);
+            }
+            // a fingerprint among the bucket's slots, or the overflow flag of slot 7, is a candidate
+            u32 hmask = 0;
 #pragma unroll
-                for (int q = 0; q < PROBE_BATCH; ++q) {
-                    const u32 fs2 = (0x8000u | (lo[q] & fmask)) * 0x10001u;
-                    bool hit = half_hit(ta[q], fs2, mm);
-                    if (ta[q].w & 0x40000000u) {
-                        // more than 8 D records: the second half, and slot 15 taken means full
-                        const uint4 tb = *(const uint4 *)(T + (bpart + (lo[q] >> kh.fpb)) * TSLOTS + 8);
-                        hit |= half_hit(tb, fs2, mm) | ((tb.w >> 16) != 0);
-                    }
-                    if (hit && v0 + (u32)q * 64 + lane < fill) hmask |= 1u << q;
+            for (int e = 0; e < NB2; ++e) {
+                const u32 fs2 = (0x8000u | (lo[e] & fmask)) * 0x10001u;
+                hmask |= (u32)(half_hit(ta[e], fs2, mm) | ((ta[e].w & 0x40000000u) != 0)) << e;
+            }
+            hmask &= look;
+            if (__any(hmask != 0)) {
+                const u32 c = (u32)__popc(hmask);
+                u32 inc = c;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const u32 o = __shfl_up(inc, d, 64);
+                    if (lane >= d) inc += o;
                 }
-                if (__any(hmask != 0)) {
-                    const u32 c = (u32)__popc(hmask);
-                    u32 inc = c;
-#pragma unroll
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const u32 o = __shfl_up(inc, d, 64);
-                        if (lane >= d) inc += o;
-                    }
-                    unsigned long long base = 0;
-                    if (lane == 63) base = atomicAdd(ncand, (unsigned long long)inc);
-                    base = __shfl(base, 63, 64);
-                    u64 at = base + inc - c;
-                    const u64 e0 = seg_start(S, g) + v0 + lane;
-                    while (hmask) {
-                        const int q = __ffs(hmask) - 1;
-                        hmask &= hmask - 1;
-                        if (at < ccap) cand[at] = e0 + (u64)q * 64;
-                        ++at;
-                    }
+                const u32 total = (u32)__builtin_amdgcn_readlane((int)inc, 63);
+                if (total > cleft) {
+                    // the rest of the current reservation is marked unused; a new one is taken
+                    for (u32 i = lane; i < cleft; i += 64)
+                        if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
+                    const u32 want = total > CAND_CHUNK ? total : CAND_CHUNK;
+                    unsigned long long nb = 0;
+                    if (lane == 0) nb = atomicAdd(ncand, (unsigned long long)want);
+                    cbase = uniform_u64(__shfl(nb, 0, 64));
+                    cleft = want;
+                }
+                u64 at = cbase + inc - c;
+                cbase += total;
+                cleft -= total;
+                while (hmask) {
+                    const int e = __ffs(hmask) - 1;
+                    hmask &= hmask - 1;
+                    if (at < ccap) cand[at] = starts[e / PROBE_BATCH] + v0 + (u64)(e % PROBE_BATCH) * 64 + lane;
+                    ++at;
                 }
             }
         }
         __syncthreads();
     }
+    for (u32 i = lane; i < cleft; i += 64)
+        if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
 }
 
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
@@ -714,7 +830,9 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, 
                                                          u64 *__restrict__ out_inst) {
     const u64 nc = *ncand < ccap ? *ncand : ccap;
     for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
-        const u64 e = cand[c];
+        const u64 raw = cand[c];
+        if (raw == CAND_NONE) continue;
+        const u64 e = raw;
         u64 g;
         if (S.O) {   // the last segment starting at or before e (it is not empty)
             u64 a = 0, bb = S.nseg - 1;
@@ -854,7 +972,8 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) {
 }
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
-                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf) {
+                      int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
+                      DBuf<u64> &tp) {
     u64 tpb;
     u32 nblk;
     cap_geometry(n, kh, tn, tpb, nblk);
@@ -864,17 +983,15 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     lay.capl = CAPL;
     lay.fill = DBuf<u32>(ctx, lay.nseg);
     fill_u32(ctx, ovf, 1, 0);
-    DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, (const u64 *)nullptr,
               CAPL, lay.fill.p, ovf, (uint2 *)rec);
 }
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec) {
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp) {
     const int P = 1 << kh.lgP;
-    DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    LeafSrc L{descs, nodes, nn, off, lo, n, depth, tp};
     const u64 ntiles = (n + LT - 1) / LT;
     // about 8 resident blocks per CU over 256 CUs, each with a long contiguous leaf range
     const u64 tpb = (ntiles + 16383) / 16384;
@@ -890,27 +1007,52 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
               (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
 }
 
+u64 cand_margin(const Tuning &tn) {
+    // unused slots the probe may leave: a partial reservation per wave of the grid
+    const u64 g = tn.probe_blocks > 0 ? (u64)tn.probe_blocks : 256;
+    return g * (PBP / 64) * CAND_CHUNK;
+}
+
+int filter_bits(const KeyHash &kh) {
+    // the widest filter whose partition slice (with up to 31 bits of misalignment) fits in LDS
+    const u64 K = kh_k(kh);
+    for (int fb = 16; fb >= 4; fb >>= 1)
+        if (((31 + K * (u64)fb + 31) >> 5) <= (u64)FWMAX) return fb;
+    return 0;
+}
+
+u64 filter_words(const KeyHash &kh, int FB) { return FB ? (kh_nb(kh) * (u64)FB + 31) / 32 : 0; }
+
+void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const KeyHash &kh, int FB, u32 *F) {
+    const u64 nw = filter_words(kh, FB);
+    if (!nw) return;
+    CF_LAUNCH(ctx, "k_bucket_filter", k_bucket_filter, dim3(grid_for(nw, PB, 65536)), dim3(PB), 0, T, bstart,
+              kh_nb(kh), FB, F, nw);
+}
+
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
-                      u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand, const Tuning &tn) {
+                      const u32 *F, int FB, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand,
+                      const Tuning &tn) {
     fill_u32(ctx, xq, NXCD, 0);
     fill_u64(ctx, (u64 *)ncand, 1, 0);
     if (!S.nseg) return;
     // tiles are dealt per XCD, so every XCD needs at least one block: the grid is a non-zero
-    // multiple of the XCD count, about as many blocks as the chip holds at once
-    const u64 ntiles = (S.nseg + SPT - 1) / SPT;
-    const u64 gmax = tn.probe_blocks > 0 ? (u64)tn.probe_blocks : 2048;
+    // multiple of the XCD count; by default one 1024-thread block per CU (the filter slice takes
+    // most of a CU's LDS)
+    const u64 ntiles = (u64)((S.nblk + SPT - 1) / SPT) << kh.lgP;
+    const u64 gmax = tn.probe_blocks > 0 ? (u64)tn.probe_blocks : 256;
     unsigned g = (unsigned)(ntiles < gmax ? ntiles : gmax);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PB), 0, kh, S, (const uint2 *)rec, T, xq, ncand, ccap,
-              cand);
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PBP), 0, kh, S, (const uint2 *)rec, T, F, FB, xq,
+              ncand, ccap, cand);
 }
 
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec,
+                      int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
                       const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
                       u64 *oinst) {
-    LeafSrc L{descs, nodes, nn, off, lo, 0, depth, nullptr};
+    LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
     CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, (const uint2 *)rec, S,
               ncand, ccap,
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);

@@ -22,6 +22,7 @@ struct KeyHash {
 
 constexpr u64 CF_RANGE_BUCKETS_LDS = 8192;   // buckets per range the LDS counting sort takes
 constexpr int CF_RANGE_BITS_LDS = 12;         // ranges the LDS partition pass takes (2^12)
+constexpr int CF_TABLE_SLOTS = 8;             // 16-bit fingerprint slots per bucket (16 bytes)
 
 CF_HD u64 kh_nb(const KeyHash &kh) { return kh.KR << kh.lgR; }
 CF_HD u64 kh_k(const KeyHash &kh) { return kh.KR << (kh.lgR - kh.lgP); }
@@ -35,9 +36,9 @@ CF_HD u64 hbucket(u64 h, const KeyHash &kh) {
 CF_HD u32 hoff(u64 h, const KeyHash &kh) { return (u32)(hbucket(h, kh) - (u64)hpart(h, kh.lgP) * kh_k(kh)); }
 CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
 
-// Geometry for a radius group with nD D records: about 5 D records per bucket by default, so a
-// 16-slot bucket is rarely full (Poisson tail P(X >= 16) ~ 7e-5); P = 2^lgP probe partitions, as few
-// as keep one partition's slice of the 32-byte-per-bucket table within about 2 MB by default (an
+// Geometry for a radius group with nD D records: about 2 D records per bucket by default, so an
+// 8-slot bucket rarely overflows (Poisson tail P(X > 8) ~ 2.4e-4); P = 2^lgP probe partitions, as few
+// as keep one partition's slice of the 16-byte-per-bucket table within about 2 MB by default (an
 // XCD's L2 holds it while the partition is probed), 16 <= P <= 256; as few hash ranges as keep a range within the buckets
 // the LDS counting sort takes (at most 2^12 ranges; beyond that the D side is bucketed by global
 // atomics). An I record keeps the bucket offset within its partition and 32 - that many bits of
@@ -50,12 +51,13 @@ inline int kh_bitlen(u64 v) {
     }
     return b;
 }
-// d_per_bucket and slice_bytes are the tunable targets (defaults 5 and 2 MiB).
-inline bool kh_choose(u64 nD, KeyHash &kh, int d_per_bucket = 5, u64 slice_bytes = 2u << 20) {
-    const u64 dpb = d_per_bucket > 0 ? (u64)d_per_bucket : 5;
+// d_per_bucket and slice_bytes are the tunable targets (defaults 2 and 2 MiB). A D side too large
+// for 2^24 buckets per partition at that many records per bucket takes the fewest more per bucket
+// that fit (such buckets overflow more often, which costs candidates, not results).
+inline bool kh_choose_at(u64 nD, KeyHash &kh, u64 dpb, u64 slice_bytes) {
     const u64 want = nD / dpb > 16 ? nD / dpb : 16;
     int lgP = 4;
-    while (lgP < 8 && want * 32 / ((u64)1 << lgP) > slice_bytes) ++lgP;
+    while (lgP < 8 && want * (2 * CF_TABLE_SLOTS) / ((u64)1 << lgP) > slice_bytes) ++lgP;
     int lgR = lgP;
     while (lgR < CF_RANGE_BITS_LDS && ((want + ((u64)1 << lgR) - 1) >> lgR) > CF_RANGE_BUCKETS_LDS) ++lgR;
     const u64 KR = (want + ((u64)1 << lgR) - 1) >> lgR;
@@ -66,4 +68,9 @@ inline bool kh_choose(u64 nD, KeyHash &kh, int d_per_bucket = 5, u64 slice_bytes
     kh.fpb = 32 - obits < 14 ? 32 - obits : 14;
     kh.pad = 0;
     return obits <= 24;
+}
+inline bool kh_choose(u64 nD, KeyHash &kh, int d_per_bucket = 2, u64 slice_bytes = 2u << 20) {
+    for (u64 dpb = d_per_bucket > 0 ? (u64)d_per_bucket : 2; dpb <= 64; ++dpb)
+        if (kh_choose_at(nD, kh, dpb, slice_bytes)) return true;
+    return false;
 }

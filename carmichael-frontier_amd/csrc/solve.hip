@@ -352,8 +352,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             DBuf<u32> bcount(ctx, pt.NB);
             DBuf<u64> bstart(ctx, pt.NB + 1);
             DBuf<u64> DR(ctx, 2 * (nD ? nD : 1));
-            DBuf<unsigned short> T(ctx, pt.NB * 16);
-            if (nD) fill_u32(ctx, (u32 *)T.p, pt.NB * 8, 0);
+            DBuf<unsigned short> T(ctx, pt.NB * CF_TABLE_SLOTS);
+            if (nD) fill_u32(ctx, (u32 *)T.p, pt.NB * CF_TABLE_SLOTS / 2, 0);
             if (nD && !(d_lds && part_d_lds(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
                                             DR.p, T.p, kerr.p))) {
                 fill_u32(ctx, bcount.p, pt.NB, 0);
@@ -363,6 +363,10 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                 part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
                                     bcount.p, DR.p, T.p);
             }
+            // bucket filter: the probe tests it in LDS before reading the table
+            const int FB = nD ? filter_bits(kh) : 0;
+            DBuf<u32> F(ctx, std::max<u64>(filter_words(kh, FB), 1));
+            part_filter(ctx, T.p, bstart.p, kh, FB, F.p);
             fd = Frontier();
             if (!capturing) gc.mark();
             // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
@@ -372,6 +376,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             if (f.total && nD) {
                 DBuf<u64> rec;
                 DBuf<u64> cand;
+                DBuf<u64> tp;   // the chunk's tile owner map
                 IChunkLayout lay;
                 u64 cap_m = 0, ccap = 0;
                 for (u64 L = 0; L < f.total; L += chunk) {
@@ -381,35 +386,36 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                         cap_m = need;
                         rec = DBuf<u64>(ctx, cap_m);
                     }
-                    if (ccap < std::max<u64>(m >> 10, 1 << 16)) {
-                        ccap = std::max<u64>(m >> 10, 1 << 16);
+                    // candidates: fingerprint hits and records of overflowing buckets (~1e-4 of the records)
+                    if (ccap < std::max<u64>(m >> 5, 1 << 16) + cand_margin(b->tune)) {
+                        ccap = std::max<u64>(m >> 5, 1 << 16) + cand_margin(b->tune);
                         cand = DBuf<u64>(ctx, ccap);
                     }
                     // one pass into segments sized for the expected share; the probe that follows
                     // is wasted only if a segment overflowed (heavily repeated keys), and then the
                     // chunk is laid out exactly and probed again
                     part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
-                                     ovf.p);
-                    part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                     ovf.p, tp);
+                    part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
                                      cand.p, b->tune);
                     const u64 ov = host_value_u32(ctx, tape, ovf.p);
                     u64 nc = host_value(ctx, tape, flags.p);
                     if (ov) {
                         part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                           rec.p);
-                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                                           rec.p, tp.p);
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p, b->tune);
                         nc = host_value(ctx, tape, flags.p);
                     }
                     if (nc > ccap) {
                         // more candidates than room (heavily repeated keys): probe again with room for all
-                        ccap = nc;
+                        ccap = nc + cand_margin(b->tune);
                         cand = DBuf<u64>(ctx, ccap);
-                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, xq.p, (unsigned long long *)flags.p, ccap,
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p, b->tune);
                     }
                     part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
-                                     (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
+                                     tp.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
                                      DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }
@@ -511,9 +517,10 @@ int cf_batch_footprint(cf_batch *b, uint64_t *bytes, uint64_t *nD_total, uint64_
         sd += nD;
         si += nI;
         const Partitioning pt = assign_buckets(nD, b->tune);
-        const u64 tables = 16 * nD + 44 * pt.NB;            // DR, T, bstart, counters
+        const u64 tables = 16 * nD + 30 * pt.NB;            // DR, T, bstart, counters, filter
         const u64 d_build = tables + 24 * nD + 60 * nnD;    // + range-sorted temporaries + D parents
-        const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) + 60 * nnI + nI / 128;
+        const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) +
+                            60 * nnI + nI / 4;
         peak = std::max(peak, std::max(d_build, i_phase));
     }
     *bytes = peak + (64u << 20);

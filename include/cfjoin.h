@@ -183,9 +183,9 @@ int cf_batch_destroy(cf_batch *batch);
  * in the reference, whose kernels run with fixed launch shapes (gpu/mitm_gpu2.py:93-97). */
 typedef struct cf_tuning {
     int32_t slice_kib;        /* probe-table slice per partition, KiB (default 2048) */
-    int32_t d_per_bucket;     /* D records per table bucket (default 5, at most 12) */
+    int32_t d_per_bucket;     /* D records per table bucket (default 2, at most 12) */
     int32_t seg_fill_pct;     /* expected fill of an I capacity segment, percent (default 70, at most 90) */
-    int32_t probe_blocks;     /* probe grid, blocks (default 2048) */
+    int32_t probe_blocks;     /* probe grid, 1024-thread blocks (default 256: one per CU) */
 } cf_tuning;
 int cf_batch_set_tuning(cf_batch *batch, const cf_tuning *tuning);
 

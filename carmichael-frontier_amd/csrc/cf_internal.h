@@ -242,12 +242,6 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
 SideSpec spec_of_d(const cf_instance &in, u64 koff);
 SideSpec spec_of_i(const cf_instance &in, u64 koff);
 
-// D side bucketed through LDS (hash ranges, then a counting sort per range); false when the
-// bucket geometry is too large for it (the caller then uses the per-record atomic path)
-// kerr: set non-zero if a record falls outside its range's buckets (an internal inconsistency;
-// the record is then dropped instead of written out of bounds)
-bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
-                const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr);
 // Where one chunk's I records live: nseg segments, partition-major (nblk per partition); segment g
 // holds fill[g] records from slot O[g] (exact layout) or g << capl (capacity layout, O null).
 struct SegLayout {
@@ -279,10 +273,33 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // pass 2 take as given
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp);
+                      DBuf<u64> &tp, u64 *keys = nullptr);
 // count, scan and scatter: n records exactly, in partition order
+// keys (D side): each record's exact key at the record's slot
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp);
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
+                        u64 *keys = nullptr);
+// D side bucketed from its partitioned records (after the scatter with keys): windows of 2^wb
+// buckets within each partition (dwin_geometry: false if a partition would need more than 256);
+// part_d_windows moves the records into fixed-capacity window regions (wreg: P * R * dwin_capacity()
+// records of 16 bytes; *ovf set if a window overflowed); after the caller's exclusive scan of wcount
+// into wbase (P * R + 1 entries), part_d_sort writes bstart (NB + 1), DR in bucket order
+// ({key, D leaf}), the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket).
+bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R);
+u64 dwin_capacity();
+void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
+                    const u64 *keys, u32 *wcount, void *wreg, u32 *ovf);
+void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
+                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB);
+// the D leaves of a radius group, for recomputing a pair's D id in pass 2
+struct DLeaves {
+    const SideDesc *descs;
+    const Node *nodes;
+    size_t nn;
+    const u64 *off;
+    u64 n;
+    const u64 *tp;    // tile owner map of [0, n), or null
+};
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
 // rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)
 // bucket filter of a D table: bits per bucket for the partition geometry (0: none fits in LDS), its
@@ -297,9 +314,9 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
                       const Tuning &tn);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
-                      const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
-                      const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
-                      u64 *oinst);
+                      const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
+                      const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
+                      u64 cap, u64 *od, u64 *oi, u64 *oinst);
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
 
 // joins

@@ -51,7 +51,7 @@ struct LeafSrc {
 };
 
 struct DRec {
-    u64 key, id;
+    u64 key, ref;     // exact key and D leaf index (the record id is recomputed for an output pair)
 };
 
 // residue state of the leaf choosing index x below node nd
@@ -232,6 +232,31 @@ __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, con
     }
 }
 
+// Hashes and exact keys of the LPT leaves of this lane (the D side of the partitioned build).
+__device__ __forceinline__ void tile_hash_keys(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT],
+                                               u64 (&key)[LPT]) {
+    if (T.fast) {
+        const FastCtx c(S);
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            u64 xm, x2;
+            int x;
+            u32 q;
+            h[k] = leaf_fast(S, c, i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
+            key[k] = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
+        }
+    } else {
+        for (int k = 0; k < LPT; ++k) {
+            const u32 i = threadIdx.x + PB * k;
+            u64 xm, x2;
+            LeafRef r;
+            h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
+            key[k] = cf_key(xm, x2, r.s->mc) + r.s->koff;
+        }
+    }
+}
+
 // Owners of the first and the last leaf of every LT-leaf tile of [lo, lo + n): one binary search
 // per lane, all tiles at once, so no leaf kernel starts a tile with a serial search.
 __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off, size_t nn, u64 lo, u64 n,
@@ -261,8 +286,7 @@ __global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 
 }
 
 // Scatter into bucket order; the rank a record takes in its bucket is also its table slot.
-__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key,
-                                       u64 &id);
+__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key);
 __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
                                                        u32 *__restrict__ bcur, DRec *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
@@ -273,12 +297,12 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + PB * k;
             if (i >= tl.cnt) continue;
-            u64 h, key, id;
-            d_leaf(L, tl, S, i, h, key, id);
+            u64 h, key;
+            d_leaf(L, tl, S, i, h, key);
             const u64 b = hbucket(h, kh);
             const u32 slot = atomicAdd(&bcur[b], 1u);
             const u64 b0 = bstart[b];
-            DR[b0 + slot] = DRec{key, id};
+            DR[b0 + slot] = DRec{key, tl.j0 + i};
             if (slot < (u32)TSLOTS) {
                 const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));
@@ -288,168 +312,19 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
     }
 }
 
-// ---- D side in LDS: hash range partition, then a counting sort per range -----------------
-// The D records are first placed in their 2^lgR hash ranges (range d owns buckets
-// [d * KR, (d + 1) * KR)), two passes per block range as on the I side; then one block per range
-// counts its records per bucket in LDS, writes the bucket starts, the records in bucket order and
-// the table slots. No global atomics.
-constexpr int PDMAX = 1 << CF_RANGE_BITS_LDS;          // hash ranges
-constexpr int WMAX = (int)CF_RANGE_BUCKETS_LDS;         // buckets per range
-
-// D leaf of a tile: hash, exact key and id
-__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key,
-                                       u64 &id) {
+// D leaf of a tile: hash and exact key
+__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key) {
     u64 xm, x2;
     if (T.fast) {
         const FastCtx c(S);
         int x;
         u32 q;
         h = leaf_fast(S, c, i, xm, x2, x, q);
-        const u64 t = ((x2 - xm) * S.minv) & c.mask2;
-        key = xm + c.m * t + c.koff;
-        id = id_update(L.descs[S.inst], L.nodes[T.p0 + q].id, x, L.depth);
+        key = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
     } else {
         LeafRef r;
         h = leaf_generic(L, T, S, i, xm, x2, r);
         key = cf_key(xm, x2, r.s->mc) + r.s->koff;
-        id = id_update(*r.s, r.nd.id, r.x, L.depth);
-    }
-}
-
-__global__ void __launch_bounds__(PB) k_drange_hist(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, u32 *__restrict__ H) {
-    __shared__ TileLds S;
-    __shared__ u32 hist[PDMAX];
-    const int PD = 1 << kh.lgR;
-    for (int d = threadIdx.x; d < PD; d += PB) hist[d] = 0;
-    const u64 ntiles = (L.n + LT - 1) / LT;
-    const u64 t0 = (u64)blockIdx.x * tpb;
-    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
-    for (u64 t = t0; t < t1; ++t) {
-        const Tile T = tile_setup(L, t, S);
-        u64 h[LPT];
-        tile_hashes(L, T, S, h);
-#pragma unroll
-        for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&hist[hrange(h[k], kh)], 1u);
-        __syncthreads();
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < PD; d += PB) H[(u64)d * nblk + blockIdx.x] = hist[d];
-}
-
-// Record slots of block k in range d start at O[d * nblk + k]; the block keeps its running slot per
-// range in LDS (the D side of a join taking this path has fewer than 2^32 records).
-__global__ void __launch_bounds__(PB) k_drange_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
-                                                       const u64 *__restrict__ O, u64 *__restrict__ th,
-                                                       DRec *__restrict__ tkv) {
-    __shared__ TileLds S;
-    __shared__ u32 slot[PDMAX];
-    const int PD = 1 << kh.lgR;
-    for (int d = threadIdx.x; d < PD; d += PB) slot[d] = (u32)O[(u64)d * nblk + blockIdx.x];
-    const u64 ntiles = (L.n + LT - 1) / LT;
-    const u64 t0 = (u64)blockIdx.x * tpb;
-    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
-    for (u64 t = t0; t < t1; ++t) {
-        const Tile T = tile_setup(L, t, S);
-        for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
-            if (i >= T.cnt) continue;
-            u64 h, key, id;
-            d_leaf(L, T, S, i, h, key, id);
-            const u32 g = atomicAdd(&slot[hrange(h, kh)], 1u);
-            th[g] = h;
-            tkv[g] = DRec{key, id};
-        }
-        __syncthreads();
-    }
-}
-
-// One block per hash range d: records [dstart[d], dstart[d + 1]) of (th, tkv), whose buckets are
-// [d * KR, (d + 1) * KR), go to bucket order. A record outside those buckets (which the geometry
-// rules out) sets *kerr and is dropped rather than written out of bounds.
-constexpr int DB = 8;              // records per thread loaded before any is used
-__global__ void __launch_bounds__(PB) k_drange_buckets(KeyHash kh, const u64 *__restrict__ th,
-                                                       const DRec *__restrict__ tkv, const u64 *__restrict__ dstart,
-                                                       u64 *__restrict__ bstart, DRec *__restrict__ DR,
-                                                       unsigned short *__restrict__ T, u32 *__restrict__ kerr) {
-    __shared__ u32 cnt[WMAX];
-    __shared__ u32 pos[WMAX + 1];
-    __shared__ u32 s_w[PB / 64];
-    const u64 d = blockIdx.x;
-    const u32 w = (u32)kh.KR;
-    const u64 b_lo = d * kh.KR;
-    const u64 r0 = dstart[d], r1 = dstart[d + 1];
-    for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
-    __syncthreads();
-    for (u64 e0 = r0; e0 < r1; e0 += (u64)PB * DB) {
-        u64 hv[DB];
-#pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            const u64 e = e0 + threadIdx.x + (u64)PB * k;
-            hv[k] = e < r1 ? th[e] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            if (e0 + threadIdx.x + (u64)PB * k >= r1) continue;
-            const u64 b = hbucket(hv[k], kh) - b_lo;
-            if (b < w) atomicAdd(&cnt[b], 1u);
-            else atomicOr(kerr, 1u);
-        }
-    }
-    __syncthreads();
-    // exclusive scan of cnt[0, w) into pos (PB threads, WMAX / PB consecutive entries each)
-    constexpr int PER = WMAX / PB;
-    u32 sum = 0;
-    const u32 base = threadIdx.x * PER;
-    for (int k = 0; k < PER; ++k) sum += base + k < w ? cnt[base + k] : 0u;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    u32 inc = sum;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const u32 o = __shfl_up(inc, dd, 64);
-        if (lane >= dd) inc += o;
-    }
-    if (lane == 63) s_w[wv] = inc;
-    __syncthreads();
-    u32 run = inc - sum;
-    for (int q = 0; q < wv; ++q) run += s_w[q];
-    for (int k = 0; k < PER; ++k) {
-        const u32 i = base + k;
-        if (i < w) {
-            pos[i] = run;
-            bstart[b_lo + i] = r0 + run;
-            run += cnt[i];
-        }
-    }
-    if (threadIdx.x == PB - 1) pos[w] = run;
-    if (d + 1 == gridDim.x && threadIdx.x == PB - 1) bstart[b_lo + w] = r0 + run;
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < w; i += PB) cnt[i] = 0;
-    __syncthreads();
-    for (u64 e0 = r0; e0 < r1; e0 += (u64)PB * DB) {
-        u64 hv[DB];
-        DRec kv[DB];
-#pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            const u64 e = e0 + threadIdx.x + (u64)PB * k;
-            if (e < r1) {
-                hv[k] = th[e];
-                kv[k] = tkv[e];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < DB; ++k) {
-            if (e0 + threadIdx.x + (u64)PB * k >= r1) continue;
-            const u64 bb = hbucket(hv[k], kh) - b_lo;
-            if (bb >= w) continue;
-            const u32 b = (u32)bb;
-            const u32 slot = atomicAdd(&cnt[b], 1u);
-            DR[r0 + pos[b] + slot] = kv[k];
-            if (slot < (u32)TSLOTS) {
-                const u32 more = (slot == TSLOTS - 1 && pos[b + 1] - pos[b] > (u32)TSLOTS) ? 0x4000u : 0u;
-                T[(b_lo + b) * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(hv[k]));
-            }
-        }
     }
 }
 
@@ -485,15 +360,19 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
 // *ovf (its excess records are not written), and the caller redoes the chunk with the exact layout.
 // The capacity layout needs no first pass: its segments are sized for the expected share
 // (hashing spreads records evenly) with a margin of about 8 standard deviations.
+// With KEYS (the D side) every record's exact key goes to keys[] at the record's slot.
+template <bool KEYS>
 __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                      const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
-                                                     u32 *__restrict__ ovf, uint2 *__restrict__ rec) {
+                                                     u32 *__restrict__ ovf, uint2 *__restrict__ rec,
+                                                     u64 *__restrict__ keys) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
     __shared__ u64 cur[256];
     __shared__ u8_t s_p[LT];
     __shared__ u32 s_ovf;
+    __shared__ u64 s_k[KEYS ? LT : 1];
     uint2 *s_r = (uint2 *)S.xm;        // LT records staged over the parents' residue state (16 KB)
     static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
     const int P = 1 << kh.lgP;
@@ -509,7 +388,9 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
         hist[threadIdx.x] = 0;
         const Tile T = tile_setup(L, t, S);
         u64 h[LPT];
-        tile_hashes(L, T, S, h);
+        u64 kk[KEYS ? LPT : 1];
+        if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
+        else tile_hashes(L, T, S, h);
         u32 rank[LPT];
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
@@ -540,6 +421,7 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
                 const u32 at = loc[part] + rank[k];
                 s_r[at] = make_uint2((hoff(h[k], kh) << kh.fpb) | (hfp(h[k]) & fmask), (u32)(T.j0 + i - L.lo));
                 s_p[at] = (u8_t)part;
+                if constexpr (KEYS) s_k[at] = kk[k];
             }
         }
         __syncthreads();
@@ -557,6 +439,7 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
                 g = (((u64)pi * nblk + blockIdx.x) << capl) + o;
             }
             rec[g] = s_r[q];
+            if constexpr (KEYS) keys[g] = s_k[q];
         }
         __syncthreads();
         if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
@@ -817,9 +700,232 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
         if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
 }
 
+// ---- D side, partitioned: the D records were scattered by partition like the I side (with their
+// exact keys alongside). A partition's K buckets are cut into windows of WIN = 2^wb buckets
+// (about 2048 D records each); two more passes bucket them with no per-record global atomic:
+//   window scatter: each block ranks a tile of its partition's records by window in LDS and
+//          reserves room per window with one atomic per (tile, window) in a fixed-capacity region
+//          of CAPW records per window, then writes one run per window; a window whose records do
+//          not fit sets *ovf (heavily repeated keys: the caller then buckets this group with
+//          per-record global atomics instead)
+//   window sort: one block per window loads its records into LDS, counting-sorts them by bucket,
+//          and writes the window's bucket starts, its records in bucket order ({key, leaf}), its
+//          table slots and its filter words, all contiguous
+// The window starts in DR are the exclusive scan of the window counts (windows are contiguous
+// bucket ranges in partition order, so that order is bucket order).
+constexpr int CAPW = 3072;             // records a window holds
+constexpr int WPT = 4;                 // D records of a window-scatter tile: 4 segments, one per wave
+
+struct DWin {
+    int wb;        // log2 buckets per window
+    u32 R;         // windows per partition
+};
+
+__global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLayout S, const uint2 *__restrict__ recs,
+                                                     const u64 *__restrict__ keys, u32 *__restrict__ wcount,
+                                                     uint4 *__restrict__ wreg, u32 *__restrict__ ovf) {
+    __shared__ u32 hist[256];
+    __shared__ u32 loc[256];
+    __shared__ u32 woff[256];
+    __shared__ u32 s_w[PB / 64];
+    __shared__ uint4 s_r[WPT * 512];
+    __shared__ u8_t s_win[WPT * 512];
+    __shared__ u32 s_ovf;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u64 tpp = (S.nblk + WPT - 1) / WPT;
+    const u64 part = blockIdx.x / tpp;
+    const u64 g = part * S.nblk + (blockIdx.x - part * tpp) * WPT + wv;
+    const bool has = g < (part + 1) * S.nblk;
+    const u32 fill = has ? S.fill[g] : 0u;
+    const u64 st = has ? seg_start(S, g) : 0;
+    if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_ovf = 0;
+    __syncthreads();
+    // a segment holds at most 512 records (capacity layout) -- the exact layout's larger segments
+    // are taken 512 at a time by the loop below
+    for (u32 c0 = 0;; c0 += 512) {
+        const u32 cn = c0 < fill ? (fill - c0 < 512u ? fill - c0 : 512u) : 0u;
+        uint2 r[8];
+        u64 kk[8];
+        u32 rank[8], win[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const u32 v = (u32)k * 64 + lane;
+            if (v < cn) {
+                r[k] = recs[st + c0 + v];
+                kk[k] = keys[st + c0 + v];
+                win[k] = (r[k].x >> kh.fpb) >> dw.wb;
+                rank[k] = atomicAdd(&hist[win[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // tile offsets of the windows (block scan of hist), and one reservation per window present
+        const u32 v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
+        {
+            u32 inc = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const u32 o = __shfl_up(inc, d, 64);
+                if (lane >= d) inc += o;
+            }
+            if (lane == 63) s_w[wv] = inc;
+            __syncthreads();
+            u32 pre = 0;
+            for (int q = 0; q < wv; ++q) pre += s_w[q];
+            if (threadIdx.x < 256) {
+                loc[threadIdx.x] = pre + inc - v;
+                woff[threadIdx.x] = v ? atomicAdd(&wcount[part * dw.R + threadIdx.x], v) : 0u;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const u32 vv = (u32)k * 64 + lane;
+            if (vv < cn) {
+                const u32 at = loc[win[k]] + rank[k] + 0u;
+                s_r[at] = make_uint4(r[k].x, r[k].y, (u32)kk[k], (u32)(kk[k] >> 32));
+                s_win[at] = (u8_t)win[k];
+            }
+        }
+        __syncthreads();
+        u32 tot = 0;
+        for (int q = 0; q < PB / 64; ++q) tot += s_w[q];
+        for (u32 q = threadIdx.x; q < tot; q += PB) {
+            const u32 w = s_win[q];
+            const u32 o = woff[w] + (q - loc[w]);
+            if (o >= (u32)CAPW) {
+                s_ovf = 1;
+                continue;
+            }
+            wreg[((u64)part * dw.R + w) * CAPW + o] = s_r[q];
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+        // every wave of the block runs the same number of rounds: the largest of its segments
+        const u32 more = __syncthreads_or(c0 + 512 < fill);
+        if (!more) break;
+    }
+    if (threadIdx.x == 0 && s_ovf) atomicOr(ovf, 1u);
+}
+
+constexpr int DSB = 512;               // threads of a window-sort block
+__global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u32 *__restrict__ wcount,
+                                                   const u64 *__restrict__ wbase, const uint4 *__restrict__ wreg,
+                                                   u64 *__restrict__ bstart, DRec *__restrict__ DR,
+                                                   unsigned short *__restrict__ T, u32 *__restrict__ F, int FB) {
+    constexpr int WMAXB = 4096;        // buckets a window may have
+    __shared__ uint4 s_r[CAPW];
+    __shared__ unsigned short s_idx[CAPW];
+    __shared__ u32 pos[WMAXB + 1];     // per-bucket counts, then (scanned in place) bucket starts
+    __shared__ u32 s_w[DSB / 64];
+    u32 *cnt = pos;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const u64 pw = blockIdx.x;
+    const u64 part = pw / dw.R, w = pw - part * dw.R;
+    const u64 K = kh_k(kh);
+    const u32 WIN = 1u << dw.wb;
+    const u64 b_lo = w * WIN;                                  // first bucket, within the partition
+    const u32 nb = (u32)((b_lo + WIN <= K ? b_lo + WIN : K) - b_lo);
+    const u32 n = wcount[pw] < (u32)CAPW ? wcount[pw] : (u32)CAPW;
+    const u64 base = wbase[pw];
+    const u64 gb = part * K + b_lo;                            // first bucket, global
+    for (u32 i = threadIdx.x; i <= nb; i += DSB) cnt[i] = 0;
+    for (u32 i = threadIdx.x; i < n; i += DSB) s_r[i] = wreg[pw * CAPW + i];
+    __syncthreads();
+    constexpr int RPT = (CAPW + DSB - 1) / DSB;
+    u32 rk[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const u32 i = threadIdx.x + (u32)k * DSB;
+        if (i < n) rk[k] = atomicAdd(&cnt[(s_r[i].x >> kh.fpb) - (u32)b_lo], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt[0, nb) into pos (DSB threads, consecutive runs of PER entries)
+    constexpr int PER = WMAXB / DSB;
+    u32 sum = 0, cv[PER];
+    const u32 b0 = threadIdx.x * PER;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        cv[k] = b0 + k < nb ? cnt[b0 + k] : 0u;
+        sum += cv[k];
+    }
+    u32 inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    u32 run = inc - sum;
+    for (int q = 0; q < wv; ++q) run += s_w[q];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const u32 i = b0 + k;
+        if (i < nb) {
+            pos[i] = run;
+            bstart[gb + i] = base + run;
+            run += cv[k];
+        }
+    }
+    if (threadIdx.x == DSB - 1) pos[nb] = run;
+    // the last window of the last partition closes bstart
+    if (pw + 1 == gridDim.x && threadIdx.x == DSB - 1) bstart[gb + nb] = base + run;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const u32 i = threadIdx.x + (u32)k * DSB;
+        if (i < n) s_idx[pos[(s_r[i].x >> kh.fpb) - (u32)b_lo] + rk[k]] = (unsigned short)i;
+    }
+    __syncthreads();
+    // records in bucket order
+    for (u32 k = threadIdx.x; k < n; k += DSB) {
+        const uint4 r = s_r[s_idx[k]];
+        DR[base + k] = DRec{(u64)r.z | ((u64)r.w << 32), (u64)r.y};
+    }
+    // table slots: one bucket per thread, 16 bytes each
+    const u32 fmask = (1u << kh.fpb) - 1u;
+    for (u32 i = threadIdx.x; i < nb; i += DSB) {
+        const u32 c = pos[i + 1] - pos[i];
+        u32 sl[TSLOTS];
+#pragma unroll
+        for (int q = 0; q < TSLOTS; ++q)
+            sl[q] = (u32)q < c ? (0x8000u | (s_r[s_idx[pos[i] + q]].x & fmask)) : 0u;
+        if (c > (u32)TSLOTS) sl[TSLOTS - 1] |= 0x4000u;
+        static_assert(TSLOTS == 8, "a bucket is one 16-byte store");
+        *(uint4 *)(T + (gb + i) * TSLOTS) =
+            make_uint4(sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16), sl[4] | (sl[5] << 16), sl[6] | (sl[7] << 16));
+    }
+    // filter words: FB bits per bucket; a word shared with a neighbouring window is merged atomically
+    if (FB > 0) {
+        const u64 bit0 = gb * (u64)FB, bit1 = (gb + nb) * (u64)FB;
+        const u32 fones = FB >= 32 ? ~0u : (1u << FB) - 1u;
+        for (u64 wd = (bit0 >> 5) + threadIdx.x; wd <= ((bit1 - 1) >> 5); wd += DSB) {
+            u32 word = 0;
+            for (u32 q = 0; q < 32u / (u32)FB; ++q) {
+                const u64 gbk = (wd * 32 + (u64)q * FB) / FB;   // global bucket of this field
+                if (gbk < gb || gbk >= gb + nb) continue;
+                const u32 i = (u32)(gbk - gb);
+                const u32 c = pos[i + 1] - pos[i];
+                u32 bits = 0;
+                if (c > (u32)TSLOTS) {
+                    bits = fones;
+                } else {
+                    for (u32 e = 0; e < c; ++e) bits |= 1u << (s_r[s_idx[pos[i] + e]].x & (u32)(FB - 1));
+                }
+                word |= bits << (q * FB);
+            }
+            const bool edge = wd * 32 < bit0 || wd * 32 + 32 > bit1;
+            if (edge) atomicOr(&F[wd], word);
+            else F[wd] = word;
+        }
+    }
+}
+
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
 // with the D records of its bucket; every equal key emits a pair.
-__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, const uint2 *__restrict__ recs,
+__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, KeyHash kh,
+                                                         const uint2 *__restrict__ recs,
                                                          SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
@@ -852,9 +958,11 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, KeyHash kh, 
         for (u64 q = lo; q < hi; ++q) {
             const DRec d = DR[q];
             if (d.key != key) continue;
+            u64 dkey, did;
+            leaf_exact(LD, d.ref, dkey, did);
             const unsigned long long at = atomicAdd(npairs, 1ULL);
             if (at < cap) {
-                out_d[at] = d.id;
+                out_d[at] = did;
                 out_i[at] = iid;
                 out_inst[at] = inst_of_key(koffs, ninst, key);
             }
@@ -928,30 +1036,39 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
               (DRec *)DR, T);
 }
 
-// D side in LDS (when the geometry allows it, see d_ranges): records of D leaves [0, n) in bucket
-// order into DR (16-byte records), bucket starts into bstart (NB + 1), table slots into T (zeroed).
-bool part_d_lds(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n, int depth,
-                const KeyHash &kh, u64 *bstart, void *DR, unsigned short *T, u32 *kerr) {
-    if (kh.lgR > CF_RANGE_BITS_LDS || kh.KR > CF_RANGE_BUCKETS_LDS || n >= (1ULL << 32)) return false;
-    const int PD = 1 << kh.lgR;
-    DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
-    const u64 ntiles = (n + LT - 1) / LT;
-    const u64 tpb = (ntiles + 1023) / 1024;
-    const u32 nblk = (u32)((ntiles + tpb - 1) / tpb);
-    DBuf<u32> H(ctx, (size_t)PD * nblk);
-    DBuf<u64> O(ctx, (size_t)PD * nblk + 1);
-    DBuf<u64> dstart(ctx, PD + 1);
-    DBuf<u64> th(ctx, n);
-    DBuf<DRec> tkv(ctx, n);
-    CF_LAUNCH(ctx, "k_drange_hist", k_drange_hist, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, H.p);
-    scan_u32_to_u64_async(ctx, H.p, O.p, (size_t)PD * nblk);
-    CF_LAUNCH(ctx, "k_part_bases", k_part_bases, dim3(1), dim3(PB), 0, O.p, nblk, PD, dstart.p);
-    CF_LAUNCH(ctx, "k_drange_scatter", k_drange_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, O.p, th.p, tkv.p);
-    CF_LAUNCH(ctx, "k_drange_buckets", k_drange_buckets, dim3(PD), dim3(PB), 0, kh, th.p, tkv.p, dstart.p, bstart,
-              (DRec *)DR, T, kerr);
-    return true;
+// D side, partitioned (after part_scatter_cap / part_scatter_exact with keys): window geometry,
+// window scatter and window sort.
+bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R) {
+    // about 2048 D records per window at the group's records per bucket
+    const u64 NB = kh_nb(kh);
+    const double dpb = NB ? (double)nD / (double)NB : 1.0;
+    wb = 12;
+    while (wb > 4 && (double)((u64)1 << wb) * dpb > 2048.0) --wb;
+    const u64 K = kh_k(kh);
+    const u64 r = (K + ((u64)1 << wb) - 1) >> wb;
+    R = (u32)r;
+    return r <= 256;
 }
+
+void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
+                    const u64 *keys, u32 *wcount, void *wreg, u32 *ovf) {
+    const int P = 1 << kh.lgP;
+    fill_u32(ctx, wcount, (size_t)P * R, 0);
+    fill_u32(ctx, ovf, 1, 0);
+    if (!S.nseg) return;
+    const u64 tpp = (S.nblk + WPT - 1) / WPT;
+    CF_LAUNCH(ctx, "k_dwin_scatter", k_dwin_scatter, dim3((unsigned)(tpp * P)), dim3(PB), 0, kh, DWin{wb, R}, S,
+              (const uint2 *)rec, keys, wcount, (uint4 *)wreg, ovf);
+}
+
+void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
+                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB) {
+    const int P = 1 << kh.lgP;
+    CF_LAUNCH(ctx, "k_dwin_sort", k_dwin_sort, dim3((unsigned)((u64)P * R)), dim3(DSB), 0, kh, DWin{wb, R}, wcount,
+              wbase, (const uint4 *)wreg, bstart, (DRec *)DR, T, F, FB);
+}
+
+u64 dwin_capacity() { return CAPW; }
 
 // Capacity layout of a chunk of n I leaves: blocks of tpb tiles whose expected share per partition
 // is a set fraction of a 2^9-slot segment (default 70%: mean ~350, segment ~ mean + 8 sd).
@@ -973,7 +1090,7 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) {
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp) {
+                      DBuf<u64> &tp, u64 *keys) {
     u64 tpb;
     u32 nblk;
     cap_geometry(n, kh, tn, tpb, nblk);
@@ -984,12 +1101,17 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     lay.fill = DBuf<u32>(ctx, lay.nseg);
     fill_u32(ctx, ovf, 1, 0);
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk, (const u64 *)nullptr,
-              CAPL, lay.fill.p, ovf, (uint2 *)rec);
+    if (keys)
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter<true>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, keys);
+    else
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter<false>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, (u64 *)nullptr);
 }
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
-                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp) {
+                        u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
+                        u64 *keys) {
     const int P = 1 << kh.lgP;
     LeafSrc L{descs, nodes, nn, off, lo, n, depth, tp};
     const u64 ntiles = (n + LT - 1) / LT;
@@ -1003,8 +1125,12 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     lay.O = DBuf<u64>(ctx, lay.nseg + 1);
     CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
-    CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
-              (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
+    if (keys)
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter<true>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, keys);
+    else
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter<false>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, (u64 *)nullptr);
 }
 
 u64 cand_margin(const Tuning &tn) {
@@ -1049,11 +1175,12 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
 
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
-                      const unsigned long long *ncand, u64 ccap, const u64 *cand, const u64 *koffs, int ninst,
-                      const u64 *bstart, const void *DR, unsigned long long *npairs, u64 cap, u64 *od, u64 *oi,
-                      u64 *oinst) {
+                      const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
+                      const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
+                      u64 cap, u64 *od, u64 *oi, u64 *oinst) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
-    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, kh, (const uint2 *)rec, S,
+    LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, 0, dl.n, depth, dl.tp};
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const uint2 *)rec, S,
               ncand, ccap,
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
 }

@@ -306,7 +306,7 @@ u64 default_chunk() {
 
 // tape: null for a plain eager solve; RECORD to note every value read back; REPLAY while the
 // stream is being captured (no value is read back and no timing is taken)
-void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_lds = true, Tape *tape = nullptr) {
+void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_part = true, Tape *tape = nullptr) {
     cf_ctx *ctx = b->ctx;
     const int n = b->n;
     const auto &groups = b->groups;
@@ -349,25 +349,54 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
             const Partitioning pt = assign_buckets(nD, b->tune);
             const KeyHash &kh = pt.kh;
-            DBuf<u32> bcount(ctx, pt.NB);
             DBuf<u64> bstart(ctx, pt.NB + 1);
             DBuf<u64> DR(ctx, 2 * (nD ? nD : 1));
             DBuf<unsigned short> T(ctx, pt.NB * CF_TABLE_SLOTS);
-            if (nD) fill_u32(ctx, (u32 *)T.p, pt.NB * CF_TABLE_SLOTS / 2, 0);
-            if (nD && !(d_lds && part_d_lds(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
-                                            DR.p, T.p, kerr.p))) {
-                fill_u32(ctx, bcount.p, pt.NB, 0);
-                part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bcount.p);
-                scan_u32_to_u64_async(ctx, bcount.p, bstart.p, pt.NB);
-                fill_u32(ctx, bcount.p, pt.NB, 0);   // reused as the scatter cursors
-                part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
-                                    bcount.p, DR.p, T.p);
-            }
             // bucket filter: the probe tests it in LDS before reading the table
             const int FB = nD ? filter_bits(kh) : 0;
             DBuf<u32> F(ctx, std::max<u64>(filter_words(kh, FB), 1));
-            part_filter(ctx, T.p, bstart.p, kh, FB, F.p);
-            fd = Frontier();
+            DBuf<u64> tpD;   // tile owner map of the D leaves (partitioned build)
+            if (nD) {
+                fill_u32(ctx, F.p, filter_words(kh, FB), 0);
+                int wb = 0;
+                u32 R = 0;
+                bool done = false;
+                if (d_part && nD < ((u64)1 << 32) && dwin_geometry(kh, nD, wb, R)) {
+                    // partitioned like the I side (keys alongside), then moved into bucket windows
+                    // and counting-sorted one window per block in LDS
+                    IChunkLayout dl;
+                    const u64 slots = std::max<u64>(part_cap_slots(nD, kh, b->tune), nD);
+                    DBuf<u64> drec(ctx, slots), dkeys(ctx, slots);
+                    part_scatter_cap(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, kh, b->tune, dl,
+                                     drec.p, ovf.p, tpD, dkeys.p);
+                    if (host_value_u32(ctx, tape, ovf.p))
+                        part_scatter_exact(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, kh, dl,
+                                           drec.p, tpD.p, dkeys.p);
+                    const u64 nw = (u64)pt.P() * R;
+                    DBuf<u32> wcount(ctx, nw);
+                    DBuf<u64> wreg(ctx, 2 * nw * dwin_capacity()), wbase(ctx, nw + 1);
+                    part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, dkeys.p, wcount.p, wreg.p, ovf.p);
+                    if (!host_value_u32(ctx, tape, ovf.p)) {
+                        scan_u32_to_u64_async(ctx, wcount.p, wbase.p, nw);
+                        part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, bstart.p, DR.p, T.p, F.p, FB);
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    // per-record global atomics (D sides of 2^32 records or more, or a window
+                    // overflowing on heavily repeated keys)
+                    fill_u32(ctx, (u32 *)T.p, pt.NB * CF_TABLE_SLOTS / 2, 0);
+                    DBuf<u32> cnt(ctx, pt.NB);
+                    fill_u32(ctx, cnt.p, pt.NB, 0);
+                    part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, cnt.p);
+                    scan_u32_to_u64_async(ctx, cnt.p, bstart.p, pt.NB);
+                    fill_u32(ctx, cnt.p, pt.NB, 0);   // reused as the scatter cursors
+                    part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
+                                        cnt.p, DR.p, T.p);
+                    part_filter(ctx, T.p, bstart.p, kh, FB, F.p);
+                }
+            }
+            const DLeaves dleaves{d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, tpD.p};
             if (!capturing) gc.mark();
             // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
             // scatter), probe the table, check the candidates exactly
@@ -415,7 +444,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                                          cand.p, b->tune);
                     }
                     part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
-                                     tp.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
+                                     tp.p, dleaves, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
                                      DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
                 }
             }
@@ -518,9 +547,10 @@ int cf_batch_footprint(cf_batch *b, uint64_t *bytes, uint64_t *nD_total, uint64_
         si += nI;
         const Partitioning pt = assign_buckets(nD, b->tune);
         const u64 tables = 16 * nD + 30 * pt.NB;            // DR, T, bstart, counters, filter
-        const u64 d_build = tables + 24 * nD + 60 * nnD;    // + range-sorted temporaries + D parents
+        // + partitioned D records and keys + D parents (kept through the I phase for pass 2)
+        const u64 d_build = tables + 16 * std::max<u64>(part_cap_slots(nD, pt.kh, b->tune), nD) + 32 * nD + 60 * nnD;
         const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) +
-                            60 * nnI + nI / 4;
+                            60 * (nnI + nnD) + nI / 4 + nD / 128;
         peak = std::max(peak, std::max(d_build, i_phase));
     }
     *bytes = peak + (64u << 20);

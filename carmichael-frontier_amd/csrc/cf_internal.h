@@ -224,6 +224,10 @@ void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
 // `off` is the exclusive scan of the per-node counts with nn + 1 entries (off[nn] = total)
 void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                          size_t nout, int depth, Node *out);
+// one level down (children of nodes[0, nn) with exclusive child scan off) together with each child's
+// own child count (cnt, nout entries)
+void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, Node *out, u32 *cnt);
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                        u64 leaf_lo, size_t nleaf, int depth, u64 *keys, u64 *ids);
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
@@ -317,7 +321,8 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
                       u64 cap, u64 *od, u64 *oi, u64 *oinst);
-void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out);
+// per-instance leaf totals of a frontier added into out[0, ninst) (nodes of an instance are contiguous)
+void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
 
 // joins
 u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off);

@@ -127,6 +127,37 @@ __global__ void __launch_bounds__(BLK) k_expand_write(const SideDesc *__restrict
     }
 }
 
+// Owner of the first output of every BLK-output tile: parent p's outputs are [off[p], off[p+1]), so
+// it owns the tiles that start there (a parent has at most a pool's worth of children, so a few
+// tiles at most). One thread per parent, one coalesced pass over off.
+__global__ void __launch_bounds__(BLK) k_mark_owners(const u64 *__restrict__ off, size_t nn, u64 *__restrict__ tp) {
+    for (size_t p = (size_t)blockIdx.x * BLK + threadIdx.x; p < nn; p += (size_t)gridDim.x * BLK) {
+        const u64 a = off[p], b = off[p + 1];
+        if (b <= a) continue;
+        for (u64 t = (a + BLK - 1) / BLK; t <= (b - 1) / BLK; ++t) tp[t] = p;
+    }
+}
+
+// One level down with the next level's counts: child j of the level (parent found within its
+// tile's owner range) is written with the number of its own children, so the next level needs no
+// separate count pass.
+__global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__restrict__ descs,
+                                                           const Node *__restrict__ nodes, size_t nn,
+                                                           const u64 *__restrict__ off,
+                                                           const u64 *__restrict__ tp, size_t nout, int depth,
+                                                           Node *__restrict__ out, u32 *__restrict__ cnt) {
+    const u64 j = (u64)blockIdx.x * BLK + threadIdx.x;
+    if (j >= nout) return;
+    // the tile's parents lie between the owners of its first output and of the next tile's first
+    const size_t p1 = blockIdx.x + 1 < gridDim.x ? tp[blockIdx.x + 1] : nn - 1;
+    const size_t p = find_parent(off, tp[blockIdx.x], p1, j);
+    const Node nd = nodes[p];
+    const SideDesc &s = descs[nd.meta >> 8];
+    const Node c = make_child(s, nd, depth, (int)(j - off[p]));
+    out[j] = c;
+    cnt[j] = (u32)child_count(s, c, depth + 1);
+}
+
 // Leaves [leaf_lo, leaf_lo + nleaf) of the depth-(r-1) nodes; writes record j - leaf_lo.
 __global__ void __launch_bounds__(BLK) k_leaf_write(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
                                                     size_t nn, const u64 *__restrict__ off, u64 leaf_lo, size_t nleaf,
@@ -209,6 +240,16 @@ void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
     if (!nout) return;
     CF_LAUNCH(ctx, "k_expand_write", k_expand_write, dim3(grid_for(nout, BLK)), dim3(BLK), 0, d_desc, nodes, nn, off, nout,
                        depth, out);
+}
+
+void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, Node *out, u32 *cnt) {
+    if (!nout) return;
+    const u64 ntiles = (nout + BLK - 1) / BLK;
+    DBuf<u64> tp(ctx, ntiles);
+    CF_LAUNCH(ctx, "k_mark_owners", k_mark_owners, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, off, nn, tp.p);
+    CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc, nodes,
+              nn, off, tp.p, nout, depth, out, cnt);
 }
 
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off, u64 leaf_lo,

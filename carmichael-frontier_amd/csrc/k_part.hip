@@ -970,18 +970,24 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
     }
 }
 
-// Per-instance record totals from node boundaries: nodes of one instance are contiguous, so
-// instance j's leaves are off[last+1] - off[first].
-__global__ void __launch_bounds__(PB) k_inst_bounds(const Node *__restrict__ nodes, const u64 *__restrict__ off, size_t nn,
-                                                    u64 *__restrict__ out) {
-    for (size_t j = (size_t)blockIdx.x * PB + threadIdx.x; j < nn; j += (size_t)gridDim.x * PB) {
-        const u32 inst = nodes[j].meta >> 8;
-        // the first and the last node of an instance may be different lanes: add atomically
-        if (j == 0 || (nodes[j - 1].meta >> 8) != inst)
-            atomicAdd((unsigned long long *)&out[inst], (unsigned long long)(0 - off[j]));
-        if (j == nn - 1 || (nodes[j + 1].meta >> 8) != inst)
-            atomicAdd((unsigned long long *)&out[inst], (unsigned long long)off[j + 1]);
+// Per-instance record totals from node boundaries: nodes of one instance are contiguous and in
+// instance order, so instance i's nodes are [lo, hi) with lo and hi found by binary search on the
+// instance field, and its leaves are off[hi] - off[lo]. One thread per instance.
+__global__ void __launch_bounds__(PB) k_inst_bounds(const Node *__restrict__ nodes, const u64 *__restrict__ off,
+                                                    size_t nn, int ninst, u64 *__restrict__ out) {
+    const int i = blockIdx.x * PB + threadIdx.x;
+    if (i >= ninst) return;
+    size_t bounds[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {   // first node whose instance is >= i + e
+        size_t a = 0, b = nn;
+        while (a < b) {
+            const size_t mid = (a + b) >> 1;
+            if ((int)(nodes[mid].meta >> 8) < i + e) a = mid + 1; else b = mid;
+        }
+        bounds[e] = a;
     }
+    if (bounds[1] > bounds[0]) out[i] += off[bounds[1]] - off[bounds[0]];
 }
 
 __global__ void __launch_bounds__(PB) k_fill_u64(u64 *__restrict__ a, size_t n, u64 v) {
@@ -1185,7 +1191,8 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
 }
 
-void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *out) {
-    if (!nn) return;
-    CF_LAUNCH(ctx, "k_inst_bounds", k_inst_bounds, dim3(grid_for(nn, PB, 65536)), dim3(PB), 0, nodes, off, nn, out);
+void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {
+    if (!nn || ninst <= 0) return;
+    CF_LAUNCH(ctx, "k_inst_bounds", k_inst_bounds, dim3((unsigned)((ninst + PB - 1) / PB)), dim3(PB), 0, nodes, off, nn,
+              ninst, out);
 }

@@ -12,6 +12,7 @@ namespace {
 constexpr int SB = 256;         // scan block
 constexpr int SI = 8;           // items per thread
 constexpr int ST = SB * SI;     // 2048 per block
+static_assert(SI == 8, "the vector path moves 8 items");
 
 constexpr int RB = 256;         // radix block
 constexpr int RI = 8;           // items per thread
@@ -66,21 +67,49 @@ template <typename T>
 __global__ void __launch_bounds__(SB) k_scan_final(const T *__restrict__ in, size_t n, const u64 *__restrict__ boff,
                                                    u64 *__restrict__ out) {
     const size_t base = (size_t)blockIdx.x * ST + (size_t)threadIdx.x * SI;  // thread-major items
+    // a thread's SI items are one run: 16-byte loads and stores when the run is whole and aligned
+    const bool vec = base + SI <= n && ((((uintptr_t)in | (uintptr_t)out) & 15) == 0);
     u64 v[SI];
     u64 s = 0;
+    if (vec) {
+        if constexpr (sizeof(T) == 4) {
+            const uint4 a = ((const uint4 *)(in + base))[0], b = ((const uint4 *)(in + base))[1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        } else {
 #pragma unroll
-    for (int k = 0; k < SI; ++k) {
-        const size_t e = base + k;
-        v[k] = e < n ? (u64)in[e] : 0;
-        s += v[k];
+            for (int k = 0; k < SI / 2; ++k) {
+                const ulonglong2 a = ((const ulonglong2 *)(in + base))[k];
+                v[2 * k] = a.x;
+                v[2 * k + 1] = a.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SI; ++k) {
+            const size_t e = base + k;
+            v[k] = e < n ? (u64)in[e] : 0;
+        }
     }
+#pragma unroll
+    for (int k = 0; k < SI; ++k) s += v[k];
     u64 tot;
     u64 pre = block_excl_scan(s, &tot) + (boff ? boff[blockIdx.x] : 0);
+    if (vec) {
 #pragma unroll
-    for (int k = 0; k < SI; ++k) {
-        const size_t e = base + k;
-        if (e < n) out[e] = pre;
-        pre += v[k];
+        for (int k = 0; k < SI / 2; ++k) {
+            const u64 a = pre;
+            pre += v[2 * k];
+            ((ulonglong2 *)(out + base))[k] = make_ulonglong2(a, pre);
+            pre += v[2 * k + 1];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SI; ++k) {
+            const size_t e = base + k;
+            if (e < n) out[e] = pre;
+            pre += v[k];
+        }
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SB - 1) out[n] = pre;
 }

@@ -65,16 +65,19 @@ Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const Node *roots, 
     DBuf<Node> cur;
     const Node *curp = roots;
     size_t ncur = nroots;
+    // the roots' child counts; every level below gets its counts from the pass that writes it
+    DBuf<u32> c(ctx, ncur);
+    launch_expand_count(ctx, d_desc, curp, ncur, 0, c.p);
     for (int depth = 0; depth + 1 < r; ++depth) {
-        DBuf<u32> c(ctx, ncur);
-        launch_expand_count(ctx, d_desc, curp, ncur, depth, c.p);
         DBuf<u64> o(ctx, ncur + 1);
         scan_u32_to_u64_async(ctx, c.p, o.p, ncur);
         const u64 tot = host_value(ctx, tape, o.p + ncur);
         if (tot == 0) return f;
         DBuf<Node> nx(ctx, tot);
-        launch_expand_write(ctx, d_desc, curp, ncur, o.p, tot, depth, nx.p);
+        DBuf<u32> cn(ctx, tot);
+        launch_expand_write_count(ctx, d_desc, curp, ncur, o.p, tot, depth, nx.p, cn.p);
         cur = std::move(nx);
+        c = std::move(cn);
         curp = cur.p;
         ncur = tot;
     }
@@ -82,8 +85,7 @@ Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const Node *roots, 
         cur = DBuf<Node>(ctx, ncur);
         CF_HIP(hipMemcpyAsync(cur.p, roots, ncur * sizeof(Node), hipMemcpyDeviceToDevice, ctx->stream));
     }
-    f.cnt = DBuf<u32>(ctx, ncur);
-    launch_expand_count(ctx, d_desc, cur.p, ncur, r - 1, f.cnt.p);
+    f.cnt = std::move(c);
     f.off = DBuf<u64>(ctx, ncur + 1);
     scan_u32_to_u64_async(ctx, f.cnt.p, f.off.p, ncur);
     f.total = host_value(ctx, tape, f.off.p + ncur);
@@ -346,7 +348,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             if (!capturing) gc.mark();
             Frontier fd = expand_to_last(ctx, d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first, tape);
             const u64 nD = fd.total;
-            if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, dcount.p);
+            if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, n, dcount.p);
             const Partitioning pt = assign_buckets(nD, b->tune);
             const KeyHash &kh = pt.kh;
             DBuf<u64> bstart(ctx, pt.NB + 1);
@@ -401,7 +403,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
             // scatter), probe the table, check the candidates exactly
             Frontier f = expand_to_last(ctx, i_desc.p, b->iroots.at(g.first).p, b->ni_roots.at(g.first), g.first, tape);
-            if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, icount.p);
+            if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, n, icount.p);
             if (f.total && nD) {
                 DBuf<u64> rec;
                 DBuf<u64> cand;

@@ -104,6 +104,11 @@ struct Tile {
     bool fast;
 };
 
+// OWN_ZEROED: the caller keeps own[] all zero between tiles (it clears it once a tile's leaves are
+// hashed), so a parent marks its first leaf in the same pass that stages it. The instance's primes
+// and constants are staged only when a fast tile's instance differs from the last one staged
+// (S.inst, -1 before the first tile: the caller sets it).
+template <bool OWN_ZEROED = false>
 __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) {
     Tile T;
     T.j0 = L.lo + t * LT;
@@ -114,29 +119,42 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
     const u32 i0 = L.nodes[T.p0].meta >> 8, i1 = L.nodes[T.p0 + T.np - 1].meta >> 8;
     const SideDesc &sd = L.descs[i0];
     T.fast = i0 == i1 && T.np <= (u32)NCAP && sd.leaf32 && sd.n <= PCAP;
-    for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
+    const bool restage = T.fast && S.inst != (int)i0;   // read before any thread rewrites S.inst
+    if (!OWN_ZEROED)
+        for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
     if (T.fast) {
         for (u32 q = threadIdx.x; q < T.np; q += PB) {
             const Node &nd = L.nodes[T.p0 + q];
+            const u64 o = L.off[T.p0 + q];
             S.xm[q] = nd.xm;
             S.x2[q] = nd.x2;
             S.nx[q] = node_nxt(nd);
-            S.kb[q] = (int)(long long)(T.j0 - L.off[T.p0 + q]);
+            S.kb[q] = (int)(long long)(T.j0 - o);
+            if (OWN_ZEROED && q) S.own[o - T.j0] = (unsigned short)q;
         }
-        for (int x = threadIdx.x; x < sd.n; x += PB) S.prime[x] = (u32)sd.p[x];
-        if (threadIdx.x == 0) {
-            S.m = sd.mc.m;
-            S.mask2 = sd.mc.mask2;
-            S.koff = sd.koff;
-            S.minv = sd.mc.minv;
-            S.mneg32 = (u32)sd.mc.mneg;
-            S.sideI = sd.side == SIDE_I;
-            S.inst = (int)i0;
+        if (restage) {
+            for (int x = threadIdx.x; x < sd.n; x += PB) S.prime[x] = (u32)sd.p[x];
+            if (threadIdx.x == 0) {
+                S.m = sd.mc.m;
+                S.mask2 = sd.mc.mask2;
+                S.koff = sd.koff;
+                S.minv = sd.mc.minv;
+                S.mneg32 = (u32)sd.mc.mneg;
+                S.sideI = sd.side == SIDE_I;
+            }
         }
+    } else if (OWN_ZEROED) {
+        for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
     }
     __syncthreads();
-    for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
-    __syncthreads();
+    if (!OWN_ZEROED) {
+        if (T.fast)
+            for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[-S.kb[q]] = (unsigned short)q;
+        else
+            for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
+    }
+    if (restage && threadIdx.x == 0) S.inst = (int)i0;
+    if (!OWN_ZEROED) __syncthreads();
     // running maximum over own: LPT entries per thread, then a block scan of the thread maxima
     u32 run = 0;
     unsigned short v[LPT];
@@ -166,6 +184,13 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
     }
     __syncthreads();
     return T;
+}
+
+// Before a kernel's first tile_setup: no instance staged yet, owner map clear.
+__device__ __forceinline__ void tile_lds_init(TileLds &S) {
+    if (threadIdx.x == 0) S.inst = -1;
+    for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
+    __syncthreads();
 }
 
 // Modulus constants of a fast tile, read once from LDS.
@@ -273,6 +298,7 @@ __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off
 // ---- D side: counting sort by bucket --------------------------------------------------------
 __global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 *__restrict__ bcount) {
     __shared__ TileLds S;
+    tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile T = tile_setup(L, t, S);
@@ -291,6 +317,7 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
                                                        u32 *__restrict__ bcur, DRec *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
     __shared__ TileLds S;
+    tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tile_setup(L, t, S);
@@ -338,6 +365,7 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
     __shared__ u32 hist[256];
     const int P = 1 << lgP;
     hist[threadIdx.x] = 0;
+    tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
@@ -370,23 +398,28 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
     __shared__ u64 cur[256];
-    __shared__ u8_t s_p[LT];
     __shared__ u32 s_ovf;
     __shared__ u64 s_k[KEYS ? LT : 1];
-    uint2 *s_r = (uint2 *)S.xm;        // LT records staged over the parents' residue state (16 KB)
+    // once a tile's leaves are hashed its parents' staged state is dead: the records are staged over
+    // xm and x2 (16 KB) and each record's destination over kb and nx (8 KB)
+    uint2 *s_r = (uint2 *)S.xm;
+    u32 *s_dst = (u32 *)S.kb;
     static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
+    static_assert(sizeof(u32) * LT <= sizeof(int) * 2 * NCAP && offsetof(TileLds, nx) == offsetof(TileLds, kb) +
+                  sizeof(int) * NCAP, "destinations fit over kb, nx");
     const int P = 1 << kh.lgP;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const bool exact = O != nullptr;
     const u64 cap = (u64)1 << capl;
     if (threadIdx.x < (unsigned)P) cur[threadIdx.x] = exact ? O[(u64)threadIdx.x * nblk + blockIdx.x] : 0;
     if (threadIdx.x == 0) s_ovf = 0;
+    tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
     for (u64 t = t0; t < t1; ++t) {
         hist[threadIdx.x] = 0;
-        const Tile T = tile_setup(L, t, S);
+        const Tile T = tile_setup<true>(L, t, S);
         u64 h[LPT];
         u64 kk[KEYS ? LPT : 1];
         if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
@@ -395,7 +428,9 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
             if (threadIdx.x + PB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
-        __syncthreads();   // also: every lane is done with the staged parents
+        __syncthreads();   // also: every lane is done with the staged parents and the owner map
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
         // tile-local exclusive offsets of the partitions (block scan of hist)
         const u32 v = hist[threadIdx.x];
         {
@@ -413,31 +448,35 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
             loc[threadIdx.x] = pre + inc - v;
         }
         __syncthreads();
+        // staging: each record at its partition's run in the tile, with its final slot beside it
+        // (capacity layout: partition << 16 | slot in the segment, or ~0 past the segment's end;
+        // exact layout: the record's index, below 2^32 for a chunk of at most 2^32 leaves)
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + PB * k;
             if (i < T.cnt) {
                 const u32 part = hpart(h[k], kh.lgP);
                 const u32 at = loc[part] + rank[k];
+                const u64 o = cur[part] + rank[k];
+                u32 d;
+                if (exact) {
+                    d = (u32)o;
+                } else if (o < cap) {
+                    d = (part << 16) | (u32)o;
+                } else {
+                    d = ~0u;
+                    s_ovf = 1;
+                }
                 s_r[at] = make_uint2((hoff(h[k], kh) << kh.fpb) | (hfp(h[k]) & fmask), (u32)(T.j0 + i - L.lo));
-                s_p[at] = (u8_t)part;
+                s_dst[at] = d;
                 if constexpr (KEYS) s_k[at] = kk[k];
             }
         }
         __syncthreads();
         for (u32 q = threadIdx.x; q < T.cnt; q += PB) {
-            const u32 pi = s_p[q];
-            const u64 o = cur[pi] + (u64)(q - loc[pi]);
-            u64 g;
-            if (exact) {
-                g = o;
-            } else {
-                if (o >= cap) {
-                    s_ovf = 1;
-                    continue;
-                }
-                g = (((u64)pi * nblk + blockIdx.x) << capl) + o;
-            }
+            const u32 d = s_dst[q];
+            if (d == ~0u) continue;
+            const u64 g = exact ? (u64)d : ((((u64)(d >> 16) * nblk + blockIdx.x) << capl) + (d & 0xFFFFu));
             rec[g] = s_r[q];
             if constexpr (KEYS) keys[g] = s_k[q];
         }

@@ -239,7 +239,8 @@ void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount);
-// DR: n records of 16 bytes (exact key, id) in bucket order; T: 16 u16 slots per bucket, zeroed
+// DR: n records of 16 bytes (exact key, D leaf index) in bucket order; T: CF_TABLE_SLOTS u16 slots per
+// bucket, zeroed
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
 // validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
@@ -288,13 +289,14 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
 // part_d_windows moves the records into fixed-capacity window regions (wreg: P * R * dwin_capacity()
 // records of 16 bytes; *ovf set if a window overflowed); after the caller's exclusive scan of wcount
 // into wbase (P * R + 1 entries), part_d_sort writes bstart (NB + 1), DR in bucket order
-// ({key, D leaf}), the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket).
+// ({key, D leaf}), the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket);
+// kerr is set if a window holds a record outside its buckets (inconsistent input)
 bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R);
 u64 dwin_capacity();
 void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
                     const u64 *keys, u32 *wcount, void *wreg, u32 *ovf);
 void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
-                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB);
+                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB, u32 *kerr);
 // the D leaves of a radius group, for recomputing a pair's D id in pass 2
 struct DLeaves {
     const SideDesc *descs;

@@ -390,10 +390,10 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
 // (hashing spreads records evenly) with a margin of about 8 standard deviations.
 // With KEYS (the D side) every record's exact key goes to keys[] at the record's slot.
 template <bool KEYS>
-__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
-                                                     const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
-                                                     u32 *__restrict__ ovf, uint2 *__restrict__ rec,
-                                                     u64 *__restrict__ keys) {
+__device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
+                                                  const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
+                                                  u32 *__restrict__ ovf, uint2 *__restrict__ rec,
+                                                  u64 *__restrict__ keys) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
@@ -491,6 +491,20 @@ __global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 
         }
         if (threadIdx.x == 0 && s_ovf) atomicOr(ovf, 1u);
     }
+}
+
+// I side (records only) and D side (records and exact keys): separate kernels so that profiles tell
+// them apart.
+__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec) {
+    part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
+}
+__global__ void __launch_bounds__(PB) k_part_scatter_keys(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
+                                                          const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
+                                                          u32 *__restrict__ ovf, uint2 *__restrict__ rec,
+                                                          u64 *__restrict__ keys) {
+    part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, keys);
 }
 
 // pbase[p] = O[p * nblk] for p <= P (O[P * nblk] is the total): where each partition starts
@@ -851,7 +865,8 @@ constexpr int DSB = 512;               // threads of a window-sort block
 __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u32 *__restrict__ wcount,
                                                    const u64 *__restrict__ wbase, const uint4 *__restrict__ wreg,
                                                    u64 *__restrict__ bstart, DRec *__restrict__ DR,
-                                                   unsigned short *__restrict__ T, u32 *__restrict__ F, int FB) {
+                                                   unsigned short *__restrict__ T, u32 *__restrict__ F, int FB,
+                                                   u32 *__restrict__ kerr) {
     constexpr int WMAXB = 4096;        // buckets a window may have
     __shared__ uint4 s_r[CAPW];
     __shared__ unsigned short s_idx[CAPW];
@@ -873,10 +888,21 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     __syncthreads();
     constexpr int RPT = (CAPW + DSB - 1) / DSB;
     u32 rk[RPT];
+    bool bad = false;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const u32 i = threadIdx.x + (u32)k * DSB;
-        if (i < n) rk[k] = atomicAdd(&cnt[(s_r[i].x >> kh.fpb) - (u32)b_lo], 1u);
+        if (i < n) {
+            // a record's window is its bucket offset >> wb, so its bucket is in the window; a record
+            // that is not (inconsistent input) is reported through kerr and the window left unsorted
+            const u32 bi = (s_r[i].x >> kh.fpb) - (u32)b_lo;
+            if (bi < nb) rk[k] = atomicAdd(&cnt[bi], 1u);
+            else bad = true;
+        }
+    }
+    if (__syncthreads_or(bad)) {
+        if (threadIdx.x == 0) atomicOr(kerr, 1u);
+        return;
     }
     __syncthreads();
     // exclusive scan of cnt[0, nb) into pos (DSB threads, consecutive runs of PER entries)
@@ -1107,10 +1133,10 @@ void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayo
 }
 
 void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
-                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB) {
+                 const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB, u32 *kerr) {
     const int P = 1 << kh.lgP;
     CF_LAUNCH(ctx, "k_dwin_sort", k_dwin_sort, dim3((unsigned)((u64)P * R)), dim3(DSB), 0, kh, DWin{wb, R}, wcount,
-              wbase, (const uint4 *)wreg, bstart, (DRec *)DR, T, F, FB);
+              wbase, (const uint4 *)wreg, bstart, (DRec *)DR, T, F, FB, kerr);
 }
 
 u64 dwin_capacity() { return CAPW; }
@@ -1147,11 +1173,11 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     fill_u32(ctx, ovf, 1, 0);
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter<true>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, keys);
     else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter<false>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
-                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, (u64 *)nullptr);
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
 }
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
@@ -1171,11 +1197,11 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
     if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter<true>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, keys);
     else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter<false>, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
-                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, (u64 *)nullptr);
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
 }
 
 u64 cand_margin(const Tuning &tn) {

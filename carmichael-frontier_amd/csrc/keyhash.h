@@ -20,8 +20,8 @@ struct KeyHash {
     u64 KR;     // buckets per range
 };
 
-constexpr u64 CF_RANGE_BUCKETS_LDS = 8192;   // buckets per range the LDS counting sort takes
-constexpr int CF_RANGE_BITS_LDS = 12;         // ranges the LDS partition pass takes (2^12)
+constexpr u64 CF_RANGE_BUCKETS_LDS = 8192;   // most buckets per hash range
+constexpr int CF_RANGE_BITS_LDS = 12;         // most hash-range bits (2^12 ranges)
 constexpr int CF_TABLE_SLOTS = 8;             // 16-bit fingerprint slots per bucket (16 bytes)
 
 CF_HD u64 kh_nb(const KeyHash &kh) { return kh.KR << kh.lgR; }
@@ -39,10 +39,10 @@ CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
 // Geometry for a radius group with nD D records: about 2 D records per bucket by default, so an
 // 8-slot bucket rarely overflows (Poisson tail P(X > 8) ~ 2.4e-4); P = 2^lgP probe partitions, as few
 // as keep one partition's slice of the 16-byte-per-bucket table within about 2 MB by default (an
-// XCD's L2 holds it while the partition is probed), 16 <= P <= 256; as few hash ranges as keep a range within the buckets
-// the LDS counting sort takes (at most 2^12 ranges; beyond that the D side is bucketed by global
-// atomics). An I record keeps the bucket offset within its partition and 32 - that many bits of
-// fingerprint, at most 14. Returns false if a partition would need more than 2^24 buckets.
+// XCD's L2 holds it while the partition is probed), 16 <= P <= 256; as few hash ranges as keep a
+// range within 8192 buckets (at most 2^12 ranges). An I record keeps the bucket offset within its
+// partition and 32 - that many bits of fingerprint, at most 14. Returns false if a partition would
+// need more than 2^24 buckets.
 inline int kh_bitlen(u64 v) {
     int b = 0;
     while (v) {

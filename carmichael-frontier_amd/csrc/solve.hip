@@ -380,7 +380,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, dkeys.p, wcount.p, wreg.p, ovf.p);
                     if (!host_value_u32(ctx, tape, ovf.p)) {
                         scan_u32_to_u64_async(ctx, wcount.p, wbase.p, nw);
-                        part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, bstart.p, DR.p, T.p, F.p, FB);
+                        part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, bstart.p, DR.p, T.p, F.p, FB,
+                                    kerr.p);
                         done = true;
                     }
                 }

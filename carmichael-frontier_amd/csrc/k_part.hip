@@ -32,9 +32,10 @@
 
 namespace {
 
-constexpr int PB = 256;            // threads per block
-constexpr int LPT = 8;             // consecutive leaves per thread in the leaf kernels
-constexpr int LT = PB * LPT;       // 2048 leaves per tile
+constexpr int PB = 256;            // threads per block (other kernels)
+constexpr int LPB = 512;           // threads per block of the leaf kernels
+constexpr int LPT = 8;             // leaves per thread in the leaf kernels
+constexpr int LT = LPB * LPT;      // 4096 leaves per tile
 constexpr int PI = 16;             // records per thread in the probe
 constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
@@ -71,7 +72,7 @@ __device__ __forceinline__ u64 leaf_hash(u64 xm, u64 x2, u64 mask2, u64 koff) {
 }
 
 // ---- leaf tiles ------------------------------------------------------------------------------
-// A tile is LT consecutive leaves [j0, j0 + cnt). Lane i of the block takes leaves j0 + i + PB * k,
+// A tile is LT consecutive leaves [j0, j0 + cnt). Lane i of the block takes leaves j0 + i + LPB * k,
 // so the 64 lanes of a wave work on 64 neighbouring leaves. The tile's parent nodes are
 // p0 .. p0 + np - 1 (every last-level node owns at least one leaf, so np <= LT). Set-up builds the
 // owner map own[i] = parent of leaf j0 + i (each node's first leaf is marked with its number, a
@@ -80,17 +81,17 @@ __device__ __forceinline__ u64 leaf_hash(u64 xm, u64 x2, u64 mask2, u64 koff) {
 // the parents' residue state, next index and first child number, the instance's primes and its
 // modulus constants. A leaf is then a few LDS reads and one 32-bit Montgomery step; other tiles
 // read nodes and descriptors from global memory.
-constexpr int NCAP = 1024;         // parents staged per tile
+constexpr int NCAP = 2048;         // parents staged per tile
 constexpr int PCAP = 1024;         // primes staged (pools hold at most 1024)
 
 struct TileLds {
-    u64 xm[NCAP];                  // xm and x2 (16 KB) double as the scatter's staging area
+    u64 xm[NCAP];                  // xm and x2 (32 KB) double as the scatter's staging area
     u64 x2[NCAP];
     int kb[NCAP];                  // child number of leaf j0 under parent q, i.e. j0 - off[p0 + q]
     int nx[NCAP];                  // parent's next index
     u32 prime[PCAP];
     unsigned short own[LT];
-    u32 w[PB / 64];
+    u32 w[LPB / 64];
     u64 m, mask2, koff, minv;
     u32 mneg32;
     int sideI, inst;
@@ -121,9 +122,9 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
     T.fast = i0 == i1 && T.np <= (u32)NCAP && sd.leaf32 && sd.n <= PCAP;
     const bool restage = T.fast && S.inst != (int)i0;   // read before any thread rewrites S.inst
     if (!OWN_ZEROED)
-        for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
+        for (u32 i = threadIdx.x; i < (u32)LT; i += LPB) S.own[i] = 0;
     if (T.fast) {
-        for (u32 q = threadIdx.x; q < T.np; q += PB) {
+        for (u32 q = threadIdx.x; q < T.np; q += LPB) {
             const Node &nd = L.nodes[T.p0 + q];
             const u64 o = L.off[T.p0 + q];
             S.xm[q] = nd.xm;
@@ -133,7 +134,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
             if (OWN_ZEROED && q) S.own[o - T.j0] = (unsigned short)q;
         }
         if (restage) {
-            for (int x = threadIdx.x; x < sd.n; x += PB) S.prime[x] = (u32)sd.p[x];
+            for (int x = threadIdx.x; x < sd.n; x += LPB) S.prime[x] = (u32)sd.p[x];
             if (threadIdx.x == 0) {
                 S.m = sd.mc.m;
                 S.mask2 = sd.mc.mask2;
@@ -144,14 +145,14 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
             }
         }
     } else if (OWN_ZEROED) {
-        for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
+        for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
     }
     __syncthreads();
     if (!OWN_ZEROED) {
         if (T.fast)
-            for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[-S.kb[q]] = (unsigned short)q;
+            for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[-S.kb[q]] = (unsigned short)q;
         else
-            for (u32 q = 1 + threadIdx.x; q < T.np; q += PB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
+            for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
     }
     if (restage && threadIdx.x == 0) S.inst = (int)i0;
     if (!OWN_ZEROED) __syncthreads();
@@ -189,7 +190,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
 // Before a kernel's first tile_setup: no instance staged yet, owner map clear.
 __device__ __forceinline__ void tile_lds_init(TileLds &S) {
     if (threadIdx.x == 0) S.inst = -1;
-    for (u32 i = threadIdx.x; i < (u32)LT; i += PB) S.own[i] = 0;
+    for (u32 i = threadIdx.x; i < (u32)LT; i += LPB) S.own[i] = 0;
     __syncthreads();
 }
 
@@ -241,7 +242,7 @@ __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, con
         const FastCtx c(S);
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             u64 xm, x2;
             int x;
             u32 q;
@@ -249,7 +250,7 @@ __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, con
         }
     } else {
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             u64 xm, x2;
             LeafRef r;
             h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
@@ -264,7 +265,7 @@ __device__ __forceinline__ void tile_hash_keys(const LeafSrc &L, const Tile &T, 
         const FastCtx c(S);
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             u64 xm, x2;
             int x;
             u32 q;
@@ -273,7 +274,7 @@ __device__ __forceinline__ void tile_hash_keys(const LeafSrc &L, const Tile &T, 
         }
     } else {
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             u64 xm, x2;
             LeafRef r;
             h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off
 }
 
 // ---- D side: counting sort by bucket --------------------------------------------------------
-__global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 *__restrict__ bcount) {
+__global__ void __launch_bounds__(LPB) k_bucket_count(LeafSrc L, KeyHash kh, u32 *__restrict__ bcount) {
     __shared__ TileLds S;
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
@@ -306,14 +307,14 @@ __global__ void __launch_bounds__(PB) k_bucket_count(LeafSrc L, KeyHash kh, u32 
         tile_hashes(L, T, S, h);
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&bcount[hbucket(h[k], kh)], 1u);
+            if (threadIdx.x + LPB * k < T.cnt) atomicAdd(&bcount[hbucket(h[k], kh)], 1u);
         __syncthreads();
     }
 }
 
 // Scatter into bucket order; the rank a record takes in its bucket is also its table slot.
 __device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key);
-__global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
+__global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
                                                        u32 *__restrict__ bcur, DRec *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
     __shared__ TileLds S;
@@ -322,7 +323,7 @@ __global__ void __launch_bounds__(PB) k_bucket_scatter(LeafSrc L, KeyHash kh, co
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tl = tile_setup(L, t, S);
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             if (i >= tl.cnt) continue;
             u64 h, key;
             d_leaf(L, tl, S, i, h, key);
@@ -360,11 +361,11 @@ __device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const Ti
 // and first counts its leaves per partition into H[p * nblk + k]; an exclusive scan of H in that
 // order gives O[p * nblk + k], where block k's run of partition p starts, so the second pass writes
 // every record to its final slot, one contiguous run per partition and block.
-__global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u32 nblk, u32 *__restrict__ H) {
+__global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u32 nblk, u32 *__restrict__ H) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     const int P = 1 << lgP;
-    hist[threadIdx.x] = 0;
+    if (threadIdx.x < 256) hist[threadIdx.x] = 0;
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(PB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u
         tile_hashes(L, T, S, h);
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + PB * k < T.cnt) atomicAdd(&hist[hpart(h[k], lgP)], 1u);
+            if (threadIdx.x + LPB * k < T.cnt) atomicAdd(&hist[hpart(h[k], lgP)], 1u);
         __syncthreads();
     }
     __syncthreads();
@@ -418,7 +419,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
     for (u64 t = t0; t < t1; ++t) {
-        hist[threadIdx.x] = 0;
+        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         const Tile T = tile_setup<true>(L, t, S);
         u64 h[LPT];
         u64 kk[KEYS ? LPT : 1];
@@ -427,12 +428,12 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         u32 rank[LPT];
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + PB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
+            if (threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
         __syncthreads();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
         for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
         // tile-local exclusive offsets of the partitions (block scan of hist)
-        const u32 v = hist[threadIdx.x];
+        const u32 v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
         {
             const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
             u32 inc = v;
@@ -445,7 +446,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             __syncthreads();
             u32 pre = 0;
             for (int q = 0; q < wv; ++q) pre += S.w[q];
-            loc[threadIdx.x] = pre + inc - v;
+            if (threadIdx.x < 256) loc[threadIdx.x] = pre + inc - v;
         }
         __syncthreads();
         // staging: each record at its partition's run in the tile, with its final slot beside it
@@ -453,7 +454,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         // exact layout: the record's index, below 2^32 for a chunk of at most 2^32 leaves)
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + PB * k;
+            const u32 i = threadIdx.x + LPB * k;
             if (i < T.cnt) {
                 const u32 part = hpart(h[k], kh.lgP);
                 const u32 at = loc[part] + rank[k];
@@ -473,7 +474,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             }
         }
         __syncthreads();
-        for (u32 q = threadIdx.x; q < T.cnt; q += PB) {
+        for (u32 q = threadIdx.x; q < T.cnt; q += LPB) {
             const u32 d = s_dst[q];
             if (d == ~0u) continue;
             const u64 g = exact ? (u64)d : ((((u64)(d >> 16) * nblk + blockIdx.x) << capl) + (d & 0xFFFFu));
@@ -495,12 +496,12 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
 
 // I side (records only) and D side (records and exact keys): separate kernels so that profiles tell
 // them apart.
-__global__ void __launch_bounds__(PB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+__global__ void __launch_bounds__(LPB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec) {
     part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
 }
-__global__ void __launch_bounds__(PB) k_part_scatter_keys(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
+__global__ void __launch_bounds__(LPB) k_part_scatter_keys(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                           const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
                                                           u32 *__restrict__ ovf, uint2 *__restrict__ rec,
                                                           u64 *__restrict__ keys) {
@@ -597,8 +598,9 @@ __global__ void __launch_bounds__(PB) k_bucket_filter(const unsigned short *__re
 constexpr int PBP = 1024;                       // threads per probe block (16 waves)
 constexpr u64 CAND_NONE = ~0ull;                // an unused candidate slot
 constexpr u32 CAND_CHUNK = 256;                 // candidate slots a wave reserves at a time
-constexpr int SPW = 2;                          // segments per wave per tile
-constexpr int SPT = SPW * (PBP / 64);           // segments per tile
+constexpr int SPW = 2;                          // segments a wave probes together (one round)
+constexpr int GPW = 2;                          // such groups per wave per tile
+constexpr int SPT = GPW * SPW * (PBP / 64);     // segments per tile
 constexpr int FWMAX = 32768;                    // filter words a block holds (128 KB of LDS)
 
 __device__ __forceinline__ u64 seg_start(const SegLayout &S, u64 g) { return S.O ? S.O[g] : (g << S.capl); }
@@ -621,7 +623,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                                                     unsigned long long *__restrict__ ncand, u64 ccap,
                                                     u64 *__restrict__ cand) {
     __shared__ u32 sF[FWMAX];
-    __shared__ u64 s_t;
+    __shared__ u64 s_nx[2];
     const int lane = threadIdx.x & 63;
     const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int P = 1 << kh.lgP;
@@ -639,12 +641,14 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     // candidates instead of one per round that has any
     u64 cbase = 0;
     u32 cleft = 0;
-    for (;;) {
-        // the barrier at the end of the loop orders this write after every wave's read of the last tile
-        if (threadIdx.x == 0) s_t = t_lo + atomicAdd(&xq[xcd], 1u);
-        __syncthreads();
-        const u64 t = uniform_u64(s_t);
+    if (threadIdx.x == 0) s_nx[0] = t_lo + atomicAdd(&xq[xcd], 1u);
+    __syncthreads();
+    u64 t = uniform_u64(s_nx[0]);
+    for (u32 it = 1;; ++it) {
         if (t >= t_hi) break;
+        // the next tile's index is taken while this one is probed and read after the barrier that
+        // ends it; of two slots, the one written here was last read before the previous barrier
+        if (threadIdx.x == 0) s_nx[it & 1] = t_lo + atomicAdd(&xq[xcd], 1u);
         const u64 part = t / tpp;
         if (FB > 0 && part != cur_part) {
             // this block's first tile of a partition: its filter slice into LDS (the barrier above
@@ -661,93 +665,96 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
         const u64 gend = gp + S.nblk;
         // the partition's table slice: byte offset hoff * 16 of bucket hoff
         const __amdgpu_buffer_rsrc_t tr = buf_rsrc(T + part * K * TSLOTS, K * TSLOTS * 2);
-        // the wave's segments: fills and starts (wave-uniform) read before the first is probed;
-        // a segment's records through a descriptor that ends at its fill
-        u32 fills[SPW];
-        u64 starts[SPW];
-        __amdgpu_buffer_rsrc_t rr[SPW];
-        u32 rounds_end = 0;
+        for (int grp = 0; grp < GPW; ++grp) {
+            // the wave's segments: fills and starts (wave-uniform) read before the first is probed;
+            // a segment's records through a descriptor that ends at its fill
+            u32 fills[SPW];
+            u64 starts[SPW];
+            __amdgpu_buffer_rsrc_t rr[SPW];
+            u32 rounds_end = 0;
 #pragma unroll
-        for (int sj = 0; sj < SPW; ++sj) {
-            const u64 g = gp + (t - part * tpp) * SPT + (u64)wv * SPW + sj;
-            fills[sj] = g < gend ? S.fill[g] : 0u;
-            starts[sj] = g < gend ? seg_start(S, g) : 0;
-            rr[sj] = buf_rsrc(recs + starts[sj], (u64)fills[sj] * sizeof(uint2));
-            rounds_end = fills[sj] > rounds_end ? fills[sj] : rounds_end;
-        }
-        // the wave's SPW segments are probed together, 64 * PROBE_BATCH slots of each per round
-        // (one round for a capacity segment), so one round trip serves all of them
-        for (u32 v0 = 0; v0 < rounds_end; v0 += 64 * PROBE_BATCH) {
-            constexpr int NB2 = SPW * PROBE_BATCH;
-            static_assert(NB2 <= 32, "one mask bit per record of a round");
-            u32 lo[NB2];
-            uint4 ta[NB2];
-            u32 valid = 0;
-#pragma unroll
-            for (int e = 0; e < NB2; ++e) {
-                const int sj = e / PROBE_BATCH, q = e % PROBE_BATCH;
-                const u32 v = v0 + (u32)q * 64 + lane;
-                // past the fill the descriptor returns 0: bucket 0 of the partition, masked below
-                lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 0);
-                valid |= (u32)(v < fills[sj]) << e;
+            for (int sj = 0; sj < SPW; ++sj) {
+                const u64 g = gp + (t - part * tpp) * SPT + ((u64)grp * (PBP / 64) + wv) * SPW + sj;
+                fills[sj] = g < gend ? S.fill[g] : 0u;
+                starts[sj] = g < gend ? seg_start(S, g) : 0;
+                rr[sj] = buf_rsrc(recs + starts[sj], (u64)fills[sj] * sizeof(uint2));
+                rounds_end = fills[sj] > rounds_end ? fills[sj] : rounds_end;
             }
-            u32 look = valid;
-            if (FB > 0) {
-                u32 pass = 0;
+            // the wave's SPW segments are probed together, 64 * PROBE_BATCH slots of each per round
+            // (one round for a capacity segment), so one round trip serves all of them
+            for (u32 v0 = 0; v0 < rounds_end; v0 += 64 * PROBE_BATCH) {
+                constexpr int NB2 = SPW * PROBE_BATCH;
+                static_assert(NB2 <= 32, "one mask bit per record of a round");
+                u32 lo[NB2];
+                uint4 ta[NB2];
+                u32 valid = 0;
 #pragma unroll
                 for (int e = 0; e < NB2; ++e) {
-                    const u32 bit = fbit0 + (lo[e] >> kh.fpb) * (u32)FB + (lo[e] & fbm);
-                    pass |= ((sF[bit >> 5] >> (bit & 31)) & 1u) << e;
+                    const int sj = e / PROBE_BATCH, q = e % PROBE_BATCH;
+                    const u32 v = v0 + (u32)q * 64 + lane;
+                    // past the fill the descriptor returns 0: bucket 0 of the partition, masked below
+                    lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 0);
+                    valid |= (u32)(v < fills[sj]) << e;
                 }
-                look &= pass;
-            }
-            // first halves; a record not looked up takes an offset past the slice (the load returns
-            // zeros without touching memory and cannot match)
+                u32 look = valid;
+                if (FB > 0) {
+                    u32 pass = 0;
 #pragma unroll
-            for (int e = 0; e < NB2; ++e) {
-                const u32 off = (look >> e) & 1u ? (lo[e] >> kh.fpb) * (2u * TSLOTS) : 0x7FFFFFF0u;
-                const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(tr, (int)off, 0, 0);
-                ta[e] = make_uint4(x[0], x[1], x[2], x[3]);
-            }
-            // a fingerprint among the bucket's slots, or the overflow flag of slot 7, is a candidate
-            u32 hmask = 0;
-#pragma unroll
-            for (int e = 0; e < NB2; ++e) {
-                const u32 fs2 = (0x8000u | (lo[e] & fmask)) * 0x10001u;
-                hmask |= (u32)(half_hit(ta[e], fs2, mm) | ((ta[e].w & 0x40000000u) != 0)) << e;
-            }
-            hmask &= look;
-            if (__any(hmask != 0)) {
-                const u32 c = (u32)__popc(hmask);
-                u32 inc = c;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const u32 o = __shfl_up(inc, d, 64);
-                    if (lane >= d) inc += o;
+                    for (int e = 0; e < NB2; ++e) {
+                        const u32 bit = fbit0 + (lo[e] >> kh.fpb) * (u32)FB + (lo[e] & fbm);
+                        pass |= ((sF[bit >> 5] >> (bit & 31)) & 1u) << e;
+                    }
+                    look &= pass;
                 }
-                const u32 total = (u32)__builtin_amdgcn_readlane((int)inc, 63);
-                if (total > cleft) {
-                    // the rest of the current reservation is marked unused; a new one is taken
-                    for (u32 i = lane; i < cleft; i += 64)
-                        if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
-                    const u32 want = total > CAND_CHUNK ? total : CAND_CHUNK;
-                    unsigned long long nb = 0;
-                    if (lane == 0) nb = atomicAdd(ncand, (unsigned long long)want);
-                    cbase = uniform_u64(__shfl(nb, 0, 64));
-                    cleft = want;
+                // first halves; a record not looked up takes an offset past the slice (the load returns
+                // zeros without touching memory and cannot match)
+#pragma unroll
+                for (int e = 0; e < NB2; ++e) {
+                    const u32 off = (look >> e) & 1u ? (lo[e] >> kh.fpb) * (2u * TSLOTS) : 0x7FFFFFF0u;
+                    const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(tr, (int)off, 0, 0);
+                    ta[e] = make_uint4(x[0], x[1], x[2], x[3]);
                 }
-                u64 at = cbase + inc - c;
-                cbase += total;
-                cleft -= total;
-                while (hmask) {
-                    const int e = __ffs(hmask) - 1;
-                    hmask &= hmask - 1;
-                    if (at < ccap) cand[at] = starts[e / PROBE_BATCH] + v0 + (u64)(e % PROBE_BATCH) * 64 + lane;
-                    ++at;
+                // a fingerprint among the bucket's slots, or the overflow flag of slot 7, is a candidate
+                u32 hmask = 0;
+#pragma unroll
+                for (int e = 0; e < NB2; ++e) {
+                    const u32 fs2 = (0x8000u | (lo[e] & fmask)) * 0x10001u;
+                    hmask |= (u32)(half_hit(ta[e], fs2, mm) | ((ta[e].w & 0x40000000u) != 0)) << e;
+                }
+                hmask &= look;
+                if (__any(hmask != 0)) {
+                    const u32 c = (u32)__popc(hmask);
+                    u32 inc = c;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const u32 o = __shfl_up(inc, d, 64);
+                        if (lane >= d) inc += o;
+                    }
+                    const u32 total = (u32)__builtin_amdgcn_readlane((int)inc, 63);
+                    if (total > cleft) {
+                        // the rest of the current reservation is marked unused; a new one is taken
+                        for (u32 i = lane; i < cleft; i += 64)
+                            if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
+                        const u32 want = total > CAND_CHUNK ? total : CAND_CHUNK;
+                        unsigned long long nb = 0;
+                        if (lane == 0) nb = atomicAdd(ncand, (unsigned long long)want);
+                        cbase = uniform_u64(__shfl(nb, 0, 64));
+                        cleft = want;
+                    }
+                    u64 at = cbase + inc - c;
+                    cbase += total;
+                    cleft -= total;
+                    while (hmask) {
+                        const int e = __ffs(hmask) - 1;
+                        hmask &= hmask - 1;
+                        if (at < ccap) cand[at] = starts[e / PROBE_BATCH] + v0 + (u64)(e % PROBE_BATCH) * 64 + lane;
+                        ++at;
+                    }
                 }
             }
         }
         __syncthreads();
+        t = uniform_u64(s_nx[it & 1]);
     }
     for (u32 i = lane; i < cleft; i += 64)
         if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
@@ -1095,7 +1102,7 @@ void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, si
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(PB), 0, L, kh, bcount);
+    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bcount);
 }
 
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
@@ -1103,7 +1110,7 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
     if (!n) return;
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
-    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(PB), 0, L, kh, bstart, bcur,
+    CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bstart, bcur,
               (DRec *)DR, T);
 }
 
@@ -1173,10 +1180,10 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     fill_u32(ctx, ovf, 1, 0);
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, keys);
     else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
 }
 
@@ -1194,13 +1201,13 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     lay.capl = 0;
     lay.fill = DBuf<u32>(ctx, lay.nseg);
     lay.O = DBuf<u64>(ctx, lay.nseg + 1);
-    CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(PB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
+    CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(LPB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
     if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, keys);
     else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(PB), 0, L, kh, tpb, nblk,
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
 }
 

@@ -41,6 +41,16 @@ constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
 
+// Block barrier for LDS traffic only: orders the block's LDS accesses without waiting for its
+// outstanding global loads and stores (__syncthreads is also a workgroup memory fence, which on this
+// target waits for every vector memory operation in flight, prefetches included). For kernels whose
+// threads never read global data another thread of the block wrote.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct LeafSrc {
     const SideDesc *descs;
     const Node *nodes;
@@ -95,6 +105,8 @@ struct TileLds {
     u64 m, mask2, koff, minv;
     u32 mneg32;
     int sideI, inst;
+    int fastok;                    // the staged instance takes the fast path (leaf32, n <= PCAP)
+    u32 meta_first, meta_last;     // instance fields of the tile's first and last parent
 };
 
 struct Tile {
@@ -187,9 +199,136 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
     return T;
 }
 
+// Parents of a tile fetched ahead into registers (the scatter's pipeline): the tile's first and
+// last parent, and for q = threadIdx.x < np, parent p0 + q's residue state, next index, instance
+// field and first leaf.
+struct TilePF {
+    u64 p0, p1;
+    u64 xm, x2, off;
+    u32 pos, meta;
+};
+__device__ __forceinline__ void tile_fetch(const LeafSrc &L, u64 p0, u64 p1, TilePF &f) {
+    f.p0 = p0;
+    f.p1 = p1;
+    const u64 q = p0 + threadIdx.x;
+    if (q <= p1) {
+        const Node *nd = L.nodes + q;
+        f.xm = nd->xm;
+        f.x2 = nd->x2;
+        f.pos = nd->pos;
+        f.meta = nd->meta;
+        f.off = L.off[q];
+    }
+}
+// A tile's owner pair as a vector load (lane k holds tp[2t + (k & 1)]), so that waiting for it does
+// not wait for scalar loads; lanes 0 and 1 are read out when it is used.
+__device__ __forceinline__ u64 tile_owner_load(const LeafSrc &L, u64 t) { return L.tp[2 * t + (threadIdx.x & 1)]; }
+__device__ __forceinline__ u64 lane_u64(u64 v, int lane) {
+    return ((u64)(u32)__builtin_amdgcn_readlane((int)(v >> 32), lane) << 32) |
+           (u64)(u32)__builtin_amdgcn_readlane((int)(u32)v, lane);
+}
+
+// Tile set-up from fetched parents (owner map kept zero between tiles by the caller). Whether the
+// tile takes the fast path is known from the staged instance unless the tile's instance differs;
+// only then is the descriptor read (and the primes and constants restaged).
+__device__ __forceinline__ Tile tile_setup_pf(const LeafSrc &L, u64 t, TileLds &S, const TilePF &f) {
+    Tile T;
+    T.j0 = L.lo + t * LT;
+    const u64 jend = L.lo + L.n;
+    T.cnt = (u32)((T.j0 + LT < jend ? T.j0 + LT : jend) - T.j0);
+    T.p0 = f.p0;
+    T.np = (u32)(f.p1 - f.p0 + 1);
+    {
+        const u32 q = threadIdx.x;
+        if (q < T.np) {
+            if (q < (u32)NCAP) {
+                S.xm[q] = f.xm;
+                S.x2[q] = f.x2;
+                S.nx[q] = (int)(short)(f.pos & 0xFFFFu);
+                S.kb[q] = (int)(long long)(T.j0 - f.off);
+            }
+            if (q) S.own[f.off - T.j0] = (unsigned short)q;
+            if (q == 0) S.meta_first = f.meta;
+            if (q == T.np - 1) S.meta_last = f.meta;
+        }
+        // parents beyond the block's threads (a tile of many short subtrees): read here
+        for (u32 qq = LPB + threadIdx.x; qq < T.np; qq += LPB) {
+            const Node &nd = L.nodes[T.p0 + qq];
+            const u64 o = L.off[T.p0 + qq];
+            if (qq < (u32)NCAP) {
+                S.xm[qq] = nd.xm;
+                S.x2[qq] = nd.x2;
+                S.nx[qq] = node_nxt(nd);
+                S.kb[qq] = (int)(long long)(T.j0 - o);
+            }
+            S.own[o - T.j0] = (unsigned short)qq;
+            if (qq == T.np - 1) S.meta_last = nd.meta;
+        }
+    }
+    lds_sync();
+    const u32 i0 = S.meta_first >> 8, i1 = S.meta_last >> 8;
+    bool fastok;
+    if (i0 == i1 && (int)i0 != S.inst) {
+        // a new instance: its descriptor decides the path; primes and constants restaged
+        const SideDesc &sd = L.descs[i0];
+        fastok = sd.leaf32 && sd.n <= PCAP;
+        if (fastok) {
+            for (int x = threadIdx.x; x < sd.n; x += LPB) S.prime[x] = (u32)sd.p[x];
+            if (threadIdx.x == 0) {
+                S.m = sd.mc.m;
+                S.mask2 = sd.mc.mask2;
+                S.koff = sd.koff;
+                S.minv = sd.mc.minv;
+                S.mneg32 = (u32)sd.mc.mneg;
+                S.sideI = sd.side == SIDE_I;
+            }
+        }
+        lds_sync();   // every thread has read S.inst and S.fastok above
+        if (threadIdx.x == 0) {
+            S.inst = (int)i0;
+            S.fastok = fastok;
+        }
+    } else {
+        fastok = S.fastok != 0;
+    }
+    T.fast = i0 == i1 && T.np <= (u32)NCAP && fastok;
+    // running maximum over own: LPT entries per thread, then a block scan of the thread maxima
+    u32 run = 0;
+    unsigned short v[LPT];
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        v[k] = S.own[threadIdx.x * LPT + k];
+        run = run > v[k] ? run : v[k];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    u32 inc = run;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u32 o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc = inc > o ? inc : o;
+    }
+    if (lane == 63) S.w[wv] = inc;
+    lds_sync();
+    u32 pre = 0;
+    for (int q = 0; q < wv; ++q) pre = pre > S.w[q] ? pre : S.w[q];
+    u32 ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 0;
+    u32 cur = pre > ex ? pre : ex;
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        cur = cur > v[k] ? cur : v[k];
+        S.own[threadIdx.x * LPT + k] = (unsigned short)cur;
+    }
+    lds_sync();
+    return T;
+}
+
 // Before a kernel's first tile_setup: no instance staged yet, owner map clear.
 __device__ __forceinline__ void tile_lds_init(TileLds &S) {
-    if (threadIdx.x == 0) S.inst = -1;
+    if (threadIdx.x == 0) {
+        S.inst = -1;
+        S.fastok = 0;
+    }
     for (u32 i = threadIdx.x; i < (u32)LT; i += LPB) S.own[i] = 0;
     __syncthreads();
 }
@@ -418,18 +557,31 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
     const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    // two-deep pipeline: tile t + 1's parents are fetched while tile t is ranked and written, and
+    // tile t + 2's owner pair one tile earlier still
+    TilePF pf{};
+    u64 tpv = 0;
+    if (t0 < t1) {
+        const u64 v0 = tile_owner_load(L, t0);
+        tile_fetch(L, lane_u64(v0, 0), lane_u64(v0, 1), pf);
+    }
+    if (t0 + 1 < t1) tpv = tile_owner_load(L, t0 + 1);
     for (u64 t = t0; t < t1; ++t) {
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
-        const Tile T = tile_setup<true>(L, t, S);
+        const Tile T = tile_setup_pf(L, t, S, pf);
         u64 h[LPT];
         u64 kk[KEYS ? LPT : 1];
         if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
         else tile_hashes(L, T, S, h);
+        if (t + 1 < t1) {
+            tile_fetch(L, lane_u64(tpv, 0), lane_u64(tpv, 1), pf);
+            if (t + 2 < t1) tpv = tile_owner_load(L, t + 2);
+        }
         u32 rank[LPT];
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
             if (threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
-        __syncthreads();   // also: every lane is done with the staged parents and the owner map
+        lds_sync();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
         for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
         // tile-local exclusive offsets of the partitions (block scan of hist)
@@ -443,12 +595,12 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                 if (lane >= d) inc += o;
             }
             if (lane == 63) S.w[wv] = inc;
-            __syncthreads();
+            lds_sync();
             u32 pre = 0;
             for (int q = 0; q < wv; ++q) pre += S.w[q];
             if (threadIdx.x < 256) loc[threadIdx.x] = pre + inc - v;
         }
-        __syncthreads();
+        lds_sync();
         // staging: each record at its partition's run in the tile, with its final slot beside it
         // (capacity layout: partition << 16 | slot in the segment, or ~0 past the segment's end;
         // exact layout: the record's index, below 2^32 for a chunk of at most 2^32 leaves)
@@ -473,18 +625,24 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                 if constexpr (KEYS) s_k[at] = kk[k];
             }
         }
-        __syncthreads();
-        for (u32 q = threadIdx.x; q < T.cnt; q += LPB) {
-            const u32 d = s_dst[q];
-            if (d == ~0u) continue;
-            const u64 g = exact ? (u64)d : ((((u64)(d >> 16) * nblk + blockIdx.x) << capl) + (d & 0xFFFFu));
-            rec[g] = s_r[q];
-            if constexpr (KEYS) keys[g] = s_k[q];
+        lds_sync();
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) {   // a fixed number of store sites: the prefetch is not waited on
+            const u32 q = threadIdx.x + LPB * k;
+            if (q < T.cnt) {
+                const u32 d = s_dst[q];
+                if (d != ~0u) {
+                    const u64 g =
+                        exact ? (u64)d : ((((u64)(d >> 16) * nblk + blockIdx.x) << capl) + (d & 0xFFFFu));
+                    rec[g] = s_r[q];
+                    if constexpr (KEYS) keys[g] = s_k[q];
+                }
+            }
         }
-        __syncthreads();
+        lds_sync();
         if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
     }
-    __syncthreads();
+    lds_sync();
     if (!exact) {
         if (threadIdx.x < (unsigned)P) {
             const u64 c = cur[threadIdx.x];
@@ -642,7 +800,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     u64 cbase = 0;
     u32 cleft = 0;
     if (threadIdx.x == 0) s_nx[0] = t_lo + atomicAdd(&xq[xcd], 1u);
-    __syncthreads();
+    lds_sync();
     u64 t = uniform_u64(s_nx[0]);
     for (u32 it = 1;; ++it) {
         if (t >= t_hi) break;
@@ -659,7 +817,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
             for (u64 i = threadIdx.x; i < nw; i += PBP) sF[i] = F[w0 + i];
             fbit0 = (u32)(bit0 & 31);
             cur_part = part;
-            __syncthreads();
+            lds_sync();
         }
         const u64 gp = part * S.nblk;            // the partition's first segment
         const u64 gend = gp + S.nblk;
@@ -753,7 +911,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                 }
             }
         }
-        __syncthreads();
+        lds_sync();
         t = uniform_u64(s_nx[it & 1]);
     }
     for (u32 i = lane; i < cleft; i += 64)

@@ -159,7 +159,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
     } else if (OWN_ZEROED) {
         for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
     }
-    __syncthreads();
+    lds_sync();
     if (!OWN_ZEROED) {
         if (T.fast)
             for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[-S.kb[q]] = (unsigned short)q;
@@ -167,7 +167,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
             for (u32 q = 1 + threadIdx.x; q < T.np; q += LPB) S.own[L.off[T.p0 + q] - T.j0] = (unsigned short)q;
     }
     if (restage && threadIdx.x == 0) S.inst = (int)i0;
-    if (!OWN_ZEROED) __syncthreads();
+    if (!OWN_ZEROED) lds_sync();
     // running maximum over own: LPT entries per thread, then a block scan of the thread maxima
     u32 run = 0;
     unsigned short v[LPT];
@@ -184,7 +184,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
         if (lane >= d) inc = inc > o ? inc : o;
     }
     if (lane == 63) S.w[wv] = inc;
-    __syncthreads();
+    lds_sync();
     u32 pre = 0;
     for (int q = 0; q < wv; ++q) pre = pre > S.w[q] ? pre : S.w[q];
     u32 ex = __shfl_up(inc, 1, 64);
@@ -195,7 +195,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
         cur = cur > v[k] ? cur : v[k];
         S.own[threadIdx.x * LPT + k] = (unsigned short)cur;
     }
-    __syncthreads();
+    lds_sync();
     return T;
 }
 
@@ -330,7 +330,7 @@ __device__ __forceinline__ void tile_lds_init(TileLds &S) {
         S.fastok = 0;
     }
     for (u32 i = threadIdx.x; i < (u32)LT; i += LPB) S.own[i] = 0;
-    __syncthreads();
+    lds_sync();
 }
 
 // Modulus constants of a fast tile, read once from LDS.
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(LPB) k_bucket_count(LeafSrc L, KeyHash kh, u32
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
             if (threadIdx.x + LPB * k < T.cnt) atomicAdd(&bcount[hbucket(h[k], kh)], 1u);
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));
             }
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -516,9 +516,9 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
             if (threadIdx.x + LPB * k < T.cnt) atomicAdd(&hist[hpart(h[k], lgP)], 1u);
-        __syncthreads();
+        lds_sync();
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < (unsigned)P) H[(u64)threadIdx.x * nblk + blockIdx.x] = hist[threadIdx.x];
 }
 
@@ -958,7 +958,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
     const u64 st = has ? seg_start(S, g) : 0;
     if (threadIdx.x < 256) hist[threadIdx.x] = 0;
     if (threadIdx.x == 0) s_ovf = 0;
-    __syncthreads();
+    lds_sync();
     // a segment holds at most 512 records (capacity layout) -- the exact layout's larger segments
     // are taken 512 at a time by the loop below
     for (u32 c0 = 0;; c0 += 512) {
@@ -976,7 +976,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
                 rank[k] = atomicAdd(&hist[win[k]], 1u);
             }
         }
-        __syncthreads();
+        lds_sync();
         // tile offsets of the windows (block scan of hist), and one reservation per window present
         const u32 v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
         {
@@ -987,7 +987,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
                 if (lane >= d) inc += o;
             }
             if (lane == 63) s_w[wv] = inc;
-            __syncthreads();
+            lds_sync();
             u32 pre = 0;
             for (int q = 0; q < wv; ++q) pre += s_w[q];
             if (threadIdx.x < 256) {
@@ -995,7 +995,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
                 woff[threadIdx.x] = v ? atomicAdd(&wcount[part * dw.R + threadIdx.x], v) : 0u;
             }
         }
-        __syncthreads();
+        lds_sync();
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const u32 vv = (u32)k * 64 + lane;
@@ -1005,7 +1005,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
                 s_win[at] = (u8_t)win[k];
             }
         }
-        __syncthreads();
+        lds_sync();
         u32 tot = 0;
         for (int q = 0; q < PB / 64; ++q) tot += s_w[q];
         for (u32 q = threadIdx.x; q < tot; q += PB) {
@@ -1017,7 +1017,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
             }
             wreg[((u64)part * dw.R + w) * CAPW + o] = s_r[q];
         }
-        __syncthreads();
+        lds_sync();
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         // every wave of the block runs the same number of rounds: the largest of its segments
         const u32 more = __syncthreads_or(c0 + 512 < fill);
@@ -1050,7 +1050,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     const u64 gb = part * K + b_lo;                            // first bucket, global
     for (u32 i = threadIdx.x; i <= nb; i += DSB) cnt[i] = 0;
     for (u32 i = threadIdx.x; i < n; i += DSB) s_r[i] = wreg[pw * CAPW + i];
-    __syncthreads();
+    lds_sync();
     constexpr int RPT = (CAPW + DSB - 1) / DSB;
     u32 rk[RPT];
     bool bad = false;
@@ -1069,7 +1069,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         if (threadIdx.x == 0) atomicOr(kerr, 1u);
         return;
     }
-    __syncthreads();
+    lds_sync();
     // exclusive scan of cnt[0, nb) into pos (DSB threads, consecutive runs of PER entries)
     constexpr int PER = WMAXB / DSB;
     u32 sum = 0, cv[PER];
@@ -1086,7 +1086,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         if (lane >= d) inc += o;
     }
     if (lane == 63) s_w[wv] = inc;
-    __syncthreads();
+    lds_sync();
     u32 run = inc - sum;
     for (int q = 0; q < wv; ++q) run += s_w[q];
 #pragma unroll
@@ -1101,13 +1101,13 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     if (threadIdx.x == DSB - 1) pos[nb] = run;
     // the last window of the last partition closes bstart
     if (pw + 1 == gridDim.x && threadIdx.x == DSB - 1) bstart[gb + nb] = base + run;
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const u32 i = threadIdx.x + (u32)k * DSB;
         if (i < n) s_idx[pos[(s_r[i].x >> kh.fpb) - (u32)b_lo] + rk[k]] = (unsigned short)i;
     }
-    __syncthreads();
+    lds_sync();
     // records in bucket order
     for (u32 k = threadIdx.x; k < n; k += DSB) {
         const uint4 r = s_r[s_idx[k]];

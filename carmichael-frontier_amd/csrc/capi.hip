@@ -32,11 +32,12 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     const size_t sz = pool_round(bytes);
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (ctx->arena) {
-        auto a = ctx->arena->free.lower_bound(sz);
-        if (a != ctx->arena->free.end() && a->first <= sz + sz / 2) {
+        auto &fl = ctx->arena->free[ctx->arena_branch];
+        auto a = fl.lower_bound(sz);
+        if (a != fl.end() && a->first <= sz + sz / 2) {
             void *p = a->second;
             ctx->live[p] = a->first;
-            ctx->arena->free.erase(a);
+            fl.erase(a);
             return p;
         }
     }
@@ -82,7 +83,7 @@ void ctx_free(cf_ctx *ctx, void *p) {
     auto it = ctx->live.find(p);
     if (it == ctx->live.end()) return;
     if (ctx->arena) {
-        ctx->arena->free.emplace(it->second, p);
+        ctx->arena->free[ctx->arena_branch].emplace(it->second, p);
         ctx->arena->owned[p] = it->second;
         ctx->live.erase(it);
         return;

@@ -69,12 +69,16 @@ struct cf_ctx {
     // later allocations of the same capture) and every block handed out is recorded in it, so
     // nothing a graph node touches returns to the shared pool while the graph lives
     struct CaptureArena *arena = nullptr;
+    // branch of the capture being recorded (0: the capturing stream; k > 0: side stream k, forked
+    // from it and joined back): a block freed on one branch is reused only on that branch, whose
+    // stream order keeps the reuse safe while branches run side by side
+    int arena_branch = 0;
     size_t peak_live = 0;                         // largest live footprint since the last reset
 };
 
 struct CaptureArena {
-    std::multimap<size_t, void *> free;   // freed during the capture
-    std::map<void *, size_t> owned;       // every block the captured work uses
+    std::map<int, std::multimap<size_t, void *>> free;   // freed during the capture, per branch
+    std::map<void *, size_t> owned;                      // every block the captured work uses
 };
 // return an arena's blocks to its context's pool (the graph that used them is gone)
 void arena_release(cf_ctx *ctx, CaptureArena *a);

@@ -233,6 +233,10 @@ struct cf_batch {
     u64 pair_cap = 1 << 16;                      // pair capacity of the next solve (grows to fit)
     Tuning tune;                                 // cf_batch_set_tuning
     BatchGraph *graph = nullptr;                 // the captured solve, once cf_batch_capture ran
+    // side streams (default priority) and fork/join events of the captured solve: each radius
+    // group is captured on its own branch, so the replayed groups run side by side
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> cap_events;
     ~cf_batch();
 };
 
@@ -306,6 +310,31 @@ u64 default_chunk() {
     return std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 48, (u64)1 << 24));
 }
 
+// One radius group's branch of a captured solve: while it lives, launches go to the group's side
+// stream (forked from the capturing stream) and blocks the group frees are reused only on that
+// branch; on leaving it records the group's completion event, which the capturing stream waits on.
+// Declared first in the group's scope, so the group's buffers are freed while the branch is set.
+// Inactive (no fork) for an eager solve.
+struct GroupBranch {
+    cf_ctx *ctx;
+    hipStream_t main = nullptr;
+    hipEvent_t done = nullptr;
+    GroupBranch(cf_ctx *c, hipStream_t side, hipEvent_t fork, hipEvent_t done_ev, int branch) : ctx(c) {
+        if (!side) return;
+        main = ctx->stream;
+        done = done_ev;
+        CF_HIP(hipStreamWaitEvent(side, fork, 0));
+        ctx->stream = side;
+        ctx->arena_branch = branch;
+    }
+    ~GroupBranch() {
+        if (!main) return;
+        (void)hipEventRecord(done, ctx->stream);
+        ctx->stream = main;
+        ctx->arena_branch = 0;
+    }
+};
+
 // tape: null for a plain eager solve; RECORD to note every value read back; REPLAY while the
 // stream is being captured (no value is read back and no timing is taken)
 void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_part = true, Tape *tape = nullptr) {
@@ -329,8 +358,6 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
     out.kerr = DBuf<u32>(ctx, 1);           // set by a kernel that met an inconsistent record
     DBuf<u64> &cnt_pairs = out.cnt_pairs;
     DBuf<u32> &kerr = out.kerr;
-    DBuf<u32> xq(ctx, 8), ovf(ctx, 1);
-    DBuf<u64> flags(ctx, 1);                // candidate count of the last probe
     fill_u32(ctx, kerr.p, 1, 0);
     double ms_d = 0, ms_i = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -341,9 +368,37 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         fill_u64(ctx, dcount.p, n, 0);
         fill_u64(ctx, icount.p, n, 0);
         ms_d = ms_i = 0;
-        // one radius group at a time: its D side is bucketed into its own probe table, then its I
-        // side is generated, partitioned and probed against it
+        // each radius group: its D side is bucketed into its own probe table, then its I side is
+        // generated, partitioned and probed against it. Eager: one group after another. Captured:
+        // each group on its own branch (side stream), all forked from and joined back to the
+        // capturing stream, so a replay runs the groups side by side.
+        const bool fork = capturing && groups.size() > 1;
+        hipEvent_t ev_fork = nullptr;
+        std::vector<hipEvent_t> ev_done;
+        if (fork) {
+            const size_t ns = std::min<size_t>(groups.size(), 4);
+            while (b->side.size() < ns) {
+                hipStream_t st = nullptr;
+                CF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                b->side.push_back(st);
+            }
+            auto mk = [&]() {
+                hipEvent_t e = nullptr;
+                CF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                b->cap_events.push_back(e);
+                return e;
+            };
+            ev_fork = mk();
+            for (size_t k = 0; k < groups.size(); ++k) ev_done.push_back(mk());
+            CF_HIP(hipEventRecord(ev_fork, ctx->stream));
+        }
+        size_t gi = 0;
         for (auto &g : groups) {
+            const size_t si = gi % std::max<size_t>(b->side.size(), 1);
+            GroupBranch branch(ctx, fork ? b->side[si] : nullptr, ev_fork, fork ? ev_done[gi] : nullptr, (int)si + 1);
+            ++gi;
+            DBuf<u32> xq(ctx, 8), ovf(ctx, 1);
+            DBuf<u64> flags(ctx, 1);                // candidate count of the last probe
             PhaseClock gc(ctx);
             if (!capturing) gc.mark();
             Frontier fd = expand_to_last(ctx, d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first, tape);
@@ -457,6 +512,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                 ms_i += gc.ms(1, 2);
             }
         }
+        for (hipEvent_t e : ev_done) CF_HIP(hipStreamWaitEvent(ctx->stream, e, 0));
         const u64 np = host_value(ctx, tape, cnt_pairs.p);
         if (host_value_u32(ctx, tape, kerr.p))
             throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
@@ -520,6 +576,8 @@ static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
 
 cf_batch::~cf_batch() {
     if (graph) graph_destroy(ctx, graph);
+    for (hipEvent_t e : cap_events) (void)hipEventDestroy(e);
+    for (hipStream_t st : side) (void)hipStreamDestroy(st);
 }
 
 extern "C" {

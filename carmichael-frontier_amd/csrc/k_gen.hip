@@ -27,57 +27,78 @@ __device__ __forceinline__ int min_large_after(int x, int t, int split_idx) {
     return c > 0 ? c : 0;
 }
 
+// Tables a level's kernels read: the bound table, the primes and their Montgomery forms. GTab reads
+// them from the instance's descriptor in global memory; LTab from copies a block staged in LDS (one
+// row of the bound table, the one every child of the launch reads).
+struct GTab {
+    const SideDesc &s;
+    __device__ __forceinline__ U128 bound(int t, int x) const { return s.bound[(size_t)t * s.n + x]; }
+    __device__ __forceinline__ u64 p(int x) const { return s.p[x]; }
+    __device__ __forceinline__ u64 pR(int x) const { return s.pR[x]; }
+};
+struct LTab {
+    const U128 *brow;
+    const u64 *p_, *pR_;
+    __device__ __forceinline__ U128 bound(int, int x) const { return brow[x]; }
+    __device__ __forceinline__ u64 p(int x) const { return p_[x]; }
+    __device__ __forceinline__ u64 pR(int x) const { return pR_[x]; }
+};
+
 // Is index x an admissible choice at a node with product A, `nl` large primes so far, t more
 // choices to follow?  I side: A * a_x * ... * a_{x+t} <= Icap - 1 and the stratum can still hold.
-__device__ __forceinline__ bool valid_i(const SideDesc &s, U128 A, int nl, int x, int t) {
-    const U128 h = s.bound[(size_t)t * s.n + x];
+template <class Tab>
+__device__ __forceinline__ bool valid_i(const SideDesc &s, const Tab &tb, U128 A, int nl, int x, int t) {
+    const U128 h = tb.bound(t, x);
     if (u128_gt(A, h)) return false;
     const int nlx = nl + (x >= s.split_idx);
     return nlx + min_large_after(x, t, s.split_idx) <= s.hmax;
 }
 // D side: A * b_i * b_{i-1} * ... * b_{i-t} >= Dmin, i.e. A >= G[t][i].
-__device__ __forceinline__ bool valid_d(const SideDesc &s, U128 A, int i, int t) {
-    return !u128_lt(A, s.bound[(size_t)t * s.n + i]);
+template <class Tab>
+__device__ __forceinline__ bool valid_d(const Tab &tb, U128 A, int i, int t) {
+    return !u128_lt(A, tb.bound(t, i));
 }
 
 // Children of a depth-`depth` node: returns the count; the children are nxt, nxt+1, ... (I) or
 // nxt, nxt-1, ... (D).
-__device__ int child_count(const SideDesc &s, const Node &nd, int depth) {
+template <class Tab>
+__device__ int child_count(const SideDesc &s, const Tab &tb, const Node &nd, int depth) {
     const int t = s.r - depth - 1;
     const U128 A = {nd.A_lo, nd.A_hi};
     const int nxt = node_nxt(nd);
     if (s.side == SIDE_I) {
         const int nl = (int)(nd.meta & 0xFFu);
         int lo = nxt, hi = s.n - t;   // candidates [lo, hi)
-        if (lo >= hi || !valid_i(s, A, nl, lo, t)) return 0;
+        if (lo >= hi || !valid_i(s, tb, A, nl, lo, t)) return 0;
         // largest valid x: valid on [lo, v], invalid on (v, hi)
         int a = lo, b = hi - 1;
         while (a < b) {
             int mid = (a + b + 1) >> 1;
-            if (valid_i(s, A, nl, mid, t)) a = mid; else b = mid - 1;
+            if (valid_i(s, tb, A, nl, mid, t)) a = mid; else b = mid - 1;
         }
         return a - lo + 1;
     } else {
         int lo = t, hi = nxt;          // candidates hi, hi-1, ..., lo
-        if (hi < lo || !valid_d(s, A, hi, t)) return 0;
+        if (hi < lo || !valid_d(tb, A, hi, t)) return 0;
         int a = lo, b = hi;            // smallest valid i
         while (a < b) {
             int mid = (a + b) >> 1;
-            if (valid_d(s, A, mid, t)) b = mid; else a = mid + 1;
+            if (valid_d(tb, A, mid, t)) b = mid; else a = mid + 1;
         }
         return hi - a + 1;
     }
 }
 
 // Child number k of a depth-`depth` node.
-__device__ __forceinline__ Node make_child(const SideDesc &s, const Node &nd, int depth, int k) {
+template <class Tab>
+__device__ __forceinline__ Node make_child(const SideDesc &s, const Tab &tb, const Node &nd, int depth, int k) {
     const int x = (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
-    const u64 p = s.p[x];
+    const u64 p = tb.p(x);
     Node c;
     const U128 A = u128_mul64(U128{nd.A_lo, nd.A_hi}, p);
     c.A_lo = A.lo;
     c.A_hi = A.hi;
-    c.xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
+    c.xm = cf_mulmod_m(nd.xm, tb.pR(x), s.mc);
     c.x2 = nd.x2 * p;
     c.id = id_update(s, nd.id, x, depth);
     const int nn = (s.side == SIDE_I) ? x + 1 : x - 1;
@@ -92,7 +113,7 @@ __global__ void __launch_bounds__(BLK) k_expand_count(const SideDesc *__restrict
     for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) {
         const Node nd = nodes[j];
         const SideDesc &s = descs[nd.meta >> 8];
-        cnt[j] = (u32)child_count(s, nd, depth);
+        cnt[j] = (u32)child_count(s, GTab{s}, nd, depth);
     }
 }
 
@@ -122,40 +143,133 @@ __global__ void __launch_bounds__(BLK) k_expand_write(const SideDesc *__restrict
             const size_t p = find_parent(off, p0, p1, j);
             const Node nd = nodes[p];
             const SideDesc &s = descs[nd.meta >> 8];
-            out[j] = make_child(s, nd, depth, (int)(j - off[p]));
+            out[j] = make_child(s, GTab{s}, nd, depth, (int)(j - off[p]));
         }
     }
 }
 
-// Owner of the first output of every BLK-output tile: parent p's outputs are [off[p], off[p+1]), so
+// A level is written in tiles of EC = BLK * EPT outputs, one block of BLK threads each (EPT outputs
+// per thread): EPT = 4 on large levels, so a block's staged tables serve 1024 outputs; EPT = 1 on
+// small ones (the top of the trees, where tiles span many instances and take the global path), so
+// no thread runs several searches of global tables one after another.
+constexpr int NSTAGE = 512;      // instances with at most this many primes have their tables staged
+constexpr u64 EPT4_MIN = 1u << 20;   // outputs of a level written with EPT = 4
+
+// Owner of the first output of every ec-output tile: parent p's outputs are [off[p], off[p+1]), so
 // it owns the tiles that start there (a parent has at most a pool's worth of children, so a few
 // tiles at most). One thread per parent, one coalesced pass over off.
-__global__ void __launch_bounds__(BLK) k_mark_owners(const u64 *__restrict__ off, size_t nn, u64 *__restrict__ tp) {
+__global__ void __launch_bounds__(BLK) k_mark_owners(const u64 *__restrict__ off, size_t nn, u64 ec,
+                                                     u64 *__restrict__ tp) {
     for (size_t p = (size_t)blockIdx.x * BLK + threadIdx.x; p < nn; p += (size_t)gridDim.x * BLK) {
         const u64 a = off[p], b = off[p + 1];
         if (b <= a) continue;
-        for (u64 t = (a + BLK - 1) / BLK; t <= (b - 1) / BLK; ++t) tp[t] = p;
+        for (u64 t = (a + ec - 1) / ec; t <= (b - 1) / ec; ++t) tp[t] = p;
     }
 }
 
-// One level down with the next level's counts: child j of the level (parent found within its
-// tile's owner range) is written with the number of its own children, so the next level needs no
-// separate count pass.
+// One level down with the next level's counts: each child is written with the number of its own
+// children, so the next level needs no separate count pass. A tile's parents lie between the owner
+// of its first output and the owner of the next tile's first; each parent with children marks its
+// first output's slot in LDS, and a running maximum over the slots gives every output its parent.
+// When the tile's parents belong to one instance of at most NSTAGE primes (nearly every tile), its
+// primes, their Montgomery forms and the one bound-table row the children's counts read are staged
+// in LDS, so the counts' binary searches read LDS rather than global memory.
+template <int EPT>
 __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__restrict__ descs,
                                                            const Node *__restrict__ nodes, size_t nn,
                                                            const u64 *__restrict__ off,
                                                            const u64 *__restrict__ tp, size_t nout, int depth,
                                                            Node *__restrict__ out, u32 *__restrict__ cnt) {
-    const u64 j = (u64)blockIdx.x * BLK + threadIdx.x;
-    if (j >= nout) return;
-    // the tile's parents lie between the owners of its first output and of the next tile's first
+    __shared__ U128 s_b[NSTAGE];
+    __shared__ u64 s_p[NSTAGE], s_pr[NSTAGE];
+    constexpr int EC = BLK * EPT;
+    __shared__ u32 s_own[EC];
+    __shared__ u32 s_w[BLK / 64];
+    constexpr u64 MARKCAP = 8 * EC;   // parents per tile resolved through LDS marks (more: binary search)
+    const u64 j0 = (u64)blockIdx.x * EC;
+    const u32 cnt_t = (u32)(nout - j0 < (u64)EC ? nout - j0 : (u64)EC);
+    const size_t p0 = tp[blockIdx.x];
     const size_t p1 = blockIdx.x + 1 < gridDim.x ? tp[blockIdx.x + 1] : nn - 1;
-    const size_t p = find_parent(off, tp[blockIdx.x], p1, j);
-    const Node nd = nodes[p];
-    const SideDesc &s = descs[nd.meta >> 8];
-    const Node c = make_child(s, nd, depth, (int)(j - off[p]));
-    out[j] = c;
-    cnt[j] = (u32)child_count(s, c, depth + 1);
+    const u64 np = p1 - p0 + 1;
+    const u32 i0 = nodes[p0].meta >> 8, i1 = nodes[p1].meta >> 8;
+    const SideDesc &s0 = descs[i0];
+    const bool staged = i0 == i1 && s0.n <= NSTAGE;
+    const bool marked = np <= MARKCAP;
+    if (staged) {
+        const int t = s0.r - depth - 2;   // the children (depth + 1) count with row r - (depth + 1) - 1
+        for (int x = threadIdx.x; x < s0.n; x += BLK) {
+            s_b[x] = s0.bound[(size_t)t * s0.n + x];
+            s_p[x] = s0.p[x];
+            s_pr[x] = s0.pR[x];
+        }
+    }
+    if (marked) {
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) s_own[threadIdx.x * EPT + k] = 0;
+        __syncthreads();
+        for (u64 q = 1 + threadIdx.x; q < np; q += BLK) {
+            const u64 a = off[p0 + q];
+            if (a < j0 + cnt_t && off[p0 + q + 1] > a) s_own[a - j0] = (u32)q;
+        }
+        __syncthreads();
+        // running maximum: EPT slots per thread, then a block scan of the thread maxima
+        u32 v[EPT], run = 0;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            v[k] = s_own[threadIdx.x * EPT + k];
+            run = run > v[k] ? run : v[k];
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        u32 inc = run;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const u32 o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = inc > o ? inc : o;
+        }
+        if (lane == 63) s_w[wv] = inc;
+        __syncthreads();
+        u32 cur = 0;
+        for (int q = 0; q < wv; ++q) cur = cur > s_w[q] ? cur : s_w[q];
+        const u32 ex = __shfl_up(inc, 1, 64);
+        if (lane) cur = cur > ex ? cur : ex;
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) {
+            cur = cur > v[k] ? cur : v[k];
+            s_own[threadIdx.x * EPT + k] = cur;
+        }
+    }
+    __syncthreads();
+    // every parent is loaded before any child is built, so the EPT loads are in flight together
+    // (slots past the level's end repeat its last output and are not stored)
+    Node nd[EPT];
+    int kc[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const u32 i = threadIdx.x + BLK * k < cnt_t ? threadIdx.x + BLK * k : cnt_t - 1;
+        const u64 j = j0 + i;
+        const size_t p = marked ? p0 + s_own[i] : find_parent(off, p0, p1, j);
+        nd[k] = nodes[p];
+        kc[k] = (int)(j - off[p]);
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const u32 i = threadIdx.x + BLK * k;
+        Node c;
+        u32 cc;
+        if (staged) {
+            const LTab tb{s_b, s_p, s_pr};
+            c = make_child(s0, tb, nd[k], depth, kc[k]);
+            cc = (u32)child_count(s0, tb, c, depth + 1);
+        } else {
+            const SideDesc &s = descs[nd[k].meta >> 8];
+            c = make_child(s, GTab{s}, nd[k], depth, kc[k]);
+            cc = (u32)child_count(s, GTab{s}, c, depth + 1);
+        }
+        if (i < cnt_t) {
+            out[j0 + i] = c;
+            cnt[j0 + i] = cc;
+        }
+    }
 }
 
 // Leaves [leaf_lo, leaf_lo + nleaf) of the depth-(r-1) nodes; writes record j - leaf_lo.
@@ -245,11 +359,17 @@ void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
 void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                                size_t nout, int depth, Node *out, u32 *cnt) {
     if (!nout) return;
-    const u64 ntiles = (nout + BLK - 1) / BLK;
+    const bool wide = nout >= EPT4_MIN;
+    const u64 ec = wide ? 4 * BLK : BLK;
+    const u64 ntiles = (nout + ec - 1) / ec;
     DBuf<u64> tp(ctx, ntiles);
-    CF_LAUNCH(ctx, "k_mark_owners", k_mark_owners, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, off, nn, tp.p);
-    CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc, nodes,
-              nn, off, tp.p, nout, depth, out, cnt);
+    CF_LAUNCH(ctx, "k_mark_owners", k_mark_owners, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, off, nn, ec, tp.p);
+    if (wide)
+        CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count<4>, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc,
+                  nodes, nn, off, tp.p, nout, depth, out, cnt);
+    else
+        CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count<1>, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc,
+                  nodes, nn, off, tp.p, nout, depth, out, cnt);
 }
 
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off, u64 leaf_lo,

@@ -277,7 +277,9 @@ struct Tuning {
     int probe_blocks = 0;    // probe grid, 1024-thread blocks (default 256: one per CU)
 };
 u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
-// one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout)
+// one pass into fixed-size segments; *ovf set if a segment overflowed (then use the exact layout);
+// a chunk whose segments would number 2^32 slots or more is laid out exactly at once (and
+// part_cap_slots returns n for it)
 // tp receives the chunk's tile owner map (2 entries per 2048 leaves), which the exact layout and
 // pass 2 take as given
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,

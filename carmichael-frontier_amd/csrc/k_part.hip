@@ -529,6 +529,13 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // The capacity layout needs no first pass: its segments are sized for the expected share
 // (hashing spreads records evenly) with a margin of about 8 standard deviations.
 // With KEYS (the D side) every record's exact key goes to keys[] at the record's slot.
+// Slots are 32-bit: a chunk holds at most 2^32 records, and the caller takes the capacity layout
+// only when it has fewer than 2^32 slots (~0 marks a record past its segment's end).
+//
+// Per partition the block keeps its next slot gb[p] and the room left in its segment rm[p] (no
+// limit in the exact layout); per tile, one 64-bit word lc[p] = gb << 32 | min(room, 0xFFFF) << 16 |
+// the partition's offset in the tile gives a ranked record its staging place and its slot with one
+// LDS read.
 template <bool KEYS>
 __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                   const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
@@ -536,22 +543,26 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                                                   u64 *__restrict__ keys) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
-    __shared__ u32 loc[256];
-    __shared__ u64 cur[256];
+    __shared__ u32 gb[256], rm[256];
+    __shared__ u64 lc[256];
     __shared__ u32 s_ovf;
     __shared__ u64 s_k[KEYS ? LT : 1];
     // once a tile's leaves are hashed its parents' staged state is dead: the records are staged over
-    // xm and x2 (16 KB) and each record's destination over kb and nx (8 KB)
+    // xm and x2 (16 KB) and each record's slot over kb and nx (8 KB)
     uint2 *s_r = (uint2 *)S.xm;
     u32 *s_dst = (u32 *)S.kb;
     static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
     static_assert(sizeof(u32) * LT <= sizeof(int) * 2 * NCAP && offsetof(TileLds, nx) == offsetof(TileLds, kb) +
-                  sizeof(int) * NCAP, "destinations fit over kb, nx");
+                  sizeof(int) * NCAP, "slots fit over kb, nx");
     const int P = 1 << kh.lgP;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const bool exact = O != nullptr;
-    const u64 cap = (u64)1 << capl;
-    if (threadIdx.x < (unsigned)P) cur[threadIdx.x] = exact ? O[(u64)threadIdx.x * nblk + blockIdx.x] : 0;
+    const u32 cap = 1u << capl;
+    if (threadIdx.x < (unsigned)P) {
+        const u64 g = (u64)threadIdx.x * nblk + blockIdx.x;
+        gb[threadIdx.x] = exact ? (u32)O[g] : (u32)(g << capl);
+        rm[threadIdx.x] = exact ? 0xFFFFu : cap;
+    }
     if (threadIdx.x == 0) s_ovf = 0;
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
@@ -569,10 +580,20 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     for (u64 t = t0; t < t1; ++t) {
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         const Tile T = tile_setup_pf(L, t, S, pf);
-        u64 h[LPT];
+        // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
+        u32 lo[LPT], pt[LPT];
         u64 kk[KEYS ? LPT : 1];
-        if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
-        else tile_hashes(L, T, S, h);
+        {
+            u64 h[LPT];
+            if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
+            else tile_hashes(L, T, S, h);
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) {
+                const u32 hi = (u32)(h[k] >> 32);
+                pt[k] = hi >> (32 - kh.lgP);
+                lo[k] = (hoff32(hi, kh) << kh.fpb) | (hfp(h[k]) & fmask);
+            }
+        }
         if (t + 1 < t1) {
             tile_fetch(L, lane_u64(tpv, 0), lane_u64(tpv, 1), pf);
             if (t + 2 < t1) tpv = tile_owner_load(L, t + 2);
@@ -580,7 +601,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         u32 rank[LPT];
 #pragma unroll
         for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[hpart(h[k], kh.lgP)], 1u);
+            if (threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k]], 1u);
         lds_sync();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
         for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
@@ -598,56 +619,50 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             lds_sync();
             u32 pre = 0;
             for (int q = 0; q < wv; ++q) pre += S.w[q];
-            if (threadIdx.x < 256) loc[threadIdx.x] = pre + inc - v;
+            if (threadIdx.x < (unsigned)P) {
+                const u32 r = rm[threadIdx.x];
+                lc[threadIdx.x] = ((u64)gb[threadIdx.x] << 32) | ((r < 0xFFFFu ? r : 0xFFFFu) << 16) | (pre + inc - v);
+            }
         }
         lds_sync();
-        // staging: each record at its partition's run in the tile, with its final slot beside it
-        // (capacity layout: partition << 16 | slot in the segment, or ~0 past the segment's end;
-        // exact layout: the record's index, below 2^32 for a chunk of at most 2^32 leaves)
+        // staging: each record at its partition's run in the tile, its slot beside it
+        u32 over = 0;
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + LPB * k;
             if (i < T.cnt) {
-                const u32 part = hpart(h[k], kh.lgP);
-                const u32 at = loc[part] + rank[k];
-                const u64 o = cur[part] + rank[k];
-                u32 d;
-                if (exact) {
-                    d = (u32)o;
-                } else if (o < cap) {
-                    d = (part << 16) | (u32)o;
-                } else {
-                    d = ~0u;
-                    s_ovf = 1;
-                }
-                s_r[at] = make_uint2((hoff(h[k], kh) << kh.fpb) | (hfp(h[k]) & fmask), (u32)(T.j0 + i - L.lo));
-                s_dst[at] = d;
+                const u64 w = lc[pt[k]];
+                const u32 at = ((u32)w & 0xFFFFu) + rank[k];
+                const bool fits = rank[k] < ((u32)w >> 16);
+                over |= !fits;
+                s_r[at] = make_uint2(lo[k], (u32)(T.j0 + i - L.lo));
+                s_dst[at] = fits ? (u32)(w >> 32) + rank[k] : ~0u;
                 if constexpr (KEYS) s_k[at] = kk[k];
             }
         }
+        if (over) s_ovf = 1;
         lds_sync();
 #pragma unroll
         for (int k = 0; k < LPT; ++k) {   // a fixed number of store sites: the prefetch is not waited on
             const u32 q = threadIdx.x + LPB * k;
             if (q < T.cnt) {
                 const u32 d = s_dst[q];
-                if (d != ~0u) {
-                    const u64 g =
-                        exact ? (u64)d : ((((u64)(d >> 16) * nblk + blockIdx.x) << capl) + (d & 0xFFFFu));
-                    rec[g] = s_r[q];
-                    if constexpr (KEYS) keys[g] = s_k[q];
+                if (exact || d != ~0u) {
+                    rec[d] = s_r[q];
+                    if constexpr (KEYS) keys[d] = s_k[q];
                 }
             }
         }
+        if (threadIdx.x < (unsigned)P) {
+            gb[threadIdx.x] += v;
+            const u32 r = rm[threadIdx.x];
+            if (!exact) rm[threadIdx.x] = r > v ? r - v : 0u;
+        }
         lds_sync();
-        if (threadIdx.x < (unsigned)P) cur[threadIdx.x] += v;
     }
     lds_sync();
     if (!exact) {
-        if (threadIdx.x < (unsigned)P) {
-            const u64 c = cur[threadIdx.x];
-            fill[(u64)threadIdx.x * nblk + blockIdx.x] = (u32)(c < cap ? c : cap);
-        }
+        if (threadIdx.x < (unsigned)P) fill[(u64)threadIdx.x * nblk + blockIdx.x] = cap - rm[threadIdx.x];
         if (threadIdx.x == 0 && s_ovf) atomicOr(ovf, 1u);
     }
 }
@@ -1317,16 +1332,28 @@ static void cap_geometry(u64 n, const KeyHash &kh, const Tuning &tn, u64 &tpb, u
     nblk = (u32)((ntiles + tpb - 1) / tpb);
 }
 
-u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) {
+static u64 cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) {
     u64 tpb;
     u32 nblk;
     cap_geometry(n, kh, tn, tpb, nblk);
     return ((u64)nblk << kh.lgP) << CAPL;
 }
+// slots are 32-bit in the scatter: a layout of 2^32 slots or more is not taken
+static bool cap_fits(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_slots(n, kh, tn) < ((u64)1 << 32); }
+
+u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_fits(n, kh, tn) ? cap_slots(n, kh, tn) : n; }
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
                       DBuf<u64> &tp, u64 *keys) {
+    if (!cap_fits(n, kh, tn)) {
+        // too many segments for 32-bit slots (a chunk near 2^32 leaves at a low fill): the exact
+        // layout directly, never overflowing
+        fill_u32(ctx, ovf, 1, 0);
+        (void)leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+        part_scatter_exact(ctx, descs, nodes, nn, off, lo, n, depth, kh, lay, rec, tp.p, keys);
+        return;
+    }
     u64 tpb;
     u32 nblk;
     cap_geometry(n, kh, tn, tpb, nblk);

@@ -35,6 +35,14 @@ CF_HD u64 hbucket(u64 h, const KeyHash &kh) {
 // bucket offset within the record's partition, < K
 CF_HD u32 hoff(u64 h, const KeyHash &kh) { return (u32)(hbucket(h, kh) - (u64)hpart(h, kh.lgP) * kh_k(kh)); }
 CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
+// The same offset from the high word of h in 32-bit arithmetic: the range within the partition
+// times KR, plus floor(r * KR / 2^(32 - lgR)) taken as the high word of (r << lgR) * KR (r has
+// 32 - lgR bits, so r << lgR fits 32 bits; KR < 2^32 and the offset < K <= 2^24).
+CF_HD u32 umulhi32(u32 a, u32 b) { return (u32)(((u64)a * b) >> 32); }
+CF_HD u32 hoff32(u32 hi, const KeyHash &kh) {
+    const u32 rl = (hi >> (32 - kh.lgR)) & ((1u << (kh.lgR - kh.lgP)) - 1u);
+    return rl * (u32)kh.KR + umulhi32(hi << kh.lgR, (u32)kh.KR);
+}
 
 // Geometry for a radius group with nD D records: about 2 D records per bucket by default, so an
 // 8-slot bucket rarely overflows (Poisson tail P(X > 8) ~ 2.4e-4); P = 2^lgP probe partitions, as few

@@ -1,7 +1,8 @@
 // Host check of the join's bucket geometry (carmichael-frontier_amd/csrc/keyhash.h): for the
 // geometries chosen for D sides of 1 to 10^10 records, every hash lands in a bucket of its own hash
 // range and of its own partition, its offset in the partition fits the I record beside the
-// fingerprint, and the last and first hashes of every range stay on their side of the boundary.
+// fingerprint and equals the 32-bit form the leaf kernels compute (hoff32), and the last and first
+// hashes of every range stay on their side of the boundary.
 // A bucket map scaled per partition with K = ceil(NB / P) buckets each, K not a multiple of the
 // ranges per partition, lets a range's last hashes reach the next range's first bucket: that map
 // must fail the same boundary check. Exit status 0 = all hold.
@@ -17,7 +18,7 @@ static void check(const KeyHash &kh, u64 h) {
     const u64 d = hrange(h, kh), p = hpart(h, kh.lgP), K = kh_k(kh);
     const u32 o = hoff(h, kh);
     ++tot;
-    const bool ok = b < kh_nb(kh) && b >= d * kh.KR && b < (d + 1) * kh.KR && b >= p * K && b < (p + 1) * K &&
+    const bool ok = hoff32((u32)(h >> 32), kh) == o && b < kh_nb(kh) && b >= d * kh.KR && b < (d + 1) * kh.KR && b >= p * K && b < (p + 1) * K &&
                     o < K && ((u64)o << kh.fpb) < (1ULL << 32) && (d >> (kh.lgR - kh.lgP)) == p;
     if (!ok && bad++ < 5)
         printf("lgP=%d lgR=%d KR=%llu h=%016llx b=%llu\n", kh.lgP, kh.lgR, (unsigned long long)kh.KR,

@@ -13,7 +13,7 @@
 // 2 MB).
 //
 // Probe table T: 8 slots of 16 bits per bucket (16 bytes), slot k holding 0x8000 | f of the
-// bucket's k-th D record; slot 7 also carries 0x4000 when the bucket holds more than 8 records
+// bucket's k-th D record (f cut to the fpb fingerprint bits an I record keeps); slot 7 also carries 0x4000 when the bucket holds more than 8 records
 // (rare at 2 per bucket: such a bucket makes every probe of it a candidate). Bucket filter F: a
 // few bits per bucket, one per class of fingerprint low bits present; the probe keeps its
 // partition's slice of F in LDS and reads T only for records whose class is present.
@@ -457,6 +457,7 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
                                                        u32 *__restrict__ bcur, DRec *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
     __shared__ TileLds S;
+    const u32 fmask = (1u << kh.fpb) - 1u;
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
             DR[b0 + slot] = DRec{key, tl.j0 + i};
             if (slot < (u32)TSLOTS) {
                 const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
-                T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | hfp(h));
+                T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | (hfp(h) & fmask));
             }
         }
         lds_sync();
@@ -711,15 +712,16 @@ __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key,
     id = id_update(s, nd.id, x, L.depth);
 }
 
-// Is the slot pattern fs2 (0x8000 | f in both halves of a word, compared under mask mm) among the
-// 4 words of a half bucket? A slot matches when its masked value XOR the pattern is zero, so the
-// 8 slots reduce to one packed 16-bit minimum.
+// Is the slot pattern fs2 (0x8000 | f in both halves of a word) among the 8 slots of a bucket? Slots
+// hold 0x8000 | f with f cut to the record's fingerprint bits (slot 7 of an overflowing bucket also
+// has 0x4000 set, so it matches nothing), and a slot matches when it XOR the pattern is zero, so
+// the 8 slots reduce to one packed 16-bit minimum.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ bool half_hit(const uint4 &a, u32 fs2, u32 mm) {
-    const u16x2 x0 = __builtin_bit_cast(u16x2, (a.x & mm) ^ fs2);
-    const u16x2 x1 = __builtin_bit_cast(u16x2, (a.y & mm) ^ fs2);
-    const u16x2 x2 = __builtin_bit_cast(u16x2, (a.z & mm) ^ fs2);
-    const u16x2 x3 = __builtin_bit_cast(u16x2, (a.w & mm) ^ fs2);
+__device__ __forceinline__ bool bucket_hit(const uint4 &a, u32 fs2) {
+    const u16x2 x0 = __builtin_bit_cast(u16x2, a.x ^ fs2);
+    const u16x2 x1 = __builtin_bit_cast(u16x2, a.y ^ fs2);
+    const u16x2 x2 = __builtin_bit_cast(u16x2, a.z ^ fs2);
+    const u16x2 x3 = __builtin_bit_cast(u16x2, a.w ^ fs2);
     const u16x2 m = __builtin_elementwise_min(__builtin_elementwise_min(x0, x1), __builtin_elementwise_min(x2, x3));
     return (m.x == 0) | (m.y == 0);
 }
@@ -805,9 +807,9 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     const u32 xcd = blockIdx.x % NXCD;
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
     const u32 fmask = (1u << kh.fpb) - 1u;
-    const u32 mm = (0x8000u | fmask) * 0x10001u;
     const u64 K = kh_k(kh);
     const u32 fbm = FB > 0 ? (u32)FB - 1u : 0u;
+    const int lgfb = FB > 0 ? __builtin_ctz((u32)FB) : 0;   // FB is 4, 8 or 16
     u64 cur_part = ~0ull;
     u32 fbit0 = 0;                               // bit of the partition's first bucket within sF
     // the wave's reserved candidate slots [cbase, cbase + cleft): one counter update per CAND_CHUNK
@@ -826,10 +828,25 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
         if (FB > 0 && part != cur_part) {
             // this block's first tile of a partition: its filter slice into LDS (the barrier above
             // ordered every read of the previous slice before this)
+            // (every load of the slice is issued before the first LDS store waits on one)
             const u64 bit0 = part * K * (u64)FB;
             const u64 w0 = bit0 >> 5;
-            const u64 nw = ((bit0 + K * (u64)FB + 31) >> 5) - w0;
-            for (u64 i = threadIdx.x; i < nw; i += PBP) sF[i] = F[w0 + i];
+            const u32 nw = (u32)(((bit0 + K * (u64)FB + 31) >> 5) - w0);
+            constexpr int FPT = 16;   // words in flight per thread (a batch of FPT * PBP words)
+            for (u32 i0 = 0; i0 < nw; i0 += FPT * PBP) {
+                u32 fw[FPT];
+#pragma unroll
+                for (int j = 0; j < FPT; ++j) {
+                    const u32 i = i0 + threadIdx.x + (u32)j * PBP;
+                    fw[j] = i < nw ? F[w0 + i] : 0u;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < FPT; ++j) {
+                    const u32 i = i0 + threadIdx.x + (u32)j * PBP;
+                    if (i < nw) sF[i] = fw[j];
+                }
+            }
             fbit0 = (u32)(bit0 & 31);
             cur_part = part;
             lds_sync();
@@ -838,8 +855,19 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
         const u64 gend = gp + S.nblk;
         // the partition's table slice: byte offset hoff * 16 of bucket hoff
         const __amdgpu_buffer_rsrc_t tr = buf_rsrc(T + part * K * TSLOTS, K * TSLOTS * 2);
+        // fills and starts of the wave's GPW * SPW segments of this tile, one load each: lane l holds
+        // those of segment l mod (GPW * SPW), read out as wave-uniform values (a descriptor held in
+        // vector registers would put every record load in a loop over its distinct values)
+        static_assert(GPW * SPW <= 64, "one lane per segment of the wave");
+        u32 fv;
+        u64 sv;
+        {
+            const int l = lane % (GPW * SPW);
+            const u64 g = gp + (t - part * tpp) * SPT + ((u64)(l / SPW) * (PBP / 64) + wv) * SPW + (u64)(l % SPW);
+            fv = g < gend ? S.fill[g] : 0u;
+            sv = g < gend ? seg_start(S, g) : 0;
+        }
         for (int grp = 0; grp < GPW; ++grp) {
-            // the wave's segments: fills and starts (wave-uniform) read before the first is probed;
             // a segment's records through a descriptor that ends at its fill
             u32 fills[SPW];
             u64 starts[SPW];
@@ -847,9 +875,8 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
             u32 rounds_end = 0;
 #pragma unroll
             for (int sj = 0; sj < SPW; ++sj) {
-                const u64 g = gp + (t - part * tpp) * SPT + ((u64)grp * (PBP / 64) + wv) * SPW + sj;
-                fills[sj] = g < gend ? S.fill[g] : 0u;
-                starts[sj] = g < gend ? seg_start(S, g) : 0;
+                fills[sj] = (u32)__builtin_amdgcn_readlane((int)fv, grp * SPW + sj);
+                starts[sj] = lane_u64(sv, grp * SPW + sj);
                 rr[sj] = buf_rsrc(recs + starts[sj], (u64)fills[sj] * sizeof(uint2));
                 rounds_end = fills[sj] > rounds_end ? fills[sj] : rounds_end;
             }
@@ -874,25 +901,27 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     u32 pass = 0;
 #pragma unroll
                     for (int e = 0; e < NB2; ++e) {
-                        const u32 bit = fbit0 + (lo[e] >> kh.fpb) * (u32)FB + (lo[e] & fbm);
+                        const u32 bit = fbit0 + ((lo[e] >> kh.fpb) << lgfb) + (lo[e] & fbm);
                         pass |= ((sF[bit >> 5] >> (bit & 31)) & 1u) << e;
                     }
                     look &= pass;
                 }
-                // first halves; a record not looked up takes an offset past the slice (the load returns
-                // zeros without touching memory and cannot match)
+                // buckets; a record not looked up takes an offset past the slice (the load returns
+                // zeros without touching memory and cannot match). Every load of the round is issued
+                // before the first compare waits on one.
 #pragma unroll
                 for (int e = 0; e < NB2; ++e) {
                     const u32 off = (look >> e) & 1u ? (lo[e] >> kh.fpb) * (2u * TSLOTS) : 0x7FFFFFF0u;
                     const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(tr, (int)off, 0, 0);
                     ta[e] = make_uint4(x[0], x[1], x[2], x[3]);
                 }
+                __builtin_amdgcn_sched_barrier(0);
                 // a fingerprint among the bucket's slots, or the overflow flag of slot 7, is a candidate
                 u32 hmask = 0;
 #pragma unroll
                 for (int e = 0; e < NB2; ++e) {
                     const u32 fs2 = (0x8000u | (lo[e] & fmask)) * 0x10001u;
-                    hmask |= (u32)(half_hit(ta[e], fs2, mm) | ((ta[e].w & 0x40000000u) != 0)) << e;
+                    hmask |= (u32)(bucket_hit(ta[e], fs2) | ((ta[e].w & 0x40000000u) != 0)) << e;
                 }
                 hmask &= look;
                 if (__any(hmask != 0)) {

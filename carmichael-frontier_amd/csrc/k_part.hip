@@ -578,6 +578,8 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         tile_fetch(L, lane_u64(v0, 0), lane_u64(v0, 1), pf);
     }
     if (t0 + 1 < t1) tpv = tile_owner_load(L, t0 + 1);
+    // the same wait for the first tile's parents (every entry to the loop finds them in registers)
+    asm volatile("" ::"v"(pf.xm), "v"(pf.x2), "v"(pf.off), "v"(pf.pos), "v"(pf.meta), "v"(tpv));
     for (u64 t = t0; t < t1; ++t) {
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         const Tile T = tile_setup_pf(L, t, S, pf);
@@ -643,8 +645,14 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         }
         if (over) s_ovf = 1;
         lds_sync();
+        // The next tile's prefetched parents and owner pair are waited for here, before this tile's
+        // stores: loads and stores share one counter that this target treats as completing out of
+        // order, so a wait for a load issued before pending stores waits for the stores as well.
+        // Waited for after them (at the next tile's set-up), every tile would wait for its own
+        // stores to reach memory; waited for here, the stores drain while the next tile is hashed.
+        asm volatile("" ::"v"(pf.xm), "v"(pf.x2), "v"(pf.off), "v"(pf.pos), "v"(pf.meta), "v"(tpv));
 #pragma unroll
-        for (int k = 0; k < LPT; ++k) {   // a fixed number of store sites: the prefetch is not waited on
+        for (int k = 0; k < LPT; ++k) {
             const u32 q = threadIdx.x + LPB * k;
             if (q < T.cnt) {
                 const u32 d = s_dst[q];

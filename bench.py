@@ -201,6 +201,107 @@ def c4_main(args):
     print(json.dumps(line), flush=True)
 
 
+def rows_main(args):
+    """The reference's own composition, stage by stage, on the M0 anchor: M0 = 1 768 248 177 696 000,
+    r = 6, ksmall base, the affordable pool of 375 primes, h <= 2 (logged: nD 8 340 300, nI 228 944 388).
+    The driver's GPU path (gpu/mitm_gpu_mstar_driver.py:101-118,209-238) through the drop-in modules:
+    D side filtered by Dmin (the GPU stand-in of mitm64_ddump, a plan), sorted once (sort_side); I
+    side from generate_chunks in 2e8-record chunks, each sorted and joined against D (join_sorted).
+    Beside it: the same instance as one fused solve (cf_solve), the full C(64,6) D side by
+    rank-range generation (generate, the DFILTER=0 path), and the CPU join on 16 host threads.
+    Stage times are wall times between device synchronizations; counts are checked against the log."""
+    sys.path.insert(0, os.path.join(ROOT, "carmichael-frontier_amd", "gpu"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mitm_gpu2 as G
+    import igen_gpu as IG
+    import oracle
+    from cfjoin import lib, pipeline, xp
+    from cfjoin import workloads as W
+    logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))
+    inst = W.anchor("M0", 6, logged)
+    lg = inst.meta["logged"]
+    ctx = lib.default_context()
+
+    def timed(f):
+        ctx.synchronize()
+        t = time.perf_counter()
+        out = f()
+        ctx.synchronize()
+        return out, time.perf_counter() - t
+
+    # warm-up: code objects and the device pool
+    pipeline.solve(inst, ctx)
+    st = {}
+    plan, st["d_plan_s"] = timed(lambda: lib.Plan.deletion(inst.B0, inst.r, inst.M, inst.Dmin, lib.CF_ID_PACK_DESC))
+    nD = plan.total()
+    Dk, Did = xp.empty(nD), xp.empty(nD)
+    _, st["d_write_s"] = timed(lambda: plan.write(0, len(inst.B0), Dk.ptr, Did.ptr))
+    plan.close()
+    (Dks, Dids), st["d_sort_s"] = timed(lambda: G.sort_side(Dk, Did))
+    del Dk, Did
+    gen = IG.generate_chunks(inst.INS, inst.r, inst.M, inst.P0, inst.Icap, inst.split, inst.hmax,
+                             max_records=200_000_000)
+    t_gen = t_sort = t_join = 0.0
+    nI = pairs = chunks = 0
+    total = None
+    while True:
+        ctx.synchronize()
+        t = time.perf_counter()
+        try:
+            Ik, Iid = next(gen)
+        except StopIteration as e:
+            total = e.value
+            break
+        ctx.synchronize()
+        t_gen += time.perf_counter() - t
+        nI += int(Ik.size)
+        chunks += 1
+        (Iks, Iids), dt = timed(lambda: G.sort_side(Ik, Iid))
+        t_sort += dt
+        del Ik, Iid
+        (pd, pi), dt = timed(lambda: G.join_sorted(Dks, Dids, Iks, Iids))
+        t_join += dt
+        pairs += int(pd.size)
+        del Iks, Iids, pd, pi
+        print(f"[rows] I chunk {chunks}: {nI} records so far", file=sys.stderr, flush=True)
+    del Dks, Dids
+    ctx.release_pool()
+    t_comp = st["d_plan_s"] + st["d_write_s"] + st["d_sort_s"] + t_gen + t_sort + t_join
+    res, t_fused = timed(lambda: pipeline.solve(inst, ctx))
+    (fk, fi), t_full = timed(lambda: G.generate(inst.B0, inst.r, inst.M, 1, bits=6))
+    n_full = int(fk.size)
+    del fk, fi
+    ctx.release_pool()
+    cpu = None
+    if not args.no_cpu:
+        r = oracle.cpu_join(inst.M, inst.r, inst.P0 % inst.M, inst.Icap, inst.Dmin, inst.B0, inst.INS, inst.split_idx,
+                            inst.hmax, inst.rank_ids, threads=16)
+        cpu = dict(value=(r["nD"] + r["nI"]) / (r["td"] + r["ti"]), unit="residue keys joined/s", cores=16,
+                   kind="port", sample="the whole instance (oracle/oracle.c or_cpu_join: D hash build + I scan)",
+                   seconds=round(r["td"] + r["ti"], 2),
+                   pairs_equal=bool(np.array_equal(oracle.pair_set(r["pairs_d"], r["pairs_i"]), res.pair_array())))
+    keys = nD + nI
+    line = {
+        "metric": "reference call surface, stage by stage: residue keys joined/s",
+        "value": keys / t_comp, "unit": "residue keys joined/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "u64", "data": "synthetic: the anchor instance rebuilt from the census",
+        "config": {"workload": "M0 anchor, ksmall base, r=6, |INS|=375, h<=2, D filtered by Dmin, I chunks of 2e8",
+                   "nD": nD, "nI": nI, "i_chunks": chunks},
+        "stages_s": {"d_plan": round(st["d_plan_s"], 4), "d_write": round(st["d_write_s"], 4),
+                     "d_sort": round(st["d_sort_s"], 4), "i_generate_chunks": round(t_gen, 4),
+                     "i_sort_side": round(t_sort, 4), "join_sorted": round(t_join, 4), "total": round(t_comp, 4)},
+        "rates": {"i_records_per_s": nI / t_gen, "sort_keys_per_s": nI / t_sort, "join_probes_per_s": nI / t_join},
+        "fused_solve": {"value": keys / t_fused, "wall_s": round(t_fused, 4), "device_ms": res.stats["ms_total"],
+                        "pairs": len(res.pairs_d)},
+        "generate_full_C64_6": {"records": n_full, "wall_s": round(t_full, 4), "records_per_s": n_full / t_full},
+        "cpu_baseline": cpu,
+        "verify": {"nD_nI_pairs_equal_logged": (nD, nI, pairs) == (lg["nD"], lg["nI"], lg["pairs"]),
+                   "generate_chunks_total": total == nI, "fused_equals_composition": len(res.pairs_d) == pairs,
+                   "full_generate_equals_C64_6": n_full == 74974368},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def serve_main(args):
     """The serving measurement: one set of requests (both families, radii 4..8, up to 5e8 records
     each), served (a) by the continuous-batching server and (b) one request at a time (one worker,
@@ -283,6 +384,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="serving mode: server workers (streams)")
     ap.add_argument("--max-batch", type=int, default=64, help="serving mode: requests per batch at most")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--rows", action="store_true",
+                    help="the reference's own composition (generate_chunks, sort_side, join_sorted) stage by stage")
     ap.add_argument("--c4", action="store_true",
                     help="large-radius mode: the lambda(N) anchor at r = 8 with the full D side (C(64,8) records), "
                          "I side streamed in chunks, beside the reference's D-filtered run")
@@ -304,6 +407,9 @@ def main():
         return
     if args.c4:
         c4_main(args)
+        return
+    if args.rows:
+        rows_main(args)
         return
 
     logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))

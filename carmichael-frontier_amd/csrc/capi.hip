@@ -219,10 +219,12 @@ static void plan_first_counts(cf_plan *pl) {
                 pl->first_counts[x] = 1;
             }
         } else {
-            DBuf<u64> fc(pl->ctx, n);
-            CF_HIP(hipMemsetAsync(fc.p, 0, n * sizeof(u64), pl->ctx->stream));
-            launch_first_counts(pl->ctx, pl->leafp.p, pl->cnt.p, pl->nleafp, fc.p);
-            d2h(pl->ctx, pl->first_counts.data(), fc.p, n);
+            DBuf<u64> fb(pl->ctx, 2 * (size_t)n);
+            CF_HIP(hipMemsetAsync(fb.p, 0, 2 * (size_t)n * sizeof(u64), pl->ctx->stream));
+            launch_first_bounds(pl->ctx, pl->leafp.p, pl->off.p, pl->nleafp, fb.p, fb.p + n);
+            std::vector<u64> h(2 * (size_t)n);
+            d2h(pl->ctx, h.data(), fb.p, 2 * (size_t)n);
+            for (int x = 0; x < n; ++x) pl->first_counts[x] = h[n + x] - h[x];
         }
     }
     pl->have_first = true;

@@ -235,7 +235,8 @@ void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                        u64 leaf_lo, size_t nleaf, int depth, u64 *keys, u64 *ids);
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
-void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out);
+// per first index x: lo[x], hi[x] = leaf offsets where its nodes start and end (zeroed by the caller)
+void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi);
 u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 
 // radix-partitioned join (k_part.hip)

@@ -298,13 +298,17 @@ __global__ void __launch_bounds__(BLK) k_leaf_write(const SideDesc *__restrict__
     }
 }
 
-// Per first index: total leaves below it (atomic accumulation; nodes of one first index are
-// contiguous but a plain atomic is simpler and the node count is small next to the leaves).
-__global__ void k_first_counts(const Node *__restrict__ nodes, const u32 *__restrict__ cnt, size_t nn,
-                               unsigned long long *__restrict__ out) {
+// Per first index: the leaf offsets where its nodes start and end. The nodes of one first index
+// are one contiguous run (depth-first order), so the node opening a run writes lo[first] = off[j]
+// and the node closing it writes hi[first] = off[j + 1]; its leaves are hi - lo (0 - 0 for a first
+// index with no node). One streaming pass, no atomics (a per-node atomic on the run's counter
+// serializes every node of a run).
+__global__ void k_first_bounds(const Node *__restrict__ nodes, const u64 *__restrict__ off, size_t nn,
+                               u64 *__restrict__ lo, u64 *__restrict__ hi) {
     for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) {
-        const u32 c = cnt[j];
-        if (c) atomicAdd(&out[nodes[j].pos >> 16], (unsigned long long)c);
+        const u32 f = nodes[j].pos >> 16;
+        if (j == 0 || (nodes[j - 1].pos >> 16) != f) lo[f] = off[j];
+        if (j + 1 == nn || (nodes[j + 1].pos >> 16) != f) hi[f] = off[j + 1];
     }
 }
 
@@ -379,10 +383,10 @@ void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, s
                        leaf_lo, nleaf, depth, keys, ids);
 }
 
-void launch_first_counts(cf_ctx *ctx, const Node *nodes, const u32 *cnt, size_t nn, u64 *out) {
+void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi) {
     if (!nn) return;
-    CF_LAUNCH(ctx, "k_first_counts", k_first_counts, dim3(grid_for(nn, BLK, 4096)), dim3(BLK), 0, nodes, cnt, nn,
-                       (unsigned long long *)out);
+    CF_LAUNCH(ctx, "k_first_bounds", k_first_bounds, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, nodes, off, nn, lo,
+              hi);
 }
 
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids) {

@@ -302,6 +302,27 @@ def rows_main(args):
     print(json.dumps(line), flush=True)
 
 
+def audit_main(args):
+    """Every logged instance of both divisor families (1 412 lines of the reference's run logs),
+    rebuilt from the census and solved on the device in batches; (nD, nI, pairs) compared with the
+    log line by line, every pair re-checked exactly on the host (cfjoin/audit.py)."""
+    from cfjoin import audit
+    rep = audit.audit(os.path.join(ROOT, "tests", "golden", "logged_instances.json"),
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    line = {
+        "metric": "coverage audit: logged instances reproduced on the device",
+        "value": rep["keys"] / (rep["device_ms"] / 1e3) if rep["device_ms"] else 0.0,
+        "unit": "residue keys joined/s", "n_gpus": 1, "higher_is_better": True, "dtype": "u64",
+        "data": "synthetic: every logged instance rebuilt from the census",
+        "config": {"workload": "both divisor families, every logged (row, radius)", "batches": rep["batches"]},
+        "instances": rep["instances"], "solved_on_device": rep["solved"], "keys": rep["keys"],
+        "pairs": rep["pairs"], "device_ms": round(rep["device_ms"], 2), "wall_s": round(rep["wall_s"], 2),
+        "mismatches": rep["mismatches"], "recheck_failures": rep["recheck_failures"],
+        "beyond_device": rep["beyond_device"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def serve_main(args):
     """The serving measurement: one set of requests (both families, radii 4..8, up to 5e8 records
     each), served (a) by the continuous-batching server and (b) one request at a time (one worker,
@@ -384,6 +405,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="serving mode: server workers (streams)")
     ap.add_argument("--max-batch", type=int, default=64, help="serving mode: requests per batch at most")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--audit", action="store_true",
+                    help="every logged instance of both families solved and compared with the log")
     ap.add_argument("--rows", action="store_true",
                     help="the reference's own composition (generate_chunks, sort_side, join_sorted) stage by stage")
     ap.add_argument("--c4", action="store_true",
@@ -410,6 +433,9 @@ def main():
         return
     if args.rows:
         rows_main(args)
+        return
+    if args.audit:
+        audit_main(args)
         return
 
     logged = W.logged_table(os.path.join(ROOT, "tests", "golden", "logged_instances.json"))

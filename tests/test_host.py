@@ -40,6 +40,23 @@ def test_recipe_reproduces_every_logged_pool_and_stratum(workloads):
     assert checked == 1412
 
 
+def test_audit_batches_cover_every_logged_instance_once():
+    """The coverage audit's grouping (cfjoin/audit.py): every logged instance in exactly one batch,
+    each batch within its key and instance limits (a single larger instance alone) and with the sum
+    of its moduli below 2^64 - 1 (one key space per batch)."""
+    from cfjoin import audit
+    items = audit.logged_instances(os.path.join(GOLDEN, "logged_instances.json"))
+    assert len(items) == 1412
+    todo = [(r, x) for r, x in items if x.Icap.bit_length() <= audit.DEVICE_ICAP_BITS]
+    seen = 0
+    for g in audit._batches(todo, 2e9, 256):
+        keys = sum(r["nD"] + r["nI"] for r, _ in g)
+        assert len(g) == 1 or (keys <= 2e9 and len(g) <= 256)
+        assert sum(x.M for _, x in g) < (1 << 64) - 1
+        seen += len(g)
+    assert seen == len(todo) == 1411
+
+
 def test_batch_workload_shape(workloads):
     logged = W.logged_table(os.path.join(GOLDEN, "logged_instances.json"))
     insts = W.batch_instances("lambdaN", logged=logged)

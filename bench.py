@@ -86,6 +86,24 @@ def cpu_baseline(insts, res_batched, threads, budget_keys):
                 seconds=round(t_sum, 2), pair_set_mismatches=mismatches)
 
 
+def host_cpu_model() -> str:
+    """The host CPU's model name and the logical CPUs this process may use."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 0
+    return f"{name}, {n} logical CPUs available"
+
+
 def copy_probe(ctx, nbytes):
     """A plain device-to-device copy of nbytes: the calibration point for FETCH_SIZE / WRITE_SIZE."""
     from cfjoin import xp
@@ -549,6 +567,7 @@ def main():
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(insts, res_b, threads, args.cpu_keys)
+        cpu["host_cpu"] = host_cpu_model()
 
     line = {
         "metric": "residue keys joined/s (batched vs plain, 1 GPU), bit-exact match sets",
@@ -579,6 +598,11 @@ def main():
     }
     if cpu and cpu["value"]:
         line["vs_cpu"] = value / cpu["value"]
+    # the whole path against the materialized join's cost: a 16-byte (key, id) record written once and
+    # read once per key, 32 bytes; this path moves fewer (8-byte I records, D records in LDS-sorted
+    # windows), so the figure is what the reference's join shape would need to reach this rate
+    line["path_roofline"] = {"bytes_per_key": 32, "achieved": round(value * 32 / 1e9, 1), "peak": PEAK_HBM_GBS,
+                             "unit": "GB/s", "frac": round(value * 32 / 1e9 / PEAK_HBM_GBS, 4)}
     print(json.dumps(line), flush=True)
 
     if args.copy_probe:

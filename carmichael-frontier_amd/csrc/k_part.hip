@@ -28,6 +28,8 @@
 //          partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
 // HBM traffic per I record: 8 B written by the scatter + 8 B read by the probe.
 // Join semantics: every equal-key (D, I) pair (gpu/mitm_gpu2.py:107-119).
+#include <type_traits>
+
 #include "tree_dev.h"
 
 namespace {
@@ -105,6 +107,7 @@ struct TileLds {
     u64 m, mask2, koff, minv;
     u32 mneg32;
     int sideI, inst;
+    int narrow;                    // the staged instance's 2^a is at most 2^32 (x2 needs 32 bits)
     int fastok;                    // the staged instance takes the fast path (leaf32, n <= PCAP)
     u32 meta_first, meta_last;     // instance fields of the tile's first and last parent
 };
@@ -154,6 +157,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
                 S.minv = sd.mc.minv;
                 S.mneg32 = (u32)sd.mc.mneg;
                 S.sideI = sd.side == SIDE_I;
+                S.narrow = sd.mc.mask2 <= 0xFFFFFFFFull;
             }
         }
     } else if (OWN_ZEROED) {
@@ -281,6 +285,7 @@ __device__ __forceinline__ Tile tile_setup_pf(const LeafSrc &L, u64 t, TileLds &
                 S.minv = sd.mc.minv;
                 S.mneg32 = (u32)sd.mc.mneg;
                 S.sideI = sd.side == SIDE_I;
+                S.narrow = sd.mc.mask2 <= 0xFFFFFFFFull;
             }
         }
         lds_sync();   // every thread has read S.inst and S.fastok above
@@ -337,12 +342,15 @@ __device__ __forceinline__ void tile_lds_init(TileLds &S) {
 struct FastCtx {
     u64 m, mask2, koff;
     u32 mneg32;
-    bool sideI;
+    bool sideI, narrow;
     __device__ __forceinline__ explicit FastCtx(const TileLds &S)
-        : m(S.m), mask2(S.mask2), koff(S.koff), mneg32(S.mneg32), sideI(S.sideI != 0) {}
+        : m(S.m), mask2(S.mask2), koff(S.koff), mneg32(S.mneg32), sideI(S.sideI != 0), narrow(S.narrow != 0) {}
 };
 
 // Fast tile: residue state of leaf j0 + i (any i < LT whose owner entry is valid) and its hash.
+// NARROW (2^a <= 2^32): only the low word of x2 is carried (x2 is exact mod 2^a, which is all that
+// the hash and the key read).
+template <bool NARROW = false>
 __device__ __forceinline__ u64 leaf_fast(const TileLds &S, const FastCtx &c, u32 i, u64 &xm, u64 &x2, int &x,
                                          u32 &q) {
     q = S.own[i];
@@ -350,7 +358,7 @@ __device__ __forceinline__ u64 leaf_fast(const TileLds &S, const FastCtx &c, u32
     x = c.sideI ? S.nx[q] + k : S.nx[q] - k;
     const u32 p = S.prime[x];
     xm = cf_redc32_mul(S.xm[q], p, c.m, c.mneg32);
-    x2 = S.x2[q] * (u64)p;
+    x2 = NARROW ? (u64)((u32)S.x2[q] * p) : S.x2[q] * (u64)p;
     return leaf_hash(xm, x2, c.mask2, c.koff);
 }
 
@@ -375,18 +383,29 @@ __device__ __forceinline__ u64 leaf_generic(const LeafSrc &L, const Tile &T, con
     return leaf_hash(xm, x2, r.s->mc.mask2, r.s->koff);
 }
 
-// Hashes of the LPT leaves of this lane (lanes past the tile's end repeat its last leaf).
+// Hashes (and with KEYS the exact keys) of the LPT leaves of this lane on a fast tile; FULL: the
+// tile holds LT leaves, else lanes past its end repeat its last leaf.
+template <bool NARROW, bool FULL, bool KEYS>
+__device__ __forceinline__ void fast_hashes(const Tile &T, const TileLds &S, const FastCtx &c, u64 (&h)[LPT],
+                                            u64 *key) {
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        const u32 i = threadIdx.x + LPB * k;
+        u64 xm, x2;
+        int x;
+        u32 q;
+        h[k] = leaf_fast<NARROW>(S, c, FULL || i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
+        if constexpr (KEYS) key[k] = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
+    }
+}
+
+// Hashes of the LPT leaves of this lane (FULL as in fast_hashes).
+template <bool FULL = false>
 __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT]) {
     if (T.fast) {
         const FastCtx c(S);
-#pragma unroll
-        for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + LPB * k;
-            u64 xm, x2;
-            int x;
-            u32 q;
-            h[k] = leaf_fast(S, c, i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
-        }
+        if (c.narrow) fast_hashes<true, FULL, false>(T, S, c, h, nullptr);
+        else fast_hashes<false, FULL, false>(T, S, c, h, nullptr);
     } else {
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + LPB * k;
@@ -398,19 +417,13 @@ __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, con
 }
 
 // Hashes and exact keys of the LPT leaves of this lane (the D side of the partitioned build).
+template <bool FULL = false>
 __device__ __forceinline__ void tile_hash_keys(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT],
                                                u64 (&key)[LPT]) {
     if (T.fast) {
         const FastCtx c(S);
-#pragma unroll
-        for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + LPB * k;
-            u64 xm, x2;
-            int x;
-            u32 q;
-            h[k] = leaf_fast(S, c, i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
-            key[k] = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
-        }
+        if (c.narrow) fast_hashes<true, FULL, true>(T, S, c, h, key);
+        else fast_hashes<false, FULL, true>(T, S, c, h, key);
     } else {
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + LPB * k;
@@ -535,8 +548,10 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 //
 // Per partition the block keeps its next slot gb[p] and the room left in its segment rm[p] (no
 // limit in the exact layout); per tile, one 64-bit word lc[p] = gb << 32 | min(room, 0xFFFF) << 16 |
-// the partition's offset in the tile gives a ranked record its staging place and its slot with one
-// LDS read.
+// the partition's offset in the tile. A ranked record is staged at its partition's offset plus its
+// rank, with its partition and place in the tile in the staged high word; the store side reads the
+// staged records in order (so the lanes of a wave read the words of only a few partitions) and
+// takes each record's slot from its partition's word.
 template <bool KEYS>
 __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                   const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
@@ -549,12 +564,9 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     __shared__ u32 s_ovf;
     __shared__ u64 s_k[KEYS ? LT : 1];
     // once a tile's leaves are hashed its parents' staged state is dead: the records are staged over
-    // xm and x2 (16 KB) and each record's slot over kb and nx (8 KB)
+    // xm and x2 (32 KB)
     uint2 *s_r = (uint2 *)S.xm;
-    u32 *s_dst = (u32 *)S.kb;
     static_assert(sizeof(uint2) * LT <= sizeof(u64) * 2 * NCAP, "staging fits over xm, x2");
-    static_assert(sizeof(u32) * LT <= sizeof(int) * 2 * NCAP && offsetof(TileLds, nx) == offsetof(TileLds, kb) +
-                  sizeof(int) * NCAP, "slots fit over kb, nx");
     const int P = 1 << kh.lgP;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const bool exact = O != nullptr;
@@ -580,87 +592,105 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     if (t0 + 1 < t1) tpv = tile_owner_load(L, t0 + 1);
     // the same wait for the first tile's parents (every entry to the loop finds them in registers)
     asm volatile("" ::"v"(pf.xm), "v"(pf.x2), "v"(pf.off), "v"(pf.pos), "v"(pf.meta), "v"(tpv));
-    for (u64 t = t0; t < t1; ++t) {
+    // one tile: its leaves hashed, ranked, staged and written; FULL: the tile holds LT leaves (every
+    // tile but the last of the chunk), so no lane needs a bound check. Full tiles and the last one
+    // run in loops of their own.
+    auto tile = [&](u64 t, auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         const Tile T = tile_setup_pf(L, t, S, pf);
-        // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
-        u32 lo[LPT], pt[LPT];
-        u64 kk[KEYS ? LPT : 1];
+        u32 v;
         {
-            u64 h[LPT];
-            if constexpr (KEYS) tile_hash_keys(L, T, S, h, kk);
-            else tile_hashes(L, T, S, h);
+            // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
+            u32 lo[LPT], pt[LPT];
+            u64 kk[KEYS ? LPT : 1];
+            {
+                u64 h[LPT];
+                if constexpr (KEYS) tile_hash_keys<FULL>(L, T, S, h, kk);
+                else tile_hashes<FULL>(L, T, S, h);
 #pragma unroll
-            for (int k = 0; k < LPT; ++k) {
-                const u32 hi = (u32)(h[k] >> 32);
-                pt[k] = hi >> (32 - kh.lgP);
-                lo[k] = (hoff32(hi, kh) << kh.fpb) | (hfp(h[k]) & fmask);
-            }
-        }
-        if (t + 1 < t1) {
-            tile_fetch(L, lane_u64(tpv, 0), lane_u64(tpv, 1), pf);
-            if (t + 2 < t1) tpv = tile_owner_load(L, t + 2);
-        }
-        u32 rank[LPT];
-#pragma unroll
-        for (int k = 0; k < LPT; ++k)
-            if (threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k]], 1u);
-        lds_sync();   // also: every lane is done with the staged parents and the owner map
-#pragma unroll
-        for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
-        // tile-local exclusive offsets of the partitions (block scan of hist)
-        const u32 v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
-        {
-            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-            u32 inc = v;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const u32 o = __shfl_up(inc, d, 64);
-                if (lane >= d) inc += o;
-            }
-            if (lane == 63) S.w[wv] = inc;
-            lds_sync();
-            u32 pre = 0;
-            for (int q = 0; q < wv; ++q) pre += S.w[q];
-            if (threadIdx.x < (unsigned)P) {
-                const u32 r = rm[threadIdx.x];
-                lc[threadIdx.x] = ((u64)gb[threadIdx.x] << 32) | ((r < 0xFFFFu ? r : 0xFFFFu) << 16) | (pre + inc - v);
-            }
-        }
-        lds_sync();
-        // staging: each record at its partition's run in the tile, its slot beside it
-        u32 over = 0;
-#pragma unroll
-        for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + LPB * k;
-            if (i < T.cnt) {
-                const u64 w = lc[pt[k]];
-                const u32 at = ((u32)w & 0xFFFFu) + rank[k];
-                const bool fits = rank[k] < ((u32)w >> 16);
-                over |= !fits;
-                s_r[at] = make_uint2(lo[k], (u32)(T.j0 + i - L.lo));
-                s_dst[at] = fits ? (u32)(w >> 32) + rank[k] : ~0u;
-                if constexpr (KEYS) s_k[at] = kk[k];
-            }
-        }
-        if (over) s_ovf = 1;
-        lds_sync();
-        // The next tile's prefetched parents and owner pair are waited for here, before this tile's
-        // stores: loads and stores share one counter that this target treats as completing out of
-        // order, so a wait for a load issued before pending stores waits for the stores as well.
-        // Waited for after them (at the next tile's set-up), every tile would wait for its own
-        // stores to reach memory; waited for here, the stores drain while the next tile is hashed.
-        asm volatile("" ::"v"(pf.xm), "v"(pf.x2), "v"(pf.off), "v"(pf.pos), "v"(pf.meta), "v"(tpv));
-#pragma unroll
-        for (int k = 0; k < LPT; ++k) {
-            const u32 q = threadIdx.x + LPB * k;
-            if (q < T.cnt) {
-                const u32 d = s_dst[q];
-                if (exact || d != ~0u) {
-                    rec[d] = s_r[q];
-                    if constexpr (KEYS) keys[d] = s_k[q];
+                for (int k = 0; k < LPT; ++k) {
+                    const u32 hi = (u32)(h[k] >> 32);
+                    pt[k] = hi >> (32 - kh.lgP);
+                    lo[k] = (hoff32(hi, kh) << kh.fpb) | (hfp(h[k]) & fmask);
                 }
             }
+            if (t + 1 < t1) {
+                tile_fetch(L, lane_u64(tpv, 0), lane_u64(tpv, 1), pf);
+                if (t + 2 < t1) tpv = tile_owner_load(L, t + 2);
+            }
+            u32 rank[LPT];
+#pragma unroll
+            for (int k = 0; k < LPT; ++k)
+                if (FULL || threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k]], 1u);
+            lds_sync();   // also: every lane is done with the staged parents and the owner map
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
+            // tile-local exclusive offsets of the partitions (block scan of hist)
+            v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
+            {
+                const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+                u32 inc = v;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const u32 o = __shfl_up(inc, d, 64);
+                    if (lane >= d) inc += o;
+                }
+                if (lane == 63) S.w[wv] = inc;
+                lds_sync();
+                u32 pre = 0;
+                for (int q = 0; q < wv; ++q) pre += S.w[q];
+                if (threadIdx.x < (unsigned)P) {
+                    const u32 r = rm[threadIdx.x];
+                    lc[threadIdx.x] = ((u64)gb[threadIdx.x] << 32) | ((r < 0xFFFFu ? r : 0xFFFFu) << 16) | (pre + inc - v);
+                }
+            }
+            lds_sync();
+            // staging: each record at its partition's run in the tile (offset: the low half of lc)
+            const u32 *lc32 = (const u32 *)lc;
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) {
+                const u32 i = threadIdx.x + LPB * k;
+                if (FULL || i < T.cnt) {
+                    const u32 at = (lc32[2 * pt[k]] & 0xFFFFu) + rank[k];
+                    s_r[at] = make_uint2(lo[k], (pt[k] << 16) | i);
+                    if constexpr (KEYS) s_k[at] = kk[k];
+                }
+            }
+            lds_sync();
+            // The next tile's prefetched parents and owner pair are waited for here, before this
+            // tile's stores: loads and stores share one counter that this target treats as
+            // completing out of order, so a wait for a load issued before pending stores waits for
+            // the stores as well. Waited for after them (at the next tile's set-up), every tile would
+            // wait for its own stores to reach memory; waited for here, the stores drain while the
+            // next tile is hashed.
+            asm volatile("" ::"v"(pf.xm), "v"(pf.x2), "v"(pf.off), "v"(pf.pos), "v"(pf.meta), "v"(tpv));
+            // every staged record and its partition's word read first, then the stores (staged
+            // places past the tile's end hold stale words: their partition is masked to stay in
+            // lc, and they are not stored)
+            const u32 jb = (u32)(T.j0 - L.lo);
+            uint2 r[LPT];
+            u64 w[LPT];
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) r[k] = s_r[threadIdx.x + LPB * k];
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) w[k] = lc[(r[k].y >> 16) & 0xFFu];
+            u32 over = 0;
+#pragma unroll
+            for (int k = 0; k < LPT; ++k) {
+                const u32 q = threadIdx.x + LPB * k;
+                if (FULL || q < T.cnt) {
+                    const u32 rk = q - ((u32)w[k] & 0xFFFFu);
+                    const bool fits = rk < ((u32)w[k] >> 16);
+                    over |= !fits;
+                    if (exact || fits) {
+                        const u32 d = (u32)(w[k] >> 32) + rk;
+                        rec[d] = make_uint2(r[k].x, jb + (r[k].y & 0xFFFFu));
+                        if constexpr (KEYS) keys[d] = s_k[q];
+                    }
+                }
+            }
+            if (over) s_ovf = 1;
         }
         if (threadIdx.x < (unsigned)P) {
             gb[threadIdx.x] += v;
@@ -668,7 +698,11 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             if (!exact) rm[threadIdx.x] = r > v ? r - v : 0u;
         }
         lds_sync();
-    }
+    };
+    const u64 tfull = t1 == ntiles && L.n % LT ? t1 - 1 : t1;   // tiles [t0, tfull) are full
+    u64 t = t0;
+    for (; t < tfull; ++t) tile(t, std::true_type{});
+    for (; t < t1; ++t) tile(t, std::false_type{});
     lds_sync();
     if (!exact) {
         if (threadIdx.x < (unsigned)P) fill[(u64)threadIdx.x * nblk + blockIdx.x] = cap - rm[threadIdx.x];
@@ -677,8 +711,9 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
 }
 
 // I side (records only) and D side (records and exact keys): separate kernels so that profiles tell
-// them apart.
-__global__ void __launch_bounds__(LPB) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+// them apart. The I side runs two blocks per CU (four waves per SIMD: at most 128 VGPRs); the D side's
+// key staging (32 KB more LDS) leaves it one.
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec) {
     part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);

@@ -220,18 +220,13 @@ __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__re
             run = run > v[k] ? run : v[k];
         }
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        u32 inc = run;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const u32 o = __shfl_up(inc, d, 64);
-            if (lane >= d) inc = inc > o ? inc : o;
-        }
+        const u32 inc = wave_incl_max(run);
         if (lane == 63) s_w[wv] = inc;
         __syncthreads();
         u32 cur = 0;
         for (int q = 0; q < wv; ++q) cur = cur > s_w[q] ? cur : s_w[q];
-        const u32 ex = __shfl_up(inc, 1, 64);
-        if (lane) cur = cur > ex ? cur : ex;
+        const u32 ex = wave_shift_up1(inc);
+        cur = cur > ex ? cur : ex;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             cur = cur > v[k] ? cur : v[k];

@@ -181,18 +181,12 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
         run = run > v[k] ? run : v[k];
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    u32 inc = run;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u32 o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc = inc > o ? inc : o;
-    }
+    const u32 inc = wave_incl_max(run);
     if (lane == 63) S.w[wv] = inc;
     lds_sync();
     u32 pre = 0;
     for (int q = 0; q < wv; ++q) pre = pre > S.w[q] ? pre : S.w[q];
-    u32 ex = __shfl_up(inc, 1, 64);
-    if (lane == 0) ex = 0;
+    const u32 ex = wave_shift_up1(inc);
     u32 cur = pre > ex ? pre : ex;
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
@@ -306,18 +300,12 @@ __device__ __forceinline__ Tile tile_setup_pf(const LeafSrc &L, u64 t, TileLds &
         run = run > v[k] ? run : v[k];
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    u32 inc = run;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u32 o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc = inc > o ? inc : o;
-    }
+    const u32 inc = wave_incl_max(run);
     if (lane == 63) S.w[wv] = inc;
     lds_sync();
     u32 pre = 0;
     for (int q = 0; q < wv; ++q) pre = pre > S.w[q] ? pre : S.w[q];
-    u32 ex = __shfl_up(inc, 1, 64);
-    if (lane == 0) ex = 0;
+    const u32 ex = wave_shift_up1(inc);
     u32 cur = pre > ex ? pre : ex;
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
@@ -630,12 +618,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
             {
                 const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-                u32 inc = v;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const u32 o = __shfl_up(inc, d, 64);
-                    if (lane >= d) inc += o;
-                }
+                const u32 inc = wave_incl_sum(v);
                 if (lane == 63) S.w[wv] = inc;
                 lds_sync();
                 u32 pre = 0;
@@ -969,12 +952,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                 hmask &= look;
                 if (__any(hmask != 0)) {
                     const u32 c = (u32)__popc(hmask);
-                    u32 inc = c;
-#pragma unroll
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const u32 o = __shfl_up(inc, d, 64);
-                        if (lane >= d) inc += o;
-                    }
+                    const u32 inc = wave_incl_sum(c);
                     const u32 total = (u32)__builtin_amdgcn_readlane((int)inc, 63);
                     if (total > cleft) {
                         // the rest of the current reservation is marked unused; a new one is taken
@@ -1067,12 +1045,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
         // tile offsets of the windows (block scan of hist), and one reservation per window present
         const u32 v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
         {
-            u32 inc = v;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const u32 o = __shfl_up(inc, d, 64);
-                if (lane >= d) inc += o;
-            }
+            const u32 inc = wave_incl_sum(v);
             if (lane == 63) s_w[wv] = inc;
             lds_sync();
             u32 pre = 0;
@@ -1166,12 +1139,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         cv[k] = b0 + k < nb ? cnt[b0 + k] : 0u;
         sum += cv[k];
     }
-    u32 inc = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const u32 o = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += o;
-    }
+    const u32 inc = wave_incl_sum(sum);
     if (lane == 63) s_w[wv] = inc;
     lds_sync();
     u32 run = inc - sum;

@@ -5,7 +5,7 @@
 //   ->  scatter, where each tile ranks its keys stably with 64-lane ballots (8 ballots give the
 //   lanes holding the same digit), stages them digit-grouped in LDS, and writes contiguous runs.
 // Stability matches the stable argsort of gpu/mitm_gpu2.py:101-104.
-#include "cf_internal.h"
+#include "tree_dev.h"
 
 namespace {
 
@@ -19,15 +19,7 @@ constexpr int RI = 8;           // items per thread
 constexpr int RT = RB * RI;     // 2048-key tiles
 constexpr int RADIX = 256;
 
-__device__ __forceinline__ u64 wave_incl_scan(u64 v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        u64 o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
-    return v;
-}
+__device__ __forceinline__ u64 wave_incl_scan(u64 v) { return wave_incl_sum64(v); }
 
 // Block-wide exclusive scan of one value per thread (SB threads = 4 waves). Returns the
 // exclusive prefix; *total gets the block sum.

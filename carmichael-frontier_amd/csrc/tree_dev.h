@@ -57,5 +57,19 @@ __device__ __forceinline__ u32 wave_incl_max(u32 v) {
     v = umax32(v, dpp_or_zero<0x143, 0xC>(v));
     return v;
 }
+// The 64-bit sum: each step moves both halves with the same pattern and adds them as one value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ u64 dpp64_or_zero(u64 v) {
+    return ((u64)dpp_or_zero<CTRL, ROWS>((u32)(v >> 32)) << 32) | dpp_or_zero<CTRL, ROWS>((u32)v);
+}
+__device__ __forceinline__ u64 wave_incl_sum64(u64 v) {
+    v += dpp64_or_zero<0x111, 0xF>(v);
+    v += dpp64_or_zero<0x112, 0xF>(v);
+    v += dpp64_or_zero<0x114, 0xF>(v);
+    v += dpp64_or_zero<0x118, 0xF>(v);
+    v += dpp64_or_zero<0x142, 0xA>(v);
+    v += dpp64_or_zero<0x143, 0xC>(v);
+    return v;
+}
 // The value of the lane below (0 in lane 0): wave_shr:1.
 __device__ __forceinline__ u32 wave_shift_up1(u32 v) { return dpp_or_zero<0x138, 0xF>(v); }

@@ -325,6 +325,31 @@ def test_solve_primes_above_32_bits():
     assert np.array_equal(both[1].pair_array(), pipeline.solve(small).pair_array())
 
 
+@pytest.mark.parametrize("a", [31, 32, 33, 40, 62])
+def test_solve_two_adic_part_around_32_bits(a):
+    """M = 3 * 2^a: the leaf kernels carry x2 mod 2^a in 32 bits when 2^a <= 2^32 and in 64 bits
+    above; P0 is chosen so that one (D, I) pair must meet (P0 = prod D0 / prod I0 mod M), and both
+    paths, eager and batched, must return the oracle's pair set."""
+    P = [p for p in range(5, 2000) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
+    B0, INS = P[:20], P[20:80]
+    M = 3 << a
+    D0, I0 = B0[3:6], INS[10:13]
+    P0 = prod(D0) * pow(prod(I0), -1, M) % M
+    inst = I.Instance(M=M, r=3, B0=B0, INS=INS, P0=P0, Icap=prod(INS[-3:]) + 1, Dmin=1, split=None, hmax=3,
+                      tag=f"twoadic{a}")
+    say("two-adic", a)
+    Dk, Did, Ik, Iid = O.instance_records(inst)
+    o = np.argsort(Dk, kind="stable")
+    q = np.argsort(Ik, kind="stable")
+    ed, ei = O.join_sorted(Dk[o], Did[o], Ik[q], Iid[q])
+    assert len(ed) >= 1
+    res = pipeline.solve(inst)
+    assert (res.nD, res.nI) == (len(Dk), len(Ik))
+    assert np.array_equal(res.pair_array(), O.pair_set(ed, ei))
+    both, _ = pipeline.solve_batch([inst, I.c1_instance(0, 100)])
+    assert np.array_equal(both[0].pair_array(), res.pair_array())
+
+
 def test_batch_graph_replays_match_eager(workloads):
     """The captured batch (one HIP graph launch per solve) returns, on every replay, the pair sets and
     record counts of the eager solve -- with eager work and synchronizations between the replays.

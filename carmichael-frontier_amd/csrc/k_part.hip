@@ -918,8 +918,10 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                 for (int e = 0; e < NB2; ++e) {
                     const int sj = e / PROBE_BATCH, q = e % PROBE_BATCH;
                     const u32 v = v0 + (u32)q * 64 + lane;
-                    // past the fill the descriptor returns 0: bucket 0 of the partition, masked below
-                    lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 0);
+                    // past the fill the descriptor returns 0: bucket 0 of the partition, masked below;
+                    // non-temporal (cache policy 2): the records stream through L2 once and should not
+                    // evict the partition's table slice
+                    lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 2);
                     valid |= (u32)(v < fills[sj]) << e;
                 }
                 u32 look = valid;

@@ -21,8 +21,8 @@
 //  D side: records counted per bucket and scattered into bucket order (DRec = exact key, id); the
 //          scatter writes the table slots as it goes; F is built from T.
 //  I side: leaves scattered by partition as 8-byte records (low word: bucket offset << fpb | f,
-//          high word: leaf index in the chunk): each block ranks its 2048 leaves by partition in LDS
-//          and writes one run per partition into slots sized from the expected share.
+//          high word: leaf index in the chunk): each block ranks a tile of 4096 leaves by partition
+//          in LDS and writes one run per partition into slots sized from the expected share.
 //  Probe:  pass 1 reads the records in partition order: each of the 8 XCDs takes tiles of its own
 //          contiguous slice from a queue, so an XCD works on one partition at a time and that
 //          partition's table slice stays in its L2. Pass 2 recomputes candidates exactly.
@@ -38,8 +38,6 @@ constexpr int PB = 256;            // threads per block (other kernels)
 constexpr int LPB = 512;           // threads per block of the leaf kernels
 constexpr int LPT = 8;             // leaves per thread in the leaf kernels
 constexpr int LT = LPB * LPT;      // 4096 leaves per tile
-constexpr int PI = 16;             // records per thread in the probe
-constexpr int PT = PB * PI;        // 4096 records per probe tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
 
@@ -706,11 +704,6 @@ __global__ void __launch_bounds__(LPB) k_part_scatter_keys(LeafSrc L, KeyHash kh
                                                           u32 *__restrict__ ovf, uint2 *__restrict__ rec,
                                                           u64 *__restrict__ keys) {
     part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, keys);
-}
-
-// pbase[p] = O[p * nblk] for p <= P (O[P * nblk] is the total): where each partition starts
-__global__ void __launch_bounds__(PB) k_part_bases(const u64 *__restrict__ O, u32 nblk, int P, u64 *__restrict__ pbase) {
-    for (int p = threadIdx.x; p <= P; p += PB) pbase[p] = O[(u64)p * nblk];
 }
 
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).

@@ -532,8 +532,9 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // Slots are 32-bit: a chunk holds at most 2^32 records, and the caller takes the capacity layout
 // only when it has fewer than 2^32 slots (~0 marks a record past its segment's end).
 //
-// Per partition the block keeps its next slot gb[p] and the room left in its segment rm[p] (no
-// limit in the exact layout); per tile, one 64-bit word lc[p] = gb << 32 | min(room, 0xFFFF) << 16 |
+// Per partition the block keeps its next slot gb and the room left in its segment rm (no limit in
+// the exact layout), in registers of wave 0 (lane l: partitions 4l .. 4l + 3); per tile, wave 0
+// scans the partition counts and writes one 64-bit word lc[p] = gb << 32 | min(room, 0xFFFF) << 16 |
 // the partition's offset in the tile. A ranked record is staged at its partition's offset plus its
 // rank, with its partition and place in the tile in the staged high word; the store side reads the
 // staged records in order (so the lanes of a wave read the words of only a few partitions) and
@@ -545,7 +546,6 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                                                   u64 *__restrict__ keys) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
-    __shared__ u32 gb[256], rm[256];
     __shared__ u64 lc[256];
     __shared__ u32 s_ovf;
     __shared__ u64 s_k[KEYS ? LT : 1];
@@ -557,10 +557,15 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     const u32 fmask = (1u << kh.fpb) - 1u;
     const bool exact = O != nullptr;
     const u32 cap = 1u << capl;
-    if (threadIdx.x < (unsigned)P) {
-        const u64 g = (u64)threadIdx.x * nblk + blockIdx.x;
-        gb[threadIdx.x] = exact ? (u32)O[g] : (u32)(g << capl);
-        rm[threadIdx.x] = exact ? 0xFFFFu : cap;
+    constexpr int PPL = 256 / 64;   // partitions per lane of wave 0
+    u32 gb[PPL], rm[PPL], hv[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        const u32 p = threadIdx.x * PPL + j;
+        const u64 g = (u64)p * nblk + blockIdx.x;
+        gb[j] = threadIdx.x < 64 && p < (u32)P ? (exact ? (u32)O[g] : (u32)(g << capl)) : 0u;
+        rm[j] = exact ? 0xFFFFu : cap;
+        hv[j] = 0;
     }
     if (threadIdx.x == 0) s_ovf = 0;
     tile_lds_init(S);
@@ -585,7 +590,6 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         constexpr bool FULL = decltype(full_tag)::value;
         if (threadIdx.x < 256) hist[threadIdx.x] = 0;
         const Tile T = tile_setup_pf(L, t, S, pf);
-        u32 v;
         {
             // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
             u32 lo[LPT], pt[LPT];
@@ -612,18 +616,21 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             lds_sync();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
             for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
-            // tile-local exclusive offsets of the partitions (block scan of hist)
-            v = threadIdx.x < 256 ? hist[threadIdx.x] : 0u;
-            {
-                const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-                const u32 inc = wave_incl_sum(v);
-                if (lane == 63) S.w[wv] = inc;
-                lds_sync();
-                u32 pre = 0;
-                for (int q = 0; q < wv; ++q) pre += S.w[q];
-                if (threadIdx.x < (unsigned)P) {
-                    const u32 r = rm[threadIdx.x];
-                    lc[threadIdx.x] = ((u64)gb[threadIdx.x] << 32) | ((r < 0xFFFFu ? r : 0xFFFFu) << 16) | (pre + inc - v);
+            // tile-local exclusive offsets of the partitions: wave 0 scans the counts, PPL per lane
+            if (threadIdx.x < 64) {
+                u32 sum = 0;
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) {
+                    hv[j] = hist[threadIdx.x * PPL + j];
+                    sum += hv[j];
+                }
+                u32 off = wave_incl_sum(sum) - sum;
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) {
+                    const u32 p = threadIdx.x * PPL + j;
+                    if (p < (u32)P)
+                        lc[p] = ((u64)gb[j] << 32) | ((rm[j] < 0xFFFFu ? rm[j] : 0xFFFFu) << 16) | off;
+                    off += hv[j];
                 }
             }
             lds_sync();
@@ -673,10 +680,12 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             }
             if (over) s_ovf = 1;
         }
-        if (threadIdx.x < (unsigned)P) {
-            gb[threadIdx.x] += v;
-            const u32 r = rm[threadIdx.x];
-            if (!exact) rm[threadIdx.x] = r > v ? r - v : 0u;
+        if (threadIdx.x < 64) {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                gb[j] += hv[j];
+                if (!exact) rm[j] = rm[j] > hv[j] ? rm[j] - hv[j] : 0u;
+            }
         }
         lds_sync();
     };
@@ -686,7 +695,13 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     for (; t < t1; ++t) tile(t, std::false_type{});
     lds_sync();
     if (!exact) {
-        if (threadIdx.x < (unsigned)P) fill[(u64)threadIdx.x * nblk + blockIdx.x] = cap - rm[threadIdx.x];
+        if (threadIdx.x < 64) {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const u32 p = threadIdx.x * PPL + j;
+                if (p < (u32)P) fill[(u64)p * nblk + blockIdx.x] = cap - rm[j];
+            }
+        }
         if (threadIdx.x == 0 && s_ovf) atomicOr(ovf, 1u);
     }
 }

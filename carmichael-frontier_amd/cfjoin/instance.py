@@ -227,3 +227,49 @@ def c2_instance(seed: int, s: int, r: int = 6, k: int = 64) -> Optional[Instance
     B0 = Q[:k]
     INS = Q[k:]
     return build_instance(M, B0, INS, r, s * prod(B0), split=None, hmax=r, tag=f"C2_seed{seed}_s{s}")
+
+
+# ---- positive controls: instances that must contain a known k = 64 Carmichael number ---------------
+CONTROLS = {
+    "N": "results_k64_best_global.json",                           # lambda(N) = 155016423578016000, r = 6
+    "N_prev": "results_k64_best_global_prev_148_N.json",           # M0 = 1768248177696000, r = 7
+    "N0": "results_k64_best_global_prev_148_539839443357.json",    # M0, r = 8
+    "N149": "results_k64.json",                                    # 353649635539200, r = 9
+}
+
+
+@dataclass
+class Control:
+    """A positive control: the instance, the incumbent n it must reproduce and the pair that gives it."""
+    name: str
+    N: int
+    factors: List[int]
+    D_need: List[int]             # base primes n leaves out
+    I_need: List[int]             # pool primes n brings in
+    inst: Instance
+
+
+def control_instance(name: str, nins: Optional[int] = None, pool_below: int = 4000) -> Control:
+    """The incumbent's own modulus with the ksmall base (the 64 smallest pool primes below
+    pool_below) and an insertion pool holding the incumbent's inserted primes, so that the join
+    must emit the pair (D_need, I_need) whose completion is the incumbent
+    (gpu/validate_mitm_gpu2_pos.py:20-36). nins: the pool is I_need plus the first nins - r other
+    pool primes, as the validator builds it; None: every pool prime outside the base. The bound is
+    U = n + 1, so the instance's product bounds admit n itself; no stratum split."""
+    inc = load_workloads()["incumbents"][CONTROLS[name]]
+    M, N = int(inc["modulus"]), int(inc["n"])
+    fac = sorted(int(p) for p in inc["factors"])
+    P = pool(M, pool_below - 1)
+    B0 = P[:K]
+    in_base = set(B0)
+    D_need = sorted(in_base - set(fac))
+    I_need = sorted(set(fac) - in_base)
+    r = len(D_need)
+    assert r == len(I_need) and set(I_need) <= set(P), "the incumbent is not reachable from the ksmall base"
+    rest = [p for p in P if p not in in_base and p not in set(I_need)]
+    if nins is not None:
+        rest = rest[: max(nins - r, 0)]
+    inst = build_instance(M, B0, sorted(I_need + rest), r, N + 1, split=None, hmax=r,
+                          tag=f"control_{name}_r{r}" + ("" if nins is None else f"_ins{nins}"))
+    assert inst is not None and set(I_need) <= set(inst.INS)
+    return Control(name, N, fac, D_need, I_need, inst)

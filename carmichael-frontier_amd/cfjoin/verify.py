@@ -1,59 +1,77 @@
 """Host re-check of join hits: exact completion and Korselt's criterion (exact big-int).
 
-Restates the certificate check of ref/ref_carmichael.py:30-72 (deterministic Miller-Rabin with
-the first twelve primes as witnesses, then Korselt: n squarefree with >= 3 prime factors and
-(p - 1) | (n - 1) for every p | n) and the per-pair completion of
-gpu/mitm_gpu_mstar_driver.py:226-237 (S = (B0 minus D) union I, n = P0 * prod(I) / prod(D)).
+The certificate check answers the question the reference's ref/ref_carmichael.py:55-72 answers,
+with the same result tuple: is n = prod(primes) a Carmichael number with exactly len(primes)
+distinct prime factors? It is written from the textbook statements:
+  * Korselt (1899): a composite n is Carmichael iff n is squarefree and (p - 1) | (n - 1) for
+    every prime p | n; with at least three distinct prime factors n is composite and squarefree
+    by construction, so only the divisibility conditions remain;
+  * the strong-probable-prime (Miller-Rabin) test: write n - 1 = d * 2^s with d odd; n passes base
+    a if a^d = 1 or a^(d 2^j) = -1 (mod n) for some 0 <= j < s. Deterministic base sets are known:
+    Sinclair's seven bases for n < 2^64, and the first twelve primes for n < 3.3e24 (Sorenson and
+    Webster, 2015); larger n get both sets, a probabilistic answer as in the reference.
+The per-pair completion follows gpu/mitm_gpu_mstar_driver.py:226-237 (S = (B0 minus D) union I,
+n = P0 * prod(I) / prod(D)).
 """
 from __future__ import annotations
 
 from math import comb, prod
 from typing import Iterable, List, Sequence, Tuple
 
-_WITNESSES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)
+_SINCLAIR_BASES = (2, 325, 9375, 28178, 450775, 9780504, 1795265022)   # every n < 2^64
+_PRIME_BASES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37)             # every n < 3.317e24
+_SMALL = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47)
+
+
+def _strong_probable_prime(n: int, a: int, d: int, s: int) -> bool:
+    """n (odd, > 2) passes the strong test to base a, with n - 1 = d * 2^s, d odd."""
+    a %= n
+    if a == 0:          # a base that is a multiple of n says nothing
+        return True
+    x = pow(a, d, n)
+    if x == 1 or x == n - 1:
+        return True
+    for _ in range(s - 1):
+        x = x * x % n
+        if x == n - 1:
+            return True
+    return False
 
 
 def is_prime(n: int) -> bool:
-    """Strong-probable-prime test to the twelve smallest prime bases; deterministic below
-    3.3e24 (ref/ref_carmichael.py:30-52)."""
+    """Primality of n by trial division by the primes below 50, then strong tests to a base set
+    that is deterministic below 3.3e24."""
     if n < 2:
         return False
-    for p in _WITNESSES:
+    for p in _SMALL:
         if n % p == 0:
             return n == p
-    d, s = n - 1, 0
-    while d % 2 == 0:
-        d //= 2
-        s += 1
-    for a in _WITNESSES:
-        x = pow(a, d, n)
-        if x in (1, n - 1):
-            continue
-        for _ in range(s - 1):
-            x = x * x % n
-            if x == n - 1:
-                break
-        else:
-            return False
-    return True
+    if n < 53 * 53:
+        return True
+    s = ((n - 1) & -(n - 1)).bit_length() - 1      # the power of two in n - 1
+    d = (n - 1) >> s
+    bases = _SINCLAIR_BASES if n < (1 << 64) else _PRIME_BASES + _SINCLAIR_BASES
+    return all(_strong_probable_prime(n, a, d, s) for a in bases)
 
 
 def verify_certificate(primes: Sequence[int]) -> Tuple[bool, int, str]:
     """(ok, n, reason) for the claim that n = prod(primes) is Carmichael with exactly
-    len(primes) prime factors (ref/ref_carmichael.py:55-72)."""
-    primes = list(primes)
-    if len(primes) < 3:
-        return (False, 0, "a Carmichael number has at least 3 prime factors")
-    if len(set(primes)) != len(primes):
-        return (False, 0, "factors are not distinct (n must be squarefree)")
-    for p in primes:
-        if not is_prime(p):
-            return (False, 0, f"claimed factor {p} is not prime")
-    n = prod(primes)
-    for p in primes:
+    len(primes) distinct prime factors. n is reported as 0 when the factor list itself is
+    invalid (too short, repeated or non-prime entries), as the reference does."""
+    ps = [int(p) for p in primes]
+    k = len(ps)
+    if k < 3:
+        return (False, 0, f"only {k} factors: Korselt needs at least three distinct primes")
+    if len(set(ps)) < k:
+        return (False, 0, "a factor is repeated, so n is not squarefree")
+    composite = [p for p in ps if not is_prime(p)]
+    if composite:
+        return (False, 0, f"factor {composite[0]} is not prime")
+    n = prod(ps)
+    for p in ps:
         if (n - 1) % (p - 1):
-            return (False, n, f"Korselt fails: (p-1)={p - 1} does not divide n-1 for p={p}")
-    return (True, n, "Carmichael with exactly %d prime factors" % len(primes))
+            return (False, n, f"p = {p}: p - 1 does not divide n - 1")
+    return (True, n, f"Carmichael, {k} distinct prime factors")
 
 
 def decode_packed(v: int, r: int, bits: int) -> Tuple[int, ...]:

@@ -297,7 +297,10 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
 // records of 16 bytes; *ovf set if a window overflowed); after the caller's exclusive scan of wcount
 // into wbase (P * R + 1 entries), part_d_sort writes bstart (NB + 1), DR in bucket order
 // ({key, D leaf}), the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket);
-// kerr is set if a window holds a record outside its buckets (inconsistent input)
+// kerr gets KERR_DWINDOW if a window holds a record outside its buckets (inconsistent input; the
+// window's buckets are then written empty)
+constexpr u32 KERR_DWINDOW = 1;   // a D record outside its window's buckets
+constexpr u32 KERR_CAND = 2;      // more candidates reserved than the candidate buffer holds
 bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R);
 u64 dwin_capacity();
 void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
@@ -329,7 +332,7 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
-                      u64 cap, u64 *od, u64 *oi, u64 *oinst);
+                      u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr);
 // per-instance leaf totals of a frontier added into out[0, ninst) (nodes of an instance are contiguous)
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
 

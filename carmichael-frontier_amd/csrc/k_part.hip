@@ -1136,7 +1136,14 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         }
     }
     if (__syncthreads_or(bad)) {
-        if (threadIdx.x == 0) atomicOr(kerr, 1u);
+        // the window's buckets are written empty (no table slot, no filter bit, every bucket
+        // starting at the window's base), so no later read follows an unwritten start or record
+        for (u32 i = threadIdx.x; i < nb; i += DSB) {
+            bstart[gb + i] = base;
+            *(uint4 *)(T + (gb + i) * TSLOTS) = make_uint4(0u, 0u, 0u, 0u);
+        }
+        if (pw + 1 == gridDim.x && threadIdx.x == 0) bstart[gb + nb] = base + n;
+        if (threadIdx.x == 0) atomicOr(kerr, KERR_DWINDOW);
         return;
     }
     lds_sync();
@@ -1228,8 +1235,10 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
                                                          const DRec *__restrict__ DR,
                                                          unsigned long long *__restrict__ npairs, u64 cap,
                                                          u64 *__restrict__ out_d, u64 *__restrict__ out_i,
-                                                         u64 *__restrict__ out_inst) {
+                                                         u64 *__restrict__ out_inst, u32 *__restrict__ kerr) {
     const u64 nc = *ncand < ccap ? *ncand : ccap;
+    // candidates past the buffer were never stored: report it rather than lose their pairs
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *ncand > ccap) atomicOr(kerr, KERR_CAND);
     for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
         const u64 raw = cand[c];
         if (raw == CAND_NONE) continue;
@@ -1490,12 +1499,12 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
-                      u64 cap, u64 *od, u64 *oi, u64 *oinst) {
+                      u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
     LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, 0, dl.n, depth, dl.tp};
     CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const uint2 *)rec, S,
               ncand, ccap,
-              cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst);
+              cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst, kerr);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <condition_variable>
 #include <deque>
+#include <set>
 #include <thread>
 
 #include "cf_internal.h"
@@ -56,6 +57,7 @@ struct cf_server {
     std::deque<Request> pending;
     std::deque<Served> done;
     std::map<u64, Served> held;        // results handed out, arrays alive until cf_server_release
+    std::set<u64> tags;                // tags submitted and not yet released (unique)
     u64 in_use = 0;                    // priced footprint of the batches on the device
     int busy = 0;                      // batches on the device
     int outstanding = 0;               // submitted and not yet handed out
@@ -209,6 +211,10 @@ int cf_server_submit(cf_server *s, const cf_instance *inst, uint64_t tag) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         CF_REQUIRE(!s->stop, "the server is stopping");
+        // a held result is found by its tag: a second request under a tag in use would replace it
+        // and free the arrays its caller still reads
+        CF_REQUIRE(!s->tags.count(tag), "tag already in use by a request that was not released");
+        s->tags.insert(tag);
         q.t_submit = now_us(s->t0);
         s->pending.push_back(std::move(q));
         s->pending.back().inst.B = s->pending.back().B.data();
@@ -251,7 +257,13 @@ int cf_server_release(cf_server *s, uint64_t tag) {
     API_BEGIN
     CF_REQUIRE(s, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    s->held.erase(tag);
+    if (s->held.erase(tag)) {
+        s->tags.erase(tag);
+    } else {
+        // releasing a tag whose result was not handed out yet would let the tag be reused while
+        // its request is still queued or on the device
+        CF_REQUIRE(!s->tags.count(tag), "the tag's result has not been handed out by cf_server_next");
+    }
     API_END
 }
 

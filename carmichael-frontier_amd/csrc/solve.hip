@@ -494,16 +494,23 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                                          cand.p, b->tune);
                         nc = host_value(ctx, tape, flags.p);
                     }
-                    if (nc > ccap) {
-                        // more candidates than room (heavily repeated keys): probe again with room for all
-                        ccap = nc + cand_margin(b->tune);
+                    while (nc > ccap) {
+                        // more candidates than room (heavily repeated keys): probe again with room for
+                        // all. The count includes the unused tails of partial reservations, and which
+                        // wave takes which tile varies from run to run. A wave leaves unused fewer slots
+                        // than the candidates of the round that makes it reserve again, plus at most one
+                        // partial reservation at its end, so a run needs at most 2 * (candidates) +
+                        // cand_margin slots, and nc >= candidates: one rerun suffices; the loop guards it
+                        ccap = 2 * nc + cand_margin(b->tune);
                         cand = DBuf<u64>(ctx, ccap);
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
                                          cand.p, b->tune);
+                        nc = host_value(ctx, tape, flags.p);
                     }
                     part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
                                      tp.p, dleaves, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
-                                     DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p);
+                                     DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p,
+                                     kerr.p);
                 }
             }
             if (!capturing) {
@@ -514,8 +521,9 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         }
         for (hipEvent_t e : ev_done) CF_HIP(hipStreamWaitEvent(ctx->stream, e, 0));
         const u64 np = host_value(ctx, tape, cnt_pairs.p);
-        if (host_value_u32(ctx, tape, kerr.p))
-            throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        const u64 ke = host_value_u32(ctx, tape, kerr.p);
+        if (ke & KERR_DWINDOW) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        if (ke & KERR_CAND) throw CfError{CF_EINTERNAL, "the probe's candidate list was truncated"};
         out.npairs = np;
         if (np <= cap) break;
         cap = np;

@@ -21,47 +21,20 @@ first index comes from the same expansion.
 import os
 import sys
 
-import numpy as np
-
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 from cfjoin import lib as _lib  # noqa: E402
 from cfjoin import xp as cp  # noqa: E402
 
 
-def _binom_table(n, r):
-    B = np.zeros((n + 1, r + 1), dtype=object)
-    B[:, 0] = 1
-    for a in range(1, n + 1):
-        for b in range(1, min(a, r) + 1):
-            B[a, b] = B[a - 1, b - 1] + B[a - 1, b]
-    assert B[n, r] < 2 ** 64, "rank ids would overflow u64"
-    return B.astype(np.uint64)
-
-
-def _u128_pairs(vals):
-    a = np.zeros(2 * len(vals), dtype=np.uint64)
-    mask = (1 << 64) - 1
-    for i, v in enumerate(vals):
-        a[2 * i] = v & mask
-        a[2 * i + 1] = v >> 64
-    return a
-
-
 def quotient_table(INS, r, Icap):
-    """H[t][x] = floor((Icap-1) / (a_x ... a_{x+t})) for 0 <= t <= r-1 and x + t < n (0 otherwise); exact."""
+    """H[t][x] = floor((Icap-1) / (a_x ... a_{x+t})) for 0 <= t <= r-1 and x + t < n (0 otherwise); exact.
+    Window products come from exact prefix products: a_x ... a_{x+t} = pre[x+t+1] / pre[x]."""
     n = len(INS)
+    pre = [1]
+    for a in INS:
+        pre.append(pre[-1] * int(a))
     top = Icap - 1
-    H = [0] * (r * n)
-    for t in range(r):
-        w = 1
-        for x in range(t + 1):
-            w *= INS[x]
-        for x in range(n - t):
-            if x > 0:
-                w = (w // INS[x - 1]) * INS[x + t]
-            H[t * n + x] = top // w
-    return H
-
+    return [top // (pre[x + t + 1] // pre[x]) if x + t < n else 0 for t in range(r) for x in range(n)]
 
 def _prepare(INS, r, M, P0, Icap, split, hmax):
     n = len(INS)
@@ -74,11 +47,9 @@ def _prepare(INS, r, M, P0, Icap, split, hmax):
     plan = _lib.Plan.insertion(INS, r, M, P0 % M, Icap, split_idx, hmax, _lib.CF_ID_COLEX)
     return n, plan
 
-
 def comb_fits(n, r):
     from math import comb
     return comb(n, r) < 2 ** 64
-
 
 def generate_chunks(INS, r, M, P0, Icap, split=None, hmax=None, max_records=200_000_000, block_bytes=512 << 20):
     """Yield (keys, ids) device uint64 chunks covering exactly the filtered r-subsets of INS; the
@@ -114,7 +85,6 @@ def generate_chunks(INS, r, M, P0, Icap, split=None, hmax=None, max_records=200_
         return total
     finally:
         plan.close()
-
 
 def generate(INS, r, M, P0, Icap, split=None, hmax=None):
     """Single-shot convenience: concatenated (keys, ids, count)."""

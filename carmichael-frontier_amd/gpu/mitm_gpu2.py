@@ -22,16 +22,6 @@ from cfjoin import lib as _lib  # noqa: E402
 from cfjoin import xp as cp  # noqa: E402
 
 
-def _binom_table(n: int, r: int) -> np.ndarray:
-    """Pascal table C(a, b), a <= n, b <= r, as uint64 (gpu/mitm_gpu2.py:70-76)."""
-    B = np.zeros((n + 1, r + 1), dtype=np.uint64)
-    B[:, 0] = 1
-    for a in range(1, n + 1):
-        for b in range(1, min(a, r) + 1):
-            B[a, b] = B[a - 1, b - 1] + B[a - 1, b]
-    return B
-
-
 def generate(primes, r: int, M: int, mult: int = 1, bits: int = 10, lo: int = 0, hi=None):
     """Residue records for the r-combinations of `primes` with CNS rank in [lo, hi).
     Returns device uint64 (keys, ids); ids pack r indices of `bits` bits each."""
@@ -80,8 +70,8 @@ def join_sorted(Dk, Did, Ik, Iid):
 
 
 def decode_ids(ids, r: int, bits: int):
-    """Unpack ids -> list of index tuples (host)."""
-    if isinstance(ids, cp.ndarray):
-        ids = ids.get()
-    mask = (1 << bits) - 1
-    return [tuple((int(v) >> (p * bits)) & mask for p in range(r)) for v in ids]
+    """Unpack ids -> list of index tuples (host): field p of an id is bits [p*bits, (p+1)*bits)."""
+    host = np.asarray(ids.get() if isinstance(ids, cp.ndarray) else ids, dtype=np.uint64).reshape(-1)
+    shifts = np.arange(r, dtype=np.uint64) * np.uint64(bits)
+    fields = (host[:, None] >> shifts[None, :]) & np.uint64((1 << bits) - 1)
+    return [tuple(int(x) for x in row) for row in fields]

@@ -222,10 +222,13 @@ typedef struct cf_served {
     char error[160];
 } cf_served;
 int cf_server_create(int device, int workers, int max_batch, uint64_t mem_budget, cf_server **out);
-/* the instance (and its prime lists) is copied; tags must be unique among unreleased results */
+/* the instance (and its prime lists) is copied; a tag stays in use from its submission until
+ * cf_server_release, and submitting a tag in use fails with CF_EINVAL */
 int cf_server_submit(cf_server *server, const cf_instance *inst, uint64_t tag);
 /* 1: *out holds the next finished request; 0: none finished within timeout_ms */
 int cf_server_next(cf_server *server, int timeout_ms, cf_served *out);
+/* frees the result's arrays and frees its tag; fails with CF_EINVAL for a tag whose result was
+ * not handed out yet (an unknown tag is a no-op) */
 int cf_server_release(cf_server *server, uint64_t tag);
 int cf_server_pending(cf_server *server, uint64_t *queued, uint64_t *outstanding);
 /* finishes every submitted request, then stops the workers */

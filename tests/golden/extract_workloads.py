@@ -1,11 +1,11 @@
-"""Writes workloads.json beside this file: the census rows and completion bounds of the two
+"""Writes carmichael-frontier_amd/cfjoin/data/workloads.json: the census rows and completion bounds of the two
 divisor families the reference logged, so instances can be rebuilt where the reference is absent.
 
 Sources (read as data): gpu/divisor_family_lambdaJ.json (118 rows, bound U = N of
 results_k64_best_global.json), gpu/mstar_divisor_family.json (111 rows, bound U = N0 of
 results_k64_best_global_prev_148_539839443357.json), and the positive-control incumbents
 results_k64*.json (factor lists and moduli).
-Run once in a checkout that has the reference:  python extract_workloads.py /path/to/reference
+Run once where the reference is present:  python tests/golden/extract_workloads.py /path/to/reference
 """
 import json
 import os
@@ -30,7 +30,8 @@ def main(ref: str) -> None:
         d = json.load(open(os.path.join(ref, f)))
         out["incumbents"][f] = {"n": str(d["n"]), "modulus": str(d["modulus"]),
                                 "factors": [int(p) for p in d["factors"]]}
-    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "workloads.json"), "w") as fh:
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    with open(os.path.join(root, "carmichael-frontier_amd", "cfjoin", "data", "workloads.json"), "w") as fh:
         json.dump(out, fh, indent=0)
 
 

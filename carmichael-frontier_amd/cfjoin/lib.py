@@ -25,6 +25,7 @@ P_u64 = ctypes.POINTER(ctypes.c_uint64)
 
 CF_ID_COLEX, CF_ID_PACK_ASC, CF_ID_PACK_DESC, CF_ID_MASK = 0, 1, 2, 3
 CF_SOLVE_PARTITION, CF_SOLVE_SORTMERGE, CF_SOLVE_PARTITION_DGLOBAL = 0, 1, 2
+CF_JOIN_NOALLOC_MAX_I = 1 << 20
 
 _lib = None
 _lock = threading.RLock()
@@ -83,6 +84,8 @@ SIGNATURES = {
     "cf_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_u64]),
     "cf_join_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
                                ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), P_u64]),
+    "cf_join_sorted_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64, c_void_p,
+                                       c_void_p, c_u64, P_u64]),
     "cf_plan_create_i": (c_int, [c_void_p, c_void_p, c_int, c_int, c_u64, c_u64, c_u64, c_u64, c_int, c_int, c_int,
                                  ctypes.POINTER(c_void_p)]),
     "cf_plan_create_d": (c_int, [c_void_p, c_void_p, c_int, c_int, c_u64, c_u64, c_u64, c_int,

@@ -248,11 +248,13 @@ int cf_ctx_create(int device, cf_ctx **out) {
     CF_HIP(hipGetDeviceCount(&ndev));
     CF_REQUIRE(device >= 0 && device < ndev, "no such device");
     CF_HIP(hipSetDevice(device));
+    u64 *scratch = join_scratch_bind(device);
     cf_ctx *c = new cf_ctx();
     c->device = device;
     CF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CF_HIP(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault));
     c->pinned_bytes = 4096;
+    c->join_scratch = scratch;
     *out = c;
     API_END
 }

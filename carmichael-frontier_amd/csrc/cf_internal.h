@@ -74,6 +74,7 @@ struct cf_ctx {
     // stream order keeps the reuse safe while branches run side by side
     int arena_branch = 0;
     size_t peak_live = 0;                         // largest live footprint since the last reset
+    u64 *join_scratch = nullptr;                  // this device's scratch of cf_join_sorted_noalloc
 };
 
 struct CaptureArena {
@@ -218,6 +219,12 @@ int bitlen(u64 v);
 // exclusive scan of u32 or u64 counts into u64 offsets; returns the total (synchronous)
 u64 scan_u32_to_u64(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);
 u64 scan_u64(cf_ctx *ctx, const u64 *in, u64 *out, size_t n);
+// device address of the no-allocation join's scratch on `device` (which must be current); looked up
+// once per device
+u64 *join_scratch_bind(int device);
+// exclusive scan of u64 with the block partials in ws (scan_ws_words(n) words; no allocation)
+size_t scan_ws_words(size_t n);
+void scan_u64_ws(cf_ctx *ctx, const u64 *in, u64 *out, size_t n, u64 *ws);
 void scan_u32_to_u64_async(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);   // no host sync
 
 // stable LSD radix sort of (key, val) by the low `bits` bits of key; in-place via scratch

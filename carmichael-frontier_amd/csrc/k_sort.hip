@@ -243,6 +243,25 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n) {
     return d2h_u64(ctx, m.p);
 }
 
+// The same scan with its block partials in a caller's workspace (scan_ws_words(n) words) instead
+// of pool allocations.
+size_t scan_ws_words(size_t n) {
+    const size_t nb = (n + ST - 1) / ST;
+    return nb <= 1 ? 0 : 2 * nb + 1 + scan_ws_words(nb);
+}
+
+void scan_u64_ws(cf_ctx *ctx, const u64 *in, u64 *out, size_t n, u64 *ws) {
+    const size_t nb = (n + ST - 1) / ST;
+    if (nb <= 1) {
+        CF_LAUNCH(ctx, "k_scan_final", (k_scan_final<u64>), dim3(1), dim3(SB), 0, in, n, (const u64 *)nullptr, out);
+        return;
+    }
+    u64 *part = ws, *part_off = ws + nb;
+    CF_LAUNCH(ctx, "k_scan_reduce", (k_scan_reduce<u64>), dim3((unsigned)nb), dim3(SB), 0, in, n, part);
+    scan_u64_ws(ctx, part, part_off, nb, ws + 2 * nb + 1);
+    CF_LAUNCH(ctx, "k_scan_final", (k_scan_final<u64>), dim3((unsigned)nb), dim3(SB), 0, in, n, (const u64 *)part_off, out);
+}
+
 u64 scan_u32_to_u64(cf_ctx *ctx, const u32 *in, u64 *out, size_t n) {
     scan_impl<u32>(ctx, in, out, n);
     return d2h_u64(ctx, out + n);

@@ -94,6 +94,21 @@ int cf_join_sorted(cf_ctx *ctx, const uint64_t *Dk_dev, const uint64_t *Did_dev,
                    const uint64_t *Ik_dev, const uint64_t *Iid_dev, uint64_t nI,
                    uint64_t **out_d_dev, uint64_t **out_i_dev, uint64_t *npairs);
 
+/* ---- join two sorted sides without allocating ---------------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): cf_join_sorted with the
+ * outputs supplied by the caller and every intermediate in a static per-device scratch of the
+ * library (about 24 MiB, resident while the library is loaded); calls no allocator.
+ * Requires nI <= CF_JOIN_NOALLOC_MAX_I (join a longer sorted I side in slices: the pairs of a slice
+ * are that slice's part of the I-major output). Writes the pairs to out_d_dev / out_i_dev in
+ * cf_join_sorted's order and sets *npairs; if *npairs exceeds cap, nothing is written and
+ * CF_EINVAL is returned with *npairs holding the count needed. Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (two streams, two contexts or two threads) race. */
+#define CF_JOIN_NOALLOC_MAX_I (1u << 20)
+int cf_join_sorted_noalloc(cf_ctx *ctx, const uint64_t *Dk_dev, const uint64_t *Did_dev, uint64_t nD,
+                           const uint64_t *Ik_dev, const uint64_t *Iid_dev, uint64_t nI,
+                           uint64_t *out_d_dev, uint64_t *out_i_dev, uint64_t cap, uint64_t *npairs);
+
 /* ---- filtered enumeration plans ---------------------------------------------------------
  * cf_plan_create_i replaces the igen kernel and its host set-up (gpu/igen_gpu.py:30-111) and
  * the record set of src/mitm64_idump_stream2.cpp:47-57: ascending r-subsets I of INS[0..n)

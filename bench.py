@@ -86,22 +86,28 @@ def cpu_baseline(insts, res_batched, threads, budget_keys):
                 seconds=round(t_sum, 2), pair_set_mismatches=mismatches)
 
 
-def host_cpu_model() -> str:
-    """The host CPU's model name and the logical CPUs this process may use."""
-    name = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            for ln in f:
-                if ln.startswith("model name"):
-                    name = ln.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    try:
-        n = len(os.sched_getaffinity(0))
-    except (AttributeError, OSError):
-        n = os.cpu_count() or 0
-    return f"{name}, {n} logical CPUs available"
+def host_cpu_model() -> dict:
+    """The host CPU: model name, logical CPUs this process may use, physical cores, SMT state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from calibrate_cpu import host_cpu
+    return host_cpu()
+
+
+def cpu_calibration():
+    """The port's speed against the reference's own CPU join (src/mitm64.cpp compiled from its
+    sources) on identical inputs, as measured by oracle/calibrate_cpu.py on this kind of host."""
+    path = os.path.join(ROOT, "oracle", "calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    return {"port_over_reference_speed": round(c["port_over_reference_speed"], 3), "threads": c["threads"],
+            "host": c["host_cpu"].get("model"), "anchors": {a["instance"]: {
+                "reference_keys_per_s": round(a["reference_mitm64"]["keys_per_s"]),
+                "port_keys_per_s": round(a["port_or_cpu_join"]["keys_per_s"]),
+                "same_records": a["same_records"],
+                "reference_matches_within_port_pairs": a["reference_matches_within_port_pairs"]}
+                for a in c["anchors"]}, "source": "oracle/calibration.json (oracle/calibrate_cpu.py)"}
 
 
 def copy_probe(ctx, nbytes):
@@ -119,11 +125,10 @@ def kernel_name(s):
     return s.split("(")[0].split("<")[0]
 
 
-def measure_traffic(kernel, family, setting):
-    """HBM bytes per launch of `kernel`: two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE,
-    KiB per dispatch) over a child run of this script (one eager solve of the batch with the same
-    tuning, then a 1 GiB device copy); each counter is scaled by what it reads for that copy (the
-    copy reads and writes 1 GiB). Returns (bytes per launch, details) or (None, reason)."""
+def counter_pass(counters, family, setting):
+    """One rocprofv3 counter pass (`counters`, at most 8 SQ and 4 TCC counters) over a child run of
+    this script (one eager solve of the batch with the same tuning, then a 1 GiB device copy).
+    Returns ({kernel: {counter: mean per dispatch}, '_dispatches': {kernel: n}}, None) or (None, reason)."""
     import csv
     import shutil
     import subprocess
@@ -132,36 +137,94 @@ def measure_traffic(kernel, family, setting):
     if not os.path.exists(prof):
         return None, "rocprofv3 not found"
     tune_arg = ",".join(str(setting.get(k, 0)) for k in ("slice_kib", "d_per_bucket", "seg_fill_pct", "probe_blocks"))
-    out = {}
     with tempfile.TemporaryDirectory(prefix="cf_pmc_", dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
-        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-            d = os.path.join(tmp, ctr)
-            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
-                   os.path.abspath(__file__), "--pmc-child", "--family", family, "--tuning", tune_arg,
-                   "--copy-probe", str(1 << 30)]
-            try:
-                subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180, check=True)
-            except (subprocess.SubprocessError, OSError) as e:
-                return None, f"{ctr} pass failed: {type(e).__name__}"
-            files = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
-            if not files:
-                return None, f"{ctr} pass wrote no counters"
-            vals, copy = [], 0.0
-            for r in csv.DictReader(open(files[0])):
-                k = kernel_name(r["Kernel_Name"])
-                v = float(r["Counter_Value"])
-                if k == kernel:
-                    vals.append(v)
-                elif "copyBuffer" in k:
-                    copy = max(copy, v)
-            if not vals or copy <= 0:
-                return None, f"{ctr}: no dispatches of {kernel} or of the copy"
-            scale = (1 << 20) / copy                # KiB the copy moved / KiB the counter read for it
-            out[ctr] = (sum(vals) / len(vals) * 1024 * scale, scale, len(vals))
+        cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", tmp, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--pmc-child", "--family", family, "--tuning", tune_arg,
+               "--copy-probe", str(1 << 30)]
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180, check=True)
+        except (subprocess.SubprocessError, OSError) as e:
+            return None, f"{'+'.join(counters)} pass failed: {type(e).__name__}"
+        files = [os.path.join(r, f) for r, _, fs in os.walk(tmp) for f in fs if f.endswith("counter_collection.csv")]
+        if not files:
+            return None, f"{'+'.join(counters)} pass wrote no counters"
+        sums, seen = {}, {}
+        for r in csv.DictReader(open(files[0])):
+            k = "copy" if "copyBuffer" in r["Kernel_Name"] else kernel_name(r["Kernel_Name"])
+            c = r["Counter_Name"]
+            v = float(r["Counter_Value"])
+            if k == "copy":   # the calibration copy: its largest dispatch (the 1 GiB one)
+                sums.setdefault(k, {})[c] = max(sums.get(k, {}).get(c, 0.0), v)
+                seen[k] = 1
+                continue
+            sums.setdefault(k, {})[c] = sums.get(k, {}).get(c, 0.0) + v
+            seen.setdefault(k, set()).add(r["Dispatch_Id"])
+    out = {k: {c: v / (1 if k == "copy" else len(seen[k])) for c, v in d.items()} for k, d in sums.items()}
+    out["_dispatches"] = {k: (1 if k == "copy" else len(v)) for k, v in seen.items()}
+    return out, None
+
+
+def measure_traffic(kernel, family, setting):
+    """HBM bytes per launch of `kernel`: two counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch); each counter is scaled by what it reads for the 1 GiB calibration copy (the copy reads
+    and writes 1 GiB). Returns (bytes per launch, details) or (None, reason)."""
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        res, why = counter_pass([ctr], family, setting)
+        if res is None:
+            return None, why
+        if kernel not in res or "copy" not in res or res["copy"].get(ctr, 0) <= 0:
+            return None, f"{ctr}: no dispatches of {kernel} or of the copy"
+        scale = (1 << 20) / res["copy"][ctr]      # KiB the copy moved / KiB the counter read for it
+        out[ctr] = (res[kernel][ctr] * 1024 * scale, scale, res["_dispatches"][kernel])
     total = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
     return total, {"fetch_bytes": round(out["FETCH_SIZE"][0]), "write_bytes": round(out["WRITE_SIZE"][0]),
                    "fetch_scale": round(out["FETCH_SIZE"][1], 4), "write_scale": round(out["WRITE_SIZE"][1], 4),
                    "launches": out["FETCH_SIZE"][2]}
+
+
+ALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_INSTS_VALU_INT32", "SQ_INSTS_VALU_INT64", "SQ_INSTS_SALU", "SQ_WAVES"]
+
+
+def valu_peaks(ctx):
+    """This GPU's vector-ALU issue rate (wave instructions per second) for v_add_u32 (full rate) and
+    v_mul_lo_u32 / v_mad_u64_u32 (the multiply forms), from the library's microbenchmark."""
+    import ctypes
+    from cfjoin import lib
+    peaks = {}
+    for op, name, iters in ((0, "v_add_u32", 65536), (1, "v_mul_lo_u32", 16384), (3, "v_mad_u64_u32", 8192)):
+        n, ms = lib.c_u64(), ctypes.c_double()
+        best = 0.0
+        for _ in range(2):
+            lib.check(lib.load().cf_peak_valu(ctx.handle, op, iters, ctypes.byref(n), ctypes.byref(ms)), "cf_peak_valu")
+            best = max(best, n.value / (ms.value / 1e3))
+        peaks[name] = best
+    return peaks
+
+
+def measure_alu(kernels, breakdown, family, setting, peaks):
+    """Integer-ALU roofline of `kernels`: VALU wave instructions per dispatch (SQ_INSTS_VALU, of which
+    SQ_INSTS_VALU_INT32 / _INT64 are integer arithmetic) over the kernel's mean launch time (HIP events,
+    eager step), against the measured v_add_u32 issue rate. A wave instruction is 64 lane operations."""
+    res, why = counter_pass(ALU_COUNTERS, family, setting)
+    if res is None:
+        return {"error": why}
+    peak = peaks["v_add_u32"]
+    out = {"counters": ALU_COUNTERS, "peak_wave_instr_per_s": {k: round(v, -6) for k, v in peaks.items()},
+           "peak_basis": "v_add_u32 issue rate measured by cf_peak_valu", "kernels": {}}
+    for k in kernels:
+        if k not in res or k not in breakdown or not breakdown[k][1]:
+            continue
+        ms = breakdown[k][0] / breakdown[k][1]
+        c = res[k]
+        rate = c["SQ_INSTS_VALU"] / (ms / 1e3)
+        out["kernels"][k] = {"valu_per_launch": round(c["SQ_INSTS_VALU"]), "int32": round(c["SQ_INSTS_VALU_INT32"]),
+                             "int64": round(c["SQ_INSTS_VALU_INT64"]), "salu": round(c["SQ_INSTS_SALU"]),
+                             "waves": round(c["SQ_WAVES"]), "avg_launch_ms": round(ms, 4),
+                             "achieved_wave_instr_per_s": round(rate, -6),
+                             "achieved_lane_ops_per_s": round(rate * 64, -9),
+                             "frac": round(rate / peak, 4)}
+    return out
 
 
 def c4_main(args):
@@ -562,12 +625,16 @@ def main():
         roofline["traffic_detail"] = detail
         if traffic:
             roofline["traffic_vs_algorithmic"] = round(traffic / (per_rec * nI_total / max(k_launches, 1)), 3)
+        roofline["alu"] = measure_alu(["k_part_scatter", "k_part_probe", "k_expand_write_count", "k_dwin_sort",
+                                       "k_part_scatter_keys", "k_dwin_scatter"], breakdown, args.family, setting,
+                                      valu_peaks(ctx))
 
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(insts, res_b, threads, args.cpu_keys)
         cpu["host_cpu"] = host_cpu_model()
+        cpu["calibration"] = cpu_calibration()
 
     line = {
         "metric": "residue keys joined/s (batched vs plain, 1 GPU), bit-exact match sets",

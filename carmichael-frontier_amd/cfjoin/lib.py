@@ -106,6 +106,7 @@ SIGNATURES = {
     "cf_batch_capture": (c_int, [c_void_p, c_u64]),
     "cf_batch_set_tuning": (c_int, [c_void_p, ctypes.POINTER(cf_tuning)]),
     "cf_batch_footprint": (c_int, [c_void_p, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "cf_peak_valu": (c_int, [c_void_p, c_int, c_u64, P_u64, ctypes.POINTER(ctypes.c_double)]),
     "cf_server_create": (c_int, [c_int, c_int, c_int, c_u64, ctypes.POINTER(c_void_p)]),
     "cf_server_submit": (c_int, [c_void_p, c_void_p, c_u64]),
     "cf_server_next": (c_int, [c_void_p, c_int, ctypes.POINTER(cf_served)]),

@@ -218,6 +218,17 @@ int cf_batch_footprint(cf_batch *batch, uint64_t *bytes, uint64_t *nD_total, uin
  * no counterpart there. */
 int cf_batch_capture(cf_batch *batch, uint64_t max_chunk);
 
+/* ---- integer issue rate ---------------------------------------------------------------------
+ * No counterpart in the reference. Measures this GPU's vector-ALU issue rate for one integer
+ * instruction (the denominator of the integer-ALU roofline the benchmark reports): every SIMD of
+ * every CU runs independent chains of the instruction for `iters` loop trips; returns the wave-level
+ * instructions executed and the elapsed device time. */
+#define CF_PEAK_ADD_U32 0
+#define CF_PEAK_MUL_LO_U32 1
+#define CF_PEAK_MUL_HI_U32 2
+#define CF_PEAK_MAD_U64_U32 3
+int cf_peak_valu(cf_ctx *ctx, int op, uint64_t iters, uint64_t *wave_instructions, double *ms);
+
 /* ---- serving loop (continuous batching) -------------------------------------------------------
  * Requests (instances) are submitted as they arrive and solved by `workers` threads, each with its
  * own stream on the device: a free worker takes everything that has arrived (up to max_batch) and

@@ -8,9 +8,10 @@ instance from the census (cfjoin/instance.py, the reference's recipe), solves th
 (cf_batch), and compares (nD, nI, pairs) with the log line by line; every pair is also re-checked
 exactly on the host (residue identity and integrality of the completion, pipeline.recheck).
 
-An instance whose insertion bound needs more than 128 bits is beyond the device path (the reference
-sends those to its host-exact join as well, gpu/mitm_gpu_mstar_driver.py:207); it is reported, not
-solved.
+An instance whose insertion bound needs more than 128 bits is beyond the device's 128-bit products:
+it is solved by the host-exact join (cfjoin/hostexact.py: D records from the device plan, I side in
+Python integers), as the reference sends those to its host-exact join (gpu/mitm_gpu_mstar_driver.py:207),
+and compared with its log line like the others.
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ import json
 import time
 from typing import Dict, List, Optional
 
+from . import hostexact
 from . import instance as I
 from . import lib as L
 from . import pipeline
@@ -96,7 +98,21 @@ def audit(path: str, families=("M0", "lambdaN"), max_batch_keys: float = 2e9, ma
         if log:
             log(f"[audit] batch {nbatch}: {len(group)} instances, {sum(s.nD + s.nI for s in res):,} keys, "
                 f"{st['ms_total']:.1f} ms device")
-    return dict(instances=len(items), solved=len(todo), batches=nbatch, keys=keys_total, pairs=pairs_total,
+    for r, x in beyond:
+        s = hostexact.solve(x, ctx)
+        keys_total += s.nD + s.nI
+        pairs_total += len(s.pairs_d)
+        got = (s.nD, s.nI, len(s.pairs_d))
+        if got != (r["nD"], r["nI"], r["pairs"]):
+            mismatches.append(dict(family=r["family"], D=r["D"], r=r["r"], got=got,
+                                   logged=(r["nD"], r["nI"], r["pairs"]), path="host-exact"))
+        if recheck and len(s.pairs_d):
+            recheck_fail += sum(not c["integral"] for c in pipeline.recheck(x, s))
+        if log:
+            log(f"[audit] host-exact: {r['family']} D={r['D']} r={r['r']} (Icap {x.Icap.bit_length()} bits): "
+                f"D {s.nD:,} | I {s.nI:,} pairs={len(s.pairs_d)}")
+    return dict(instances=len(items), solved=len(todo), solved_host_exact=len(beyond), batches=nbatch,
+                keys=keys_total, pairs=pairs_total,
                 device_ms=dev_ms, wall_s=time.perf_counter() - t0, mismatches=mismatches,
                 recheck_failures=recheck_fail,
                 beyond_device=[dict(family=r["family"], D=r["D"], r=r["r"], icap_bits=x.Icap.bit_length(),

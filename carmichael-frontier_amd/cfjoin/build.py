@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 SO = os.path.join(HERE, "libcfjoin.so")
-SOURCES = ["k_gen.hip", "k_sort.hip", "k_join.hip", "k_part.hip", "capi.hip", "solve.hip", "serve.hip", "k_peak.hip"]
+SOURCES = ["k_gen.hip", "k_sort.hip", "k_join.hip", "k_part.hip", "capi.hip", "solve.hip", "serve.hip", "k_peak.hip", "k_wide.hip"]
 ARCH = os.environ.get("CF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -36,7 +36,7 @@ class cf_instance(ctypes.Structure):
                 ("nI", ctypes.c_int32), ("hmax", ctypes.c_int32), ("INS", c_void_p), ("P0modM", c_u64),
                 ("icap_lo", c_u64), ("icap_hi", c_u64), ("dmin_lo", c_u64), ("dmin_hi", c_u64),
                 ("split_idx", ctypes.c_int32), ("i_id_mode", ctypes.c_int32), ("d_id_mode", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("M_hi", c_u64), ("P0modM_hi", c_u64)]
 
 
 class cf_stats(ctypes.Structure):

@@ -55,10 +55,12 @@ def make_cinstance(inst, keep: list) -> L.cf_instance:
     i_mode, d_mode = id_modes(inst)
     ilo, ihi = L.split128(inst.Icap)
     dlo, dhi = L.split128(inst.Dmin)
-    return L.cf_instance(M=inst.M, r=inst.r, nB=len(B), B=B.ctypes.data_as(ctypes.c_void_p), nI=len(INS),
-                         hmax=int(inst.hmax), INS=INS.ctypes.data_as(ctypes.c_void_p), P0modM=inst.P0 % inst.M,
+    mlo, mhi = L.split128(inst.M)
+    plo, phi = L.split128(inst.P0 % inst.M)
+    return L.cf_instance(M=mlo, r=inst.r, nB=len(B), B=B.ctypes.data_as(ctypes.c_void_p), nI=len(INS),
+                         hmax=int(inst.hmax), INS=INS.ctypes.data_as(ctypes.c_void_p), P0modM=plo,
                          icap_lo=ilo, icap_hi=ihi, dmin_lo=dlo, dmin_hi=dhi, split_idx=int(inst.split_idx),
-                         i_id_mode=i_mode, d_id_mode=d_mode, pad=0)
+                         i_id_mode=i_mode, d_id_mode=d_mode, pad=0, M_hi=mhi, P0modM_hi=phi)
 
 
 MODES = {"partition": L.CF_SOLVE_PARTITION, "sortmerge": L.CF_SOLVE_SORTMERGE,

@@ -256,8 +256,16 @@ void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, si
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
 // validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
-SideSpec spec_of_d(const cf_instance &in, u64 koff);
-SideSpec spec_of_i(const cf_instance &in, u64 koff);
+// the modulus an instance's keys live in: key_M = M for M < 2^64, else a divisor of M below 2^64
+// (wide: the join's pairs are then checked modulo M); mult = P0 mod key_M
+struct KeyMod {
+    unsigned __int128 M = 1, P0 = 0;
+    u64 key_M = 1, mult = 0;
+    bool wide = false;
+};
+KeyMod key_mod(const cf_instance &in);
+SideSpec spec_of_d(const cf_instance &in, u64 koff, const KeyMod *km = nullptr);
+SideSpec spec_of_i(const cf_instance &in, u64 koff, const KeyMod *km = nullptr);
 
 // Where one chunk's I records live: nseg segments, partition-major (nblk per partition); segment g
 // holds fill[g] records from slot O[g] (exact layout) or g << capl (capacity layout, O null).
@@ -340,6 +348,19 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
                       u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr);
+// moduli of 2^64 and above: the instance is joined modulo wide_divisor(M) (the largest divisor of M
+// in [2^40, 2^62]) and its pairs are checked exactly modulo M afterwards (k_wide.hip)
+struct WideMod {
+    u64 M_lo, M_hi;    // M
+    u64 P_lo, P_hi;    // P0 mod M
+    u64 on;            // 1: the instance's M is 2^64 or more; 0: its pairs are kept as they are
+};
+u64 wide_divisor(unsigned __int128 M);
+// keep[e] = pair e is exact modulo its instance's M; the kept pairs are written in order to od, oi,
+// oj and their count to *count (pos: n + 1 words of scan)
+void wide_check_pairs(cf_ctx *ctx, const u64 *pd, const u64 *pi, const u64 *pj, u64 n, const SideDesc *dd,
+                      const SideDesc *id, const WideMod *wm, u32 *keep, u64 *pos, u64 *od, u64 *oi, u64 *oj,
+                      unsigned long long *count);
 // per-instance leaf totals of a frontier added into out[0, ninst) (nodes of an instance are contiguous)
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
 

@@ -33,6 +33,7 @@ struct Request {
     std::vector<u64> B, INS;   // owned copies of the instance's prime lists
     u64 tag;
     double t_submit;
+    u64 key_M;                 // the modulus of its keys: a batch's key moduli must sum below 2^64 - 1
 };
 
 struct Served {
@@ -148,7 +149,11 @@ void worker_main(cf_server *s) {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
             if (s->pending.empty()) break;   // stopping and drained
+            // one batch is one key space: its requests' key moduli must sum below 2^64 - 1
+            unsigned __int128 keys = 0;
             while (!s->pending.empty() && (int)take.size() < s->max_batch) {
+                keys += s->pending.front().key_M;
+                if (!take.empty() && keys >= (unsigned __int128)~0ULL) break;
                 take.push_back(std::move(s->pending.front()));
                 s->pending.pop_front();
             }
@@ -199,9 +204,11 @@ int cf_server_create(int device, int workers, int max_batch, uint64_t mem_budget
 int cf_server_submit(cf_server *s, const cf_instance *inst, uint64_t tag) {
     API_BEGIN
     CF_REQUIRE(s && inst && inst->B && inst->INS, "NULL argument");
-    (void)spec_of_d(*inst, 0);   // a malformed request fails here, not inside a batch of others
-    (void)spec_of_i(*inst, 0);
+    const KeyMod km = key_mod(*inst);
+    (void)spec_of_d(*inst, 0, &km);   // a malformed request fails here, not inside a batch of others
+    (void)spec_of_i(*inst, 0, &km);
     Request q;
+    q.key_M = km.key_M;
     q.inst = *inst;
     q.B.assign(inst->B, inst->B + inst->nB);
     q.INS.assign(inst->INS, inst->INS + inst->nI);

@@ -109,10 +109,23 @@ Node root_node(const SideDesc &sd, int side, int inst) {
 
 }  // namespace
 
+// The modulus the keys of an instance live in: M itself below 2^64, else its largest divisor in
+// [2^40, 2^62] (the pairs are then checked exactly modulo M after the join).
+KeyMod key_mod(const cf_instance &in) {
+    KeyMod k;
+    k.M = ((u128)in.M_hi << 64) | in.M;
+    CF_REQUIRE(k.M >= 1 && k.M < ((u128)1 << 126), "need 1 <= M < 2^126");
+    k.P0 = (((u128)in.P0modM_hi << 64) | in.P0modM) % k.M;
+    k.wide = in.M_hi != 0;
+    k.key_M = k.wide ? wide_divisor(k.M) : in.M;
+    k.mult = k.wide ? (u64)(k.P0 % k.key_M) : in.P0modM;
+    return k;
+}
+
 // Validated side specs of one instance (the preconditions of cf_plan_create_d / _i).
-SideSpec spec_of_d(const cf_instance &in, u64 koff) {
+SideSpec spec_of_d(const cf_instance &in, u64 koff, const KeyMod *km) {
     CF_REQUIRE(in.nB >= 1 && in.nB <= 64 && in.r >= 1 && in.r <= 16, "need nB <= 64 base primes and 1 <= r <= 16");
-    CF_REQUIRE(in.M >= 1, "need 1 <= M < 2^64");
+    const KeyMod k = km ? *km : key_mod(in);
     for (int i = 1; i < in.nB; ++i) CF_REQUIRE(in.B[i - 1] < in.B[i], "base must be strictly ascending");
     CF_REQUIRE(in.d_id_mode == ID_PACK_DESC || in.d_id_mode == ID_MASK, "D ids are 6-bit packed or deletion masks");
     if (in.d_id_mode == ID_PACK_DESC) CF_REQUIRE(in.r <= 10, "6-bit packed ids need r <= 10");
@@ -126,7 +139,7 @@ SideSpec spec_of_d(const cf_instance &in, u64 koff) {
     s.side = SIDE_D;
     s.primes.assign(in.B, in.B + in.nB);
     s.r = in.r;
-    s.M = in.M;
+    s.M = k.key_M;
     s.mult = 1;
     s.cap = ((u128)in.dmin_hi << 64) | in.dmin_lo;
     s.id_mode = in.d_id_mode;
@@ -135,9 +148,9 @@ SideSpec spec_of_d(const cf_instance &in, u64 koff) {
     return s;
 }
 
-SideSpec spec_of_i(const cf_instance &in, u64 koff) {
+SideSpec spec_of_i(const cf_instance &in, u64 koff, const KeyMod *km) {
     CF_REQUIRE(in.r >= 1 && in.r <= 16 && in.nI >= 0 && in.nI <= 1024, "need 1 <= r <= 16 and n <= 1024");
-    CF_REQUIRE(in.M >= 1, "need 1 <= M < 2^64");
+    const KeyMod k = km ? *km : key_mod(in);
     for (int i = 1; i < in.nI; ++i) CF_REQUIRE(in.INS[i - 1] < in.INS[i], "INS must be strictly ascending");
     CF_REQUIRE(in.i_id_mode == ID_COLEX || in.i_id_mode == ID_PACK_ASC, "I ids are colex ranks or 8-bit packed");
     if (in.i_id_mode == ID_PACK_ASC) CF_REQUIRE(in.nI <= 256 && in.r <= 8, "8-bit packed ids need n <= 256 and r <= 8");
@@ -154,8 +167,8 @@ SideSpec spec_of_i(const cf_instance &in, u64 koff) {
     s.side = SIDE_I;
     s.primes.assign(in.INS, in.INS + in.nI);
     s.r = in.r;
-    s.M = in.M;
-    s.mult = in.P0modM;
+    s.M = k.key_M;
+    s.mult = k.mult;
     s.cap = ((u128)in.icap_hi << 64) | in.icap_lo;
     s.split_idx = in.split_idx;
     s.hmax = in.hmax;
@@ -228,6 +241,9 @@ struct cf_batch {
     DBuf<u64> store;
     DBuf<SideDesc> d_desc, i_desc;
     DBuf<u64> koffs;
+    std::vector<WideMod> wide;                   // per instance: M and P0 mod M when M >= 2^64
+    int nwide = 0;
+    DBuf<WideMod> wide_dev;
     std::map<int, DBuf<Node>> droots, iroots;    // per radius: the enumeration roots of live sides
     std::map<int, size_t> nd_roots, ni_roots;
     u64 pair_cap = 1 << 16;                      // pair capacity of the next solve (grows to fit)
@@ -246,11 +262,20 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
     cf_ctx *ctx = b->ctx;
     b->n = n;
     b->koff.resize(n);
-    // key offsets: instance j owns keys [koff_j, koff_j + M_j); the union must stay below 2^64 - 1
+    // key offsets: instance j owns keys [koff_j, koff_j + M_j) (M_j the modulus of its keys); the
+    // union must stay below 2^64 - 1
+    std::vector<KeyMod> km(n);
+    b->wide.assign(n, WideMod{});
+    b->nwide = 0;
     u128 K = 0;
     for (int j = 0; j < n; ++j) {
+        km[j] = key_mod(insts[j]);
         b->koff[j] = (u64)K;
-        K += insts[j].M;
+        K += km[j].key_M;
+        if (km[j].wide) {
+            b->wide[j] = WideMod{(u64)km[j].M, (u64)(km[j].M >> 64), (u64)km[j].P0, (u64)(km[j].P0 >> 64), 1};
+            b->nwide += 1;
+        }
     }
     CF_REQUIRE(K < (u128)~0ULL, "the batch's moduli sum to 2^64 or more: split the batch");
     b->dd.resize(n);
@@ -260,8 +285,8 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
     std::vector<u64> buf;
     std::vector<std::array<size_t, 4>> offd(n), offi(n);
     for (int j = 0; j < n; ++j) {
-        SideSpec sd = spec_of_d(insts[j], b->koff[j]);
-        SideSpec si = spec_of_i(insts[j], b->koff[j]);
+        SideSpec sd = spec_of_d(insts[j], b->koff[j], &km[j]);
+        SideSpec si = spec_of_i(insts[j], b->koff[j], &km[j]);
         side_tables_host(sd, b->dd[j], buf, offd[j].data());
         side_tables_host(si, b->id[j], buf, offi[j].data());
         // the 32-bit scaled leaf step needs every prime of the instance (both sides) below 2^32
@@ -285,6 +310,10 @@ void batch_prepare(cf_batch *b, const cf_instance *insts, int n) {
     h2d(ctx, b->d_desc.p, b->dd.data(), n);
     h2d(ctx, b->i_desc.p, b->id.data(), n);
     h2d(ctx, b->koffs.p, b->koff.data(), n);
+    if (b->nwide) {
+        b->wide_dev = DBuf<WideMod>(ctx, n);
+        h2d(ctx, b->wide_dev.p, b->wide.data(), n);
+    }
     // enumeration roots, resident for every solve of the batch
     for (auto &g : b->groups) {
         std::vector<Node> rd, ri;
@@ -529,6 +558,18 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         cap = np;
     }
     b->pair_cap = std::max<u64>(b->pair_cap, out.npairs);
+    if (b->nwide && out.npairs) {
+        // instances of M >= 2^64 were joined modulo a divisor of M: keep the pairs exact modulo M
+        const u64 np = out.npairs;
+        DBuf<u32> keep(ctx, np);
+        DBuf<u64> pos(ctx, np + 1), od(ctx, np), oi(ctx, np), oj(ctx, np);
+        wide_check_pairs(ctx, out.pd.p, out.pi.p, out.pinst.p, np, d_desc.p, i_desc.p, b->wide_dev.p, keep.p, pos.p,
+                         od.p, oi.p, oj.p, (unsigned long long *)cnt_pairs.p);
+        out.pd = std::move(od);
+        out.pi = std::move(oi);
+        out.pinst = std::move(oj);
+        out.npairs = host_value(ctx, tape, cnt_pairs.p);
+    }
     if (capturing) return;
     clk.mark();  // 1
     out.nD.assign(n, 0);
@@ -726,7 +767,8 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
         CF_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        if (ke) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        if (ke & KERR_DWINDOW) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
+        if (ke & KERR_CAND) throw CfError{CF_EINTERNAL, "the probe's candidate list was truncated"};
         if (np != G->npairs) throw CfError{CF_EINTERNAL, "a replay of the captured solve found a different pair count"};
         std::vector<u64> nD(b->n), nI(b->n);
         d2h(ctx, nD.data(), G->out.dcount.p, b->n);
@@ -800,6 +842,7 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
     *out_i = nullptr;
     *npairs = 0;
     ctx->peak_live = 0;
+    CF_REQUIRE(mode != CF_SOLVE_SORTMERGE || inst->M_hi == 0, "the sort-merge mode needs M < 2^64");
     if (mode != CF_SOLVE_SORTMERGE) {
         cf_batch b;
         b.ctx = ctx;

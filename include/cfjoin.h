@@ -154,7 +154,15 @@ typedef struct cf_instance {
     int32_t i_id_mode;        /* CF_ID_COLEX or CF_ID_PACK_ASC (8 bits) */
     int32_t d_id_mode;        /* CF_ID_PACK_DESC (6 bits) or CF_ID_MASK */
     int32_t pad;
+    uint64_t M_hi;            /* M = M_hi * 2^64 + M: moduli up to 2^126 (see below) */
+    uint64_t P0modM_hi;       /* P0 mod M = P0modM_hi * 2^64 + P0modM */
 } cf_instance;
+/* Moduli of 2^64 and above (M_hi != 0), which the reference's runner leaves unsearched
+ * (gpu/run_mstar_family.py:70-73): keys are residues modulo M1, the largest divisor of M in
+ * [2^40, 2^62] (M factored by trial division below 2^20, the cofactor kept whole), and every pair of
+ * that join is checked exactly modulo M on the device before it is returned, so the pair set is the
+ * join's modulo M. Such an instance takes the partitioned solve modes only; a batch still needs
+ * the sum of its instances' key moduli (M1 for these) below 2^64. */
 
 typedef struct cf_stats {
     uint64_t nD, nI, npairs;

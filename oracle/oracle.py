@@ -168,6 +168,63 @@ def join_dsweep(M: int, r: int, P0modM: int, Icap: int, Dmin: int, B, INS, split
                 ti=float(times[0]), td=float(times[1]))
 
 
+def exact_join(M: int, r: int, P0: int, Icap: int, Dmin: int, B, INS, split_idx: Optional[int] = None,
+               hmax: Optional[int] = None):
+    """The join in Python integers, for moduli of any size (small instances only): every pair of an
+    r-subset D of B with prod(D) >= Dmin and an r-subset I of INS with prod(I) < Icap and at most hmax
+    indices >= split_idx, with prod(D) = P0 * prod(I) (mod M). Follows the record sets of
+    src/mitm64_ddump.cpp:29-44 (D chosen from the largest base prime down, cut once even the largest
+    completion stays below Dmin) and src/mitm64_idump_stream2.cpp:47-57 (I chosen upward, cut once the
+    cheapest completion reaches Icap), joined by residue as src/mitm64.cpp:51-121 does.
+    Returns (nD, nI, set of (D index tuple, I index tuple)), indices ascending into B and INS."""
+    B = [int(x) for x in B]
+    INS = [int(x) for x in INS]
+    split_idx = len(INS) if split_idx is None else split_idx
+    hmax = r if hmax is None else hmax
+    by_res = {}
+    nD = 0
+
+    def drec(hi, t, prod, chosen):
+        nonlocal nD
+        if t == 0:
+            nD += 1
+            by_res.setdefault(prod % M, []).append(tuple(sorted(chosen)))
+            return
+        for i in range(hi, t - 2, -1):
+            best = prod
+            for q in range(i, i - t, -1):
+                best *= B[q]
+            if best < Dmin:      # even the largest completion from here is too small, and it only shrinks
+                break
+            drec(i - 1, t - 1, prod * B[i], chosen + [i])
+
+    drec(len(B) - 1, r, 1, [])
+    pairs = set()
+    nI = 0
+    P0m = P0 % M
+
+    def irec(lo, t, prod, chosen, nlarge):
+        nonlocal nI
+        if t == 0:
+            nI += 1
+            for d in by_res.get(P0m * prod % M, ()):
+                pairs.add((d, tuple(chosen)))
+            return
+        for i in range(lo, len(INS) - t + 1):
+            cheap = prod
+            for q in range(i, i + t):
+                cheap *= INS[q]
+            if cheap >= Icap:    # the cheapest completion from here is too large, and it only grows
+                break
+            nl = nlarge + (i >= split_idx)
+            if nl > hmax:
+                continue
+            irec(i + 1, t - 1, prod * INS[i], chosen + [i], nl)
+
+    irec(0, r, 1, [], 0)
+    return nD, nI, pairs
+
+
 def instance_records(inst, rank: Optional[bool] = None):
     """(Dk, Did, Ik, Iid) of an Instance (cfjoin.instance.Instance-like object)."""
     Dk, Did = ddump(inst.M, inst.r, inst.Dmin, inst.B0, bitmask=inst.r > 10)

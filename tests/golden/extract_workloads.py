@@ -1,5 +1,6 @@
 """Writes carmichael-frontier_amd/cfjoin/data/workloads.json: the census rows and completion bounds of the two
-divisor families the reference logged, so instances can be rebuilt where the reference is absent.
+divisor families the reference logged, ten neighbourhood-census rows of moduli above 2^64, and the incumbents,
+so instances can be rebuilt where the reference is absent.
 
 Sources (read as data): gpu/divisor_family_lambdaJ.json (118 rows, bound U = N of
 results_k64_best_global.json), gpu/mstar_divisor_family.json (111 rows, bound U = N0 of
@@ -25,6 +26,17 @@ def main(ref: str) -> None:
     out["families"]["M0"] = {
         "U": str(N0["n"]), "anchor": str(N0["modulus"]), "census": "gpu/mstar_divisor_family.json",
         "rows": [{k: row[k] for k in keep} for row in (m0["survivors"] if isinstance(m0, dict) else m0)]}
+    # neighbourhood census rows of 2^64 and above (the reference's runner leaves them unsearched):
+    # two of each of 65, 68, 71 and 74 bits around lambda(N), and the first two around M0
+    nb = json.load(open(os.path.join(ref, "gpu", "nbhd_p300_v2.json")))
+    wide = [row for row in nb["survivors"] if int(row["D"]) >= 1 << 64]
+    pick = []
+    for bits in (65, 68, 71, 74):
+        pick += [row for row in wide if int(row["D"]).bit_length() == bits and row["seed"] == str(N["modulus"])][:2]
+    pick += [row for row in wide if row["seed"] == str(N0["modulus"])][:2]
+    out["families"]["nbhd_wide"] = {
+        "U": str(nb["U"]), "anchor": str(pick[0]["D"]), "census": "gpu/nbhd_p300_v2.json",
+        "rows": [{k: row[k] for k in keep} for row in pick]}
     for f in ("results_k64.json", "results_k64_best_global.json",
               "results_k64_best_global_prev_148_539839443357.json", "results_k64_best_global_prev_148_N.json"):
         d = json.load(open(os.path.join(ref, f)))

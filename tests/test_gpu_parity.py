@@ -485,10 +485,10 @@ def test_every_logged_instance_matches_log():
     """All 1 412 logged lines of both families (gpu/mitm_gpu_mstar_driver.py:241,284): each instance
     rebuilt from the census and solved on the device in batches; (nD, nI, pairs) equal to the log and
     every pair re-checked exactly. The one instance whose insertion bound needs 134 bits (M0, r = 14)
-    is beyond the device path, as it was beyond the reference's GPU path."""
+    is beyond the device's 128-bit products and takes the host-exact join, as the reference's did."""
     from cfjoin import audit
     say("audit", "all logged instances")
     rep = audit.audit(GOLDEN_LOGGED)
-    assert rep["instances"] == 1412 and rep["solved"] == 1411
+    assert rep["instances"] == 1412 and rep["solved"] == 1411 and rep["solved_host_exact"] == 1
     assert rep["mismatches"] == [] and rep["recheck_failures"] == 0
     assert [(b["family"], b["r"], b["icap_bits"]) for b in rep["beyond_device"]] == [("M0", 14, 134)]

@@ -278,3 +278,46 @@ def test_seeded_modulus_fixture_pairs_are_exact():
         assert 200 <= len(I.pool(M, cap)) <= 500
         inst = I.c2_instance(f["seed"], f["s"])
         _fixture_pairs_hold(inst, f["pairs"])
+
+
+# ---- the exact join in Python integers (moduli of any size) --------------------------------------
+def _brute_join(M, r, P0, Icap, Dmin, B, INS, split_idx, hmax):
+    Ds = [c for c in itertools.combinations(range(len(B)), r) if prod(B[i] for i in c) >= Dmin]
+    Is = [c for c in itertools.combinations(range(len(INS)), r)
+          if prod(INS[i] for i in c) < Icap and sum(i >= split_idx for i in c) <= hmax]
+    pairs = {(d, i) for d in Ds for i in Is if (prod(B[x] for x in d) - P0 * prod(INS[x] for x in i)) % M == 0}
+    return len(Ds), len(Is), pairs
+
+
+@pytest.mark.parametrize("M", [(1 << 64) + 13 * 2 ** 40, 3 << 70, 2 ** 20 * 3 ** 5 * 5 ** 3 * 7 * 11, 997])
+def test_exact_join_brute_force(M):
+    """exact_join against all r-subsets: the bounds (Dmin inclusive, Icap exclusive, stratum) and the
+    congruence, with a planted pair; a planted off-by-one in either bound would change the counts."""
+    P = [p for p in range(5, 400) if V.is_prime(p) and M % p]
+    B, INS = P[:12], P[12:30]
+    r = 3
+    D0, I0 = (9, 10, 11), (0, 4, 7)
+    P0 = prod(B[i] for i in D0) * pow(prod(INS[i] for i in I0), -1, M) % M
+    Dmin = prod(B[i] for i in D0)          # D0 sits exactly on the inclusive bound
+    Icap = prod(INS[i] for i in (3, 15, 17))
+    for split_idx, hmax in ((len(INS), r), (10, 1)):
+        got = O.exact_join(M, r, P0, Icap, Dmin, B, INS, split_idx, hmax)
+        want = _brute_join(M, r, P0, Icap, Dmin, B, INS, split_idx, hmax)
+        assert got == want and (D0, I0) in got[2]
+        assert O.exact_join(M, r, P0, Icap, Dmin + 1, B, INS, split_idx, hmax)[0] == want[0] - 1
+
+
+def test_exact_join_equals_cpu_join_below_2_64():
+    """Below 2^64 the exact join and the C oracle's hash join give the same records and pair set."""
+    for seed, s in ((0, 100), (1, 10 ** 4), (3, 10)):
+        inst = I.c1_instance(seed, s)
+        nD, nI, pairs = O.exact_join(inst.M, inst.r, inst.P0, inst.Icap, inst.Dmin, inst.B0, inst.INS,
+                                     inst.split_idx, inst.hmax)
+        res = O.cpu_join(inst.M, inst.r, inst.P0 % inst.M, inst.Icap, inst.Dmin, inst.B0, inst.INS, inst.split_idx,
+                         inst.hmax, inst.rank_ids, threads=2)
+        dd = lambda v: tuple(sorted(V.decode_packed(v, inst.r, 6)))  # noqa: E731
+        di = (lambda v: V.unrank_colex(v, inst.r, len(inst.INS))) if inst.rank_ids else \
+            (lambda v: V.decode_packed(v, inst.r, 8))
+        theirs = {(dd(int(a)), tuple(di(int(b)))) for a, b in zip(res["pairs_d"], res["pairs_i"])}
+        assert (nD, nI) == (res["nD"], res["nI"]) and len(theirs) == len(res["pairs_d"])
+        assert pairs == theirs and len(pairs) > 0

@@ -28,6 +28,12 @@ static size_t pool_round(size_t bytes) {
 }
 
 
+// the live footprint (bytes handed out and not yet freed, cached blocks included when reused) and its peak
+static void note_live(cf_ctx *ctx, long long delta) {
+    ctx->live_bytes = (size_t)((long long)ctx->live_bytes + delta);
+    ctx->peak_live = std::max(ctx->peak_live, ctx->live_bytes);
+}
+
 void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
     const size_t sz = pool_round(bytes);
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -37,6 +43,7 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
         if (a != fl.end() && a->first <= sz + sz / 2) {
             void *p = a->second;
             ctx->live[p] = a->first;
+            note_live(ctx, (long long)a->first);
             fl.erase(a);
             return p;
         }
@@ -48,6 +55,7 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
         ctx->free_blocks.erase(it);
         ctx->cached_bytes -= bs;
         ctx->live[p] = bs;
+        note_live(ctx, (long long)bs);
         if (ctx->arena) ctx->arena->owned[p] = bs;
         return p;
     }
@@ -70,10 +78,8 @@ void *ctx_alloc(cf_ctx *ctx, size_t bytes) {
         }
     }
     ctx->live[p] = sz;
+    note_live(ctx, (long long)sz);
     if (ctx->arena) ctx->arena->owned[p] = sz;
-    size_t live = 0;
-    for (auto &kv : ctx->live) live += kv.second;
-    ctx->peak_live = std::max(ctx->peak_live, live);
     return p;
 }
 
@@ -82,6 +88,7 @@ void ctx_free(cf_ctx *ctx, void *p) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->live.find(p);
     if (it == ctx->live.end()) return;
+    note_live(ctx, -(long long)it->second);
     if (ctx->arena) {
         ctx->arena->free[ctx->arena_branch].emplace(it->second, p);
         ctx->arena->owned[p] = it->second;
@@ -96,7 +103,11 @@ void ctx_free(cf_ctx *ctx, void *p) {
 void arena_release(cf_ctx *ctx, CaptureArena *a) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     for (auto &kv : a->owned) {
-        ctx->live.erase(kv.first);
+        auto it = ctx->live.find(kv.first);
+        if (it != ctx->live.end()) {
+            note_live(ctx, -(long long)it->second);
+            ctx->live.erase(it);
+        }
         ctx->free_blocks.emplace(kv.second, kv.first);
         ctx->cached_bytes += kv.second;
     }

@@ -73,6 +73,7 @@ struct cf_ctx {
     // from it and joined back): a block freed on one branch is reused only on that branch, whose
     // stream order keeps the reuse safe while branches run side by side
     int arena_branch = 0;
+    size_t live_bytes = 0;                        // bytes handed out and not yet freed
     size_t peak_live = 0;                         // largest live footprint since the last reset
     u64 *join_scratch = nullptr;                  // this device's scratch of cf_join_sorted_noalloc
 };
@@ -249,11 +250,11 @@ u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 // radix-partitioned join (k_part.hip)
 void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount);
 // DR: n records of 16 bytes (exact key, D leaf index) in bucket order; T: CF_TABLE_SLOTS u16 slots per
 // bucket, zeroed
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
 // validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
 // the modulus an instance's keys live in: key_M = M for M < 2^64, else a divisor of M below 2^64
@@ -328,8 +329,8 @@ struct DLeaves {
     const Node *nodes;
     size_t nn;
     const u64 *off;
-    u64 n;
-    const u64 *tp;    // tile owner map of [0, n), or null
+    u64 lo, n;        // the table's D leaves [lo, lo + n); its records refer to them relative to lo
+    const u64 *tp;    // tile owner map of [lo, lo + n), or null
 };
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
 // rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)

@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
             const u64 b = hbucket(h, kh);
             const u32 slot = atomicAdd(&bcur[b], 1u);
             const u64 b0 = bstart[b];
-            DR[b0 + slot] = DRec{key, tl.j0 + i};
+            DR[b0 + slot] = DRec{key, tl.j0 - L.lo + i};   // the leaf relative to the source's first
             if (slot < (u32)TSLOTS) {
                 const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | (hfp(h) & fmask));
@@ -1329,19 +1329,19 @@ static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     return LeafSrc{descs, nodes, nn, off, lo, n, depth, tp.p};
 }
 
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount) {
     if (!n) return;
     DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bcount);
 }
 
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 n,
-                         int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T) {
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+                         u64 n, int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T) {
     if (!n) return;
     DBuf<u64> tp;
-    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, 0, n, depth, tp);
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bstart, bcur,
               (DRec *)DR, T);
 }
@@ -1501,7 +1501,7 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
                       u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
-    LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, 0, dl.n, depth, dl.tp};
+    LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, dl.lo, dl.n, depth, dl.tp};
     CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const uint2 *)rec, S,
               ncand, ccap,
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst, kerr);

@@ -339,6 +339,81 @@ u64 default_chunk() {
     return std::min<u64>((u64)1 << 31, std::max<u64>((u64)free_b / 48, (u64)1 << 24));
 }
 
+// One table of D records: the D leaves [lo, lo + n) of a radius group bucketed by hash (bucket
+// starts, records in bucket order, slots, filter) for the probe.
+struct DTable {
+    u64 lo = 0, n = 0;
+    DBuf<u64> bstart, DR;
+    DBuf<unsigned short> T;
+    DBuf<u32> F;
+    DBuf<u64> tp;     // tile owner map of the range's leaves (window build), or empty
+};
+
+// The D leaves of a group in nr ranges of at most m records: one range while the window build takes
+// the whole side (fewer than 2^32 records and at most 256 windows per partition), else the fewest
+// equal ranges it takes one by one. The global-atomic build (d_part false) takes one range of any size.
+void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
+    nr = 1;
+    m = nD;
+    if (!d_part || nD == 0) return;
+    for (;;) {
+        m = (nD + nr - 1) / nr;
+        const Partitioning q = assign_buckets(m, tn);
+        int wb = 0;
+        u32 R = 0;
+        if (m < ((u64)1 << 32) && dwin_geometry(q.kh, m, wb, R)) return;
+        if (nr >= 65536) break;
+        nr = nr < 8 ? nr + 1 : nr + nr / 8;
+    }
+    nr = 1;   // no range size fits: the global-atomic build over the whole side
+    m = nD;
+}
+
+// Builds the table of D leaves [lo, lo + m): partitioned like the I side (keys alongside), moved into
+// bucket windows and counting-sorted one window per block in LDS; by per-record global atomics when
+// the window build does not fit (d_part false, or a window overflowing on heavily repeated keys).
+// DR's leaf references are relative to lo.
+void build_dtable(cf_ctx *ctx, const SideDesc *d_desc, const Frontier &fd, int depth, const Partitioning &pt, int FB,
+                  bool d_part, u64 lo, u64 m, const Tuning &tn, DTable &t, u32 *ovf, u32 *kerr, Tape *tape) {
+    const KeyHash &kh = pt.kh;
+    t.lo = lo;
+    t.n = m;
+    t.bstart = DBuf<u64>(ctx, pt.NB + 1);
+    t.DR = DBuf<u64>(ctx, 2 * (m ? m : 1));
+    t.T = DBuf<unsigned short>(ctx, pt.NB * CF_TABLE_SLOTS);
+    t.F = DBuf<u32>(ctx, std::max<u64>(filter_words(kh, FB), 1));
+    if (!m) return;
+    fill_u32(ctx, t.F.p, filter_words(kh, FB), 0);
+    int wb = 0;
+    u32 R = 0;
+    if (d_part && m < ((u64)1 << 32) && dwin_geometry(kh, m, wb, R)) {
+        IChunkLayout dl;
+        const u64 slots = std::max<u64>(part_cap_slots(m, kh, tn), m);
+        DBuf<u64> drec(ctx, slots), dkeys(ctx, slots);
+        part_scatter_cap(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, tn, dl, drec.p, ovf, t.tp, dkeys.p);
+        if (host_value_u32(ctx, tape, ovf))
+            part_scatter_exact(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, dl, drec.p, t.tp.p, dkeys.p);
+        const u64 nw = (u64)pt.P() * R;
+        DBuf<u32> wcount(ctx, nw);
+        DBuf<u64> wreg(ctx, 2 * nw * dwin_capacity()), wbase(ctx, nw + 1);
+        part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, dkeys.p, wcount.p, wreg.p, ovf);
+        if (!host_value_u32(ctx, tape, ovf)) {
+            scan_u32_to_u64_async(ctx, wcount.p, wbase.p, nw);
+            part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, t.bstart.p, t.DR.p, t.T.p, t.F.p, FB, kerr);
+            return;
+        }
+        t.tp.reset();
+    }
+    fill_u32(ctx, (u32 *)t.T.p, pt.NB * CF_TABLE_SLOTS / 2, 0);
+    DBuf<u32> cnt(ctx, pt.NB);
+    fill_u32(ctx, cnt.p, pt.NB, 0);
+    part_bucket_count(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, cnt.p);
+    scan_u32_to_u64_async(ctx, cnt.p, t.bstart.p, pt.NB);
+    fill_u32(ctx, cnt.p, pt.NB, 0);   // reused as the scatter cursors
+    part_bucket_scatter(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, t.bstart.p, cnt.p, t.DR.p, t.T.p);
+    part_filter(ctx, t.T.p, t.bstart.p, kh, FB, t.F.p);
+}
+
 // One radius group's branch of a captured solve: while it lives, launches go to the group's side
 // stream (forked from the capturing stream) and blocks the group frees are reused only on that
 // branch; on leaving it records the group's completion event, which the capturing stream waits on.
@@ -433,60 +508,23 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             Frontier fd = expand_to_last(ctx, d_desc.p, b->droots.at(g.first).p, b->nd_roots.at(g.first), g.first, tape);
             const u64 nD = fd.total;
             if (nD) inst_bounds(ctx, fd.nodes.p, fd.off.p, fd.nn, n, dcount.p);
-            const Partitioning pt = assign_buckets(nD, b->tune);
+            // D side: ranges of D leaves, each bucketed into a table of its own; every range has the
+            // same bucket geometry, so an I record's partition, bucket and fingerprint hold for all
+            u64 nr = 1, m_rng = nD;
+            d_ranges(nD, b->tune, d_part, nr, m_rng);
+            const Partitioning pt = assign_buckets(m_rng, b->tune);
             const KeyHash &kh = pt.kh;
-            DBuf<u64> bstart(ctx, pt.NB + 1);
-            DBuf<u64> DR(ctx, 2 * (nD ? nD : 1));
-            DBuf<unsigned short> T(ctx, pt.NB * CF_TABLE_SLOTS);
             // bucket filter: the probe tests it in LDS before reading the table
             const int FB = nD ? filter_bits(kh) : 0;
-            DBuf<u32> F(ctx, std::max<u64>(filter_words(kh, FB), 1));
-            DBuf<u64> tpD;   // tile owner map of the D leaves (partitioned build)
-            if (nD) {
-                fill_u32(ctx, F.p, filter_words(kh, FB), 0);
-                int wb = 0;
-                u32 R = 0;
-                bool done = false;
-                if (d_part && nD < ((u64)1 << 32) && dwin_geometry(kh, nD, wb, R)) {
-                    // partitioned like the I side (keys alongside), then moved into bucket windows
-                    // and counting-sorted one window per block in LDS
-                    IChunkLayout dl;
-                    const u64 slots = std::max<u64>(part_cap_slots(nD, kh, b->tune), nD);
-                    DBuf<u64> drec(ctx, slots), dkeys(ctx, slots);
-                    part_scatter_cap(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, kh, b->tune, dl,
-                                     drec.p, ovf.p, tpD, dkeys.p);
-                    if (host_value_u32(ctx, tape, ovf.p))
-                        part_scatter_exact(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, 0, nD, g.first - 1, kh, dl,
-                                           drec.p, tpD.p, dkeys.p);
-                    const u64 nw = (u64)pt.P() * R;
-                    DBuf<u32> wcount(ctx, nw);
-                    DBuf<u64> wreg(ctx, 2 * nw * dwin_capacity()), wbase(ctx, nw + 1);
-                    part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, dkeys.p, wcount.p, wreg.p, ovf.p);
-                    if (!host_value_u32(ctx, tape, ovf.p)) {
-                        scan_u32_to_u64_async(ctx, wcount.p, wbase.p, nw);
-                        part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, bstart.p, DR.p, T.p, F.p, FB,
-                                    kerr.p);
-                        done = true;
-                    }
-                }
-                if (!done) {
-                    // per-record global atomics (D sides of 2^32 records or more, or a window
-                    // overflowing on heavily repeated keys)
-                    fill_u32(ctx, (u32 *)T.p, pt.NB * CF_TABLE_SLOTS / 2, 0);
-                    DBuf<u32> cnt(ctx, pt.NB);
-                    fill_u32(ctx, cnt.p, pt.NB, 0);
-                    part_bucket_count(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, cnt.p);
-                    scan_u32_to_u64_async(ctx, cnt.p, bstart.p, pt.NB);
-                    fill_u32(ctx, cnt.p, pt.NB, 0);   // reused as the scatter cursors
-                    part_bucket_scatter(ctx, d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, g.first - 1, kh, bstart.p,
-                                        cnt.p, DR.p, T.p);
-                    part_filter(ctx, T.p, bstart.p, kh, FB, F.p);
-                }
+            std::vector<DTable> tabs(nD ? nr : 0);
+            for (u64 k = 0; k < tabs.size(); ++k) {
+                const u64 lo = k * m_rng;
+                build_dtable(ctx, d_desc.p, fd, g.first - 1, pt, FB, d_part, lo, std::min<u64>(m_rng, nD - lo), b->tune,
+                             tabs[k], ovf.p, kerr.p, tape);
             }
-            const DLeaves dleaves{d_desc.p, fd.nodes.p, fd.nn, fd.off.p, nD, tpD.p};
             if (!capturing) gc.mark();
-            // ---- I side: expand, then per chunk of leaves: partition exactly (count, scan,
-            // scatter), probe the table, check the candidates exactly
+            // ---- I side: expand, then per chunk of leaves: partition (one scatter for every table),
+            // probe each table, check the candidates exactly
             Frontier f = expand_to_last(ctx, i_desc.p, b->iroots.at(g.first).p, b->ni_roots.at(g.first), g.first, tape);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, n, icount.p);
             if (f.total && nD) {
@@ -512,34 +550,43 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     // chunk is laid out exactly and probed again
                     part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
                                      ovf.p, tp);
-                    part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
-                                     cand.p, b->tune);
-                    const u64 ov = host_value_u32(ctx, tape, ovf.p);
-                    u64 nc = host_value(ctx, tape, flags.p);
-                    if (ov) {
-                        part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                           rec.p, tp.p);
-                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
-                                         cand.p, b->tune);
-                        nc = host_value(ctx, tape, flags.p);
+                    for (size_t k = 0; k < tabs.size(); ++k) {
+                        const DTable &tb = tabs[k];
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
+                                         (unsigned long long *)flags.p, ccap, cand.p, b->tune);
+                        u64 nc;
+                        if (k == 0) {
+                            const u64 ov = host_value_u32(ctx, tape, ovf.p);
+                            nc = host_value(ctx, tape, flags.p);
+                            if (ov) {
+                                part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
+                                                   rec.p, tp.p);
+                                part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
+                                                 (unsigned long long *)flags.p, ccap, cand.p, b->tune);
+                                nc = host_value(ctx, tape, flags.p);
+                            }
+                        } else {
+                            nc = host_value(ctx, tape, flags.p);
+                        }
+                        while (nc > ccap) {
+                            // more candidates than room (heavily repeated keys): probe again with room for
+                            // all. The count includes the unused tails of partial reservations, and which
+                            // wave takes which tile varies from run to run. A wave leaves unused fewer slots
+                            // than the candidates of the round that makes it reserve again, plus at most one
+                            // partial reservation at its end, so a run needs at most 2 * (candidates) +
+                            // cand_margin slots, and nc >= candidates: one rerun suffices; the loop guards it
+                            ccap = 2 * nc + cand_margin(b->tune);
+                            cand = DBuf<u64>(ctx, ccap);
+                            part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
+                                             (unsigned long long *)flags.p, ccap, cand.p, b->tune);
+                            nc = host_value(ctx, tape, flags.p);
+                        }
+                        const DLeaves dleaves{d_desc.p, fd.nodes.p, fd.nn, fd.off.p, tb.lo, tb.n, tb.tp.p};
+                        part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
+                                         tp.p, dleaves, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n,
+                                         tb.bstart.p, tb.DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p,
+                                         out.pi.p, out.pinst.p, kerr.p);
                     }
-                    while (nc > ccap) {
-                        // more candidates than room (heavily repeated keys): probe again with room for
-                        // all. The count includes the unused tails of partial reservations, and which
-                        // wave takes which tile varies from run to run. A wave leaves unused fewer slots
-                        // than the candidates of the round that makes it reserve again, plus at most one
-                        // partial reservation at its end, so a run needs at most 2 * (candidates) +
-                        // cand_margin slots, and nc >= candidates: one rerun suffices; the loop guards it
-                        ccap = 2 * nc + cand_margin(b->tune);
-                        cand = DBuf<u64>(ctx, ccap);
-                        part_probe_pass1(ctx, kh, lay.view(), rec.p, T.p, F.p, FB, xq.p, (unsigned long long *)flags.p, ccap,
-                                         cand.p, b->tune);
-                        nc = host_value(ctx, tape, flags.p);
-                    }
-                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
-                                     tp.p, dleaves, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n, bstart.p,
-                                     DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p,
-                                     kerr.p);
                 }
             }
             if (!capturing) {
@@ -655,10 +702,12 @@ int cf_batch_footprint(cf_batch *b, uint64_t *bytes, uint64_t *nD_total, uint64_
         }
         sd += nD;
         si += nI;
-        const Partitioning pt = assign_buckets(nD, b->tune);
-        const u64 tables = 16 * nD + 30 * pt.NB;            // DR, T, bstart, counters, filter
-        // + partitioned D records and keys + D parents (kept through the I phase for pass 2)
-        const u64 d_build = tables + 16 * std::max<u64>(part_cap_slots(nD, pt.kh, b->tune), nD) + 32 * nD + 60 * nnD;
+        u64 nr = 1, m = nD;
+        d_ranges(nD, b->tune, true, nr, m);
+        const Partitioning pt = assign_buckets(m, b->tune);
+        const u64 tables = 16 * nD + 30 * pt.NB * nr;       // DR, T, bstart, counters, filter of every range
+        // + one range's partitioned D records and keys + D parents (kept through the I phase for pass 2)
+        const u64 d_build = tables + 16 * std::max<u64>(part_cap_slots(m, pt.kh, b->tune), m) + 32 * m + 60 * nnD;
         const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) +
                             60 * (nnI + nnD) + nI / 4 + nD / 128;
         peak = std::max(peak, std::max(d_build, i_phase));
@@ -749,7 +798,7 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
     CF_REQUIRE(b && out_d && out_i && out_inst && npairs, "NULL argument");
     *out_d = *out_i = *out_inst = nullptr;
     *npairs = 0;
-    b->ctx->peak_live = 0;
+    b->ctx->peak_live = b->ctx->live_bytes;
     BatchGraph *G = b->graph;
     if (G && (!max_chunk || max_chunk == G->chunk)) {
         // replay the captured solve: one graph launch, then the few results are read back
@@ -841,7 +890,7 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
     *out_d = nullptr;
     *out_i = nullptr;
     *npairs = 0;
-    ctx->peak_live = 0;
+    ctx->peak_live = ctx->live_bytes;
     CF_REQUIRE(mode != CF_SOLVE_SORTMERGE || inst->M_hi == 0, "the sort-merge mode needs M < 2^64");
     if (mode != CF_SOLVE_SORTMERGE) {
         cf_batch b;

@@ -114,3 +114,23 @@ def test_control_dropin_composition(name, posctl):
     best_n, best_fac, _, _ = min(kept)
     assert best_n == c.N and V.verify_certificate(best_fac)[0]
     cp.get_default_memory_pool().free_all_blocks()
+
+
+@pytest.mark.parametrize("mode", ["partition", "partition_dglobal"])
+def test_control_full_d_side_in_range_tables(posctl, mode):
+    """The validator's N_prev instance as one fused solve with the full D side (Dmin = 1: all C(64,7) =
+    621 216 192 records) and the unpruned 30-prime I side: the partitioned mode buckets the D side as
+    several range tables (each within the window build's reach) and probes every I record against each;
+    the global-atomic mode takes one table. Both must give the composition's pair set."""
+    c = I.control_instance("N_prev", nins=30)
+    base = c.inst
+    inst = I.Instance(M=base.M, r=base.r, B0=base.B0, INS=base.INS, P0=base.P0, Icap=prod(base.INS[-base.r:]) + 1,
+                      Dmin=1, split=None, hmax=base.r, tag="N_prev_full_D")
+    fx = posctl["compose"]["N_prev"]
+    say(f"full D side in range tables ({mode}): M={inst.M} r={inst.r} nD={fx['nD']} nI={fx['nI']}")
+    res = pipeline.solve(inst, mode=mode)
+    assert (res.nD, res.nI) == (fx["nD"], fx["nI"]) == (comb(64, 7), comb(30, 7))
+    d_dec, i_dec = pipeline.decoders(inst)
+    pairs = sorted([sorted(inst.B0[x] for x in d_dec(int(a))), sorted(inst.INS[x] for x in i_dec(int(b)))]
+                   for a, b in zip(res.pairs_d, res.pairs_i))
+    assert pairs == fx["pairs"]

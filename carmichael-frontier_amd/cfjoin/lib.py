@@ -42,7 +42,8 @@ class cf_instance(ctypes.Structure):
 class cf_stats(ctypes.Structure):
     _fields_ = [("nD", c_u64), ("nI", c_u64), ("npairs", c_u64), ("peak_bytes", c_u64),
                 ("ms_total", ctypes.c_double), ("ms_gen_d", ctypes.c_double), ("ms_sort_d", ctypes.c_double),
-                ("ms_gen_i", ctypes.c_double), ("ms_sort_i", ctypes.c_double), ("ms_join", ctypes.c_double)]
+                ("ms_gen_i", ctypes.c_double), ("ms_sort_i", ctypes.c_double), ("ms_join", ctypes.c_double),
+                ("nD_parents", c_u64), ("nI_parents", c_u64)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}

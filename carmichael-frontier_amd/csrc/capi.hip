@@ -425,7 +425,7 @@ int cf_gen_subsets(cf_ctx *ctx, const uint64_t *primes, int n, int r, int bits, 
     build_side_tables(ctx, s, t);
     DBuf<SideDesc> dd(ctx, 1);
     h2d(ctx, dd.p, &t.desc, 1);
-    launch_gen_rank(ctx, dd.p, lo, count, keys_dev, ids_dev);
+    launch_gen_prefix(ctx, dd.p, n, r, lo, count, keys_dev, ids_dev);
     CF_HIP(hipStreamSynchronize(ctx->stream));
     API_END
 }

@@ -243,6 +243,9 @@ void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *
 void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                        u64 leaf_lo, size_t nleaf, int depth, u64 *keys, u64 *ids);
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
+// the same records with runs sharing their first r - 1 indices computed once (one wave per 64
+// consecutive prefixes); rank range checked against C(n, r)
+void launch_gen_prefix(cf_ctx *ctx, const SideDesc *d_desc, int n, int r, u64 lo, size_t count, u64 *keys, u64 *ids);
 // per first index x: lo[x], hi[x] = leaf offsets where its nodes start and end (zeroed by the caller)
 void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi);
 u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous

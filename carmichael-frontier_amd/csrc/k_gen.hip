@@ -335,6 +335,79 @@ __global__ void __launch_bounds__(BLK) k_gen_rank(const SideDesc *__restrict__ d
     }
 }
 
+// Compatibility generator, prefix-shared. In lexicographic order the records of consecutive ranks
+// come in runs that share their first r - 1 indices while the last index counts up, so a wave takes
+// 64 consecutive prefixes (ranks pr0 + q among the (r - 1)-subsets of {0 .. n-2}): each lane
+// unranks its prefix and multiplies out its residue state and id once; the rank of its run's first
+// record is sum_pos C(n - start_pos, r - pos) - C(n - c_pos, r - pos) (hockey stick), and its run
+// is n - 1 - c_{r-2} records long. The wave then writes the records of [lo, lo + count) its runs
+// cover 64 at a time, contiguously: a lane finds the run of its record by a binary search over the
+// lanes' run starts and takes that lane's state with shuffles, then one residue step for the last
+// index. Every lane of the wave runs every step (no shuffle reads an inactive lane); only the stores
+// are masked.
+__global__ void __launch_bounds__(BLK) k_gen_prefix(const SideDesc *__restrict__ descs, u64 lo, u64 count, u64 pr0,
+                                                    u64 npre, u64 *__restrict__ keys, u64 *__restrict__ ids) {
+    const SideDesc &s = descs[0];
+    const int n = s.n, R = s.r, stride = R + 1, K = R - 1;
+    const int lane = threadIdx.x & 63;
+    const u64 hi = lo + count;
+    const u64 waves = (u64)gridDim.x * (BLK / 64);
+    for (u64 wv = (u64)blockIdx.x * (BLK / 64) + (threadIdx.x >> 6); wv * 64 < npre; wv += waves) {
+        const u64 q = wv * 64 + (u64)lane;
+        const bool has = q < npre;
+        u64 rank = pr0 + (has ? q : 0);
+        u64 xm = s.xm0, x2 = s.x20, id = 0, r0 = 0;
+        int start = 0;
+        for (int pos = 0; pos < K; ++pos) {
+            const int rem = K - 1 - pos;
+            int c = start;
+            for (; c < n - 2; ++c) {
+                const u64 cnt = s.binom[(size_t)(n - 2 - c) * stride + rem];
+                if (rank < cnt) break;
+                rank -= cnt;
+            }
+            r0 += s.binom[(size_t)(n - start) * stride + (R - pos)] - s.binom[(size_t)(n - c) * stride + (R - pos)];
+            start = c + 1;
+            xm = cf_mulmod_m(xm, s.pR[c], s.mc);
+            x2 *= s.p[c];
+            id |= (u64)c << (s.id_bits * pos);
+        }
+        // this lane's run clipped to [lo, hi), as output positions [sb, se); lanes past the last prefix
+        // start after every output
+        const u64 a = r0 > lo ? r0 : lo;
+        const u64 e0 = r0 + (u64)(n - start);
+        const u64 b = e0 < hi ? e0 : hi;
+        const u64 sb = has ? a - lo : ~0ull;
+        const u64 se = has ? b - lo : 0;
+        const u64 w_lo = (u64)__shfl((long long)sb, 0, 64);
+        u64 w_hi = se;
+        for (int d = 32; d >= 1; d >>= 1) {
+            const u64 o = (u64)__shfl_xor((long long)w_hi, d, 64);
+            w_hi = o > w_hi ? o : w_hi;
+        }
+        for (u64 o0 = w_lo; o0 < w_hi; o0 += 64) {
+            const u64 o = o0 + (u64)lane < w_hi ? o0 + (u64)lane : w_hi - 1;
+            int j = 0;
+            for (int d = 32; d >= 1; d >>= 1) {
+                const u64 sj = (u64)__shfl((long long)sb, j + d, 64);
+                if (sj <= o) j += d;
+            }
+            const u64 jxm = (u64)__shfl((long long)xm, j, 64);
+            const u64 jx2 = (u64)__shfl((long long)x2, j, 64);
+            const u64 jid = (u64)__shfl((long long)id, j, 64);
+            const u64 jr0 = (u64)__shfl((long long)r0, j, 64);
+            const int jst = __shfl(start, j, 64);
+            const int last = jst + (int)(o + lo - jr0);
+            const u64 kxm = cf_mulmod_m(jxm, s.pR[last], s.mc);
+            const u64 kx2 = jx2 * s.p[last];
+            if (o0 + (u64)lane < w_hi) {
+                keys[o] = cf_key(kxm, kx2, s.mc) + s.koff;
+                ids[o] = jid | ((u64)last << (s.id_bits * K));
+            }
+        }
+    }
+}
+
 inline unsigned grid_for(size_t n, size_t per_block, unsigned cap = 1u << 20) {
     size_t g = (n + per_block - 1) / per_block;
     if (g < 1) g = 1;
@@ -387,6 +460,55 @@ void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t 
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids) {
     if (!count) return;
     CF_LAUNCH(ctx, "k_gen_rank", k_gen_rank, dim3(grid_for(count, BLK)), dim3(BLK), 0, d_desc, lo, count, keys, ids);
+}
+
+namespace {
+typedef unsigned __int128 u128;
+u128 binom_exact(int a, int b) {
+    if (b < 0 || b > a) return 0;
+    u128 c = 1;
+    for (int j = 0; j < b; ++j) c = c * (u128)(a - j) / (u128)(j + 1);
+    return c;
+}
+// the k-subset of {0 .. m-1} of lexicographic rank `rank`
+void lex_unrank(u128 rank, int m, int k, int *out) {
+    int start = 0;
+    for (int pos = 0; pos < k; ++pos) {
+        int c = start;
+        for (;; ++c) {
+            const u128 cnt = binom_exact(m - 1 - c, k - 1 - pos);
+            if (rank < cnt) break;
+            rank -= cnt;
+        }
+        out[pos] = c;
+        start = c + 1;
+    }
+}
+// the lexicographic rank of the ascending k-subset c of {0 .. m-1}
+u128 lex_rank(const int *c, int m, int k) {
+    u128 r = 0;
+    int start = 0;
+    for (int pos = 0; pos < k; ++pos) {
+        r += binom_exact(m - start, k - pos) - binom_exact(m - c[pos], k - pos);
+        start = c[pos] + 1;
+    }
+    return r;
+}
+}  // namespace
+
+void launch_gen_prefix(cf_ctx *ctx, const SideDesc *d_desc, int n, int r, u64 lo, size_t count, u64 *keys, u64 *ids) {
+    if (!count) return;
+    CF_REQUIRE(r >= 1 && n >= r && n <= 1024 && r <= 16, "rank-range generation needs 1 <= r <= n <= 1024");
+    CF_REQUIRE((u128)lo + count <= binom_exact(n, r), "rank range beyond C(n, r)");
+    // the prefixes (first r - 1 indices, a subset of {0 .. n-2}) of the range's first and last records
+    int c0[16], c1[16];
+    lex_unrank(lo, n, r, c0);
+    lex_unrank((u128)lo + count - 1, n, r, c1);
+    const u64 pr0 = (u64)lex_rank(c0, n - 1, r - 1);
+    const u64 npre = (u64)(lex_rank(c1, n - 1, r - 1) - pr0 + 1);
+    const u64 waves = (npre + 63) / 64;
+    CF_LAUNCH(ctx, "k_gen_prefix", k_gen_prefix, dim3(grid_for(waves, BLK / 64, 1u << 16)), dim3(BLK), 0, d_desc, lo,
+              (u64)count, pr0, npre, keys, ids);
 }
 
 // ---- host: per-side tables ---------------------------------------------------------------

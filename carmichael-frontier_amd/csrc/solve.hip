@@ -464,6 +464,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
     DBuf<u32> &kerr = out.kerr;
     fill_u32(ctx, kerr.p, 1, 0);
     double ms_d = 0, ms_i = 0;
+    u64 parents_d = 0, parents_i = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         out.pd = DBuf<u64>(ctx, cap);
         out.pi = DBuf<u64>(ctx, cap);
@@ -472,6 +473,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         fill_u64(ctx, dcount.p, n, 0);
         fill_u64(ctx, icount.p, n, 0);
         ms_d = ms_i = 0;
+        parents_d = parents_i = 0;
         // each radius group: its D side is bucketed into its own probe table, then its I side is
         // generated, partitioned and probed against it. Eager: one group after another. Captured:
         // each group on its own branch (side stream), all forked from and joined back to the
@@ -527,6 +529,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             // probe each table, check the candidates exactly
             Frontier f = expand_to_last(ctx, i_desc.p, b->iroots.at(g.first).p, b->ni_roots.at(g.first), g.first, tape);
             if (f.total) inst_bounds(ctx, f.nodes.p, f.off.p, f.nn, n, icount.p);
+            parents_d += fd.nn;
+            parents_i += f.nn;
             if (f.total && nD) {
                 DBuf<u64> rec;
                 DBuf<u64> cand;
@@ -638,6 +642,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         st->ms_gen_i = ms_i;            // I expansion, partition scatter and probe
         st->ms_sort_i = 0;
         st->ms_join = 0;
+        st->nD_parents = parents_d;
+        st->nI_parents = parents_i;
     }
 }
 

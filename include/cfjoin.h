@@ -169,6 +169,7 @@ typedef struct cf_stats {
     uint64_t peak_bytes;      /* largest pool footprint during the call */
     double ms_total;          /* device time, first kernel to pairs ready */
     double ms_gen_d, ms_sort_d, ms_gen_i, ms_sort_i, ms_join;
+    uint64_t nD_parents, nI_parents;   /* last-level tree nodes (the leaves' parents) of each side */
 } cf_stats;
 
 /* mode: CF_SOLVE_PARTITION -- D records counting-sorted into hash buckets behind a fingerprint

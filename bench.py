@@ -573,7 +573,7 @@ def main():
               "pinned": True} if pinned else None
     if not args.no_tune and not pinned:
         from cfjoin import autotune
-        tr = autotune.tune(batch, reps=1)
+        tr = autotune.tune(batch, reps=3)
         tuning = {"setting": tr["best"], "ms": round(tr["ms"], 3), "default_ms": round(tr["default_ms"], 3),
                   "trials": len(tr["trials"])}
         print(f"[bench] tuned: {tr['best']} {tr['ms']:.2f} ms (default {tr['default_ms']:.2f} ms)", file=sys.stderr,

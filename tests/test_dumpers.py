@@ -145,25 +145,39 @@ def test_streams_equal_reference_dumpers(workloads, c):
     assert err.splitlines()[-1].startswith(f"TOTAL {c['nI']} records (r={c['r']}, |INS|={len(inst.INS)}, ")
 
 
+def threads_case():
+    """The threaded I-dump input of the test below (four threads, a stratum, 8-bit ids)."""
+    P = ins(120)
+    text = i_text(M0, 4, 123456789 % M0, prod(P[:4]) * 40, P, T=4)
+    return text, {"IDUMP_SPLIT": str(P[60]), "IDUMP_HMAX": "2", "IDUMP_RANK": "0"}
+
+
+def mask_cases():
+    """D dumps with 64-bit mask ids: by DDUMP_BITMASK at r = 5, and forced by r = 11."""
+    B = base()
+    return [(5, d_text(M0, 5, prod(B[-5:]) // 50, B), {"DDUMP_BITMASK": "1"}),
+            (11, d_text(M0, 11, prod(B[-11:]) // 4, B), {})]
+
+
 @pytest.mark.gpu
 def test_threaded_counts_and_masks_like_reference():
     """T > 1: the per-thread count lines and TOTAL equal the reference's (its records interleave by
-    thread, so the streams are compared as record sets); D ids as 64-bit masks, by env and for r > 10."""
+    thread, so the streams are compared as record sets); D ids as 64-bit masks, by env and for r > 10.
+    The reference programs' output is held in tests/golden/dumper_threads.json (written from the
+    compiled programs by tests/golden/make_dumper_threads.py), so the test needs no build of them."""
     import numpy as np
-    B = base()
-    P = ins(120)
-    text = i_text(M0, 4, 123456789 % M0, prod(P[:4]) * 40, P, T=4)
-    env = {"IDUMP_SPLIT": str(P[60]), "IDUMP_HMAX": "2", "IDUMP_RANK": "0"}
+    with open(os.path.join(GOLDEN, "dumper_threads.json")) as f:
+        fx = json.load(f)
+    text, env = threads_case()
     print("CASE dumpers threads", flush=True)
-    st, out, err = ref_run("mitm64_idump_stream2", text, env)
-    st2, out2, err2 = ours(DU.run_idump, text, env)
-    assert (st2, err2) == (st, err) and len(out2) == len(out) > 0
-    a = np.frombuffer(out, np.uint64).reshape(-1, 2)
-    b = np.frombuffer(out2, np.uint64).reshape(-1, 2)
-    assert np.array_equal(a[np.lexsort((a[:, 1], a[:, 0]))], b[np.lexsort((b[:, 1], b[:, 0]))])
-    for r, Dmin, env in [(5, prod(B[-5:]) // 50, {"DDUMP_BITMASK": "1"}), (11, prod(B[-11:]) // 4, {})]:
+    st, out, err = ours(DU.run_idump, text, env)
+    exp = fx["idump_threads"]
+    assert (st, err) == (exp["status"], exp["stderr"])
+    b = np.frombuffer(out, np.uint64).reshape(-1, 2)
+    b = np.ascontiguousarray(b[np.lexsort((b[:, 1], b[:, 0]))])
+    assert b.shape[0] == exp["records"] > 0 and _sha(b.tobytes()) == exp["sorted_sha"]
+    for (r, text, env), exp in zip(mask_cases(), fx["ddump_masks"]):
         print("CASE dumpers mask", r, flush=True)
-        text = d_text(M0, r, Dmin, B)
-        exp = ref_run("mitm64_ddump", text, env)
-        assert exp[0] == 0 and len(exp[1]) > 0
-        assert ours(DU.run_ddump, text, env) == exp
+        st, out, err = ours(DU.run_ddump, text, env)
+        assert exp["r"] == r and exp["bytes"] > 0
+        assert (st, err, len(out), _sha(out)) == (exp["status"], exp["stderr"], exp["bytes"], exp["sha"])

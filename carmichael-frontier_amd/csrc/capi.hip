@@ -440,6 +440,18 @@ int cf_sort_pairs(cf_ctx *ctx, uint64_t *keys_dev, uint64_t *ids_dev, uint64_t n
     API_END
 }
 
+int cf_sort_pairs_into(cf_ctx *ctx, const uint64_t *keys_dev, const uint64_t *ids_dev, uint64_t *keys_out_dev,
+                       uint64_t *ids_out_dev, uint64_t n) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "NULL context");
+    if (n == 0) return CF_OK;
+    CF_REQUIRE(keys_dev && ids_dev && keys_out_dev && ids_out_dev, "NULL array");
+    CF_REQUIRE((keys_dev == keys_out_dev) == (ids_dev == ids_out_dev), "sort both arrays in place or neither");
+    sort_pairs_into(ctx, keys_dev, ids_dev, keys_out_dev, ids_out_dev, n, bitlen(max_u64(ctx, keys_dev, n)));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
 int cf_join_sorted(cf_ctx *ctx, const uint64_t *Dk, const uint64_t *Did, uint64_t nD, const uint64_t *Ik,
                    const uint64_t *Iid, uint64_t nI, uint64_t **out_d, uint64_t **out_i, uint64_t *npairs) {
     API_BEGIN
@@ -448,11 +460,12 @@ int cf_join_sorted(cf_ctx *ctx, const uint64_t *Dk, const uint64_t *Did, uint64_
     *out_i = nullptr;
     *npairs = 0;
     if (nD == 0 || nI == 0) return CF_OK;
-    DBuf<u64> lo(ctx, nI), cnt(ctx, nI), off(ctx, nI + 1);
-    const u64 tot = join_sorted_count(ctx, Dk, nD, Ik, nI, lo.p, cnt.p, off.p);
+    const size_t ns = join_spans(nI);
+    DBuf<u64> scnt(ctx, ns), soff(ctx, ns + 1);
+    const u64 tot = join_sorted_count(ctx, Dk, nD, Ik, nI, scnt.p, soff.p);
     if (tot == 0) return CF_OK;
     DBuf<u64> od(ctx, tot), oi(ctx, tot);
-    join_sorted_write(ctx, Did, Iid, nI, lo.p, cnt.p, off.p, od.p, oi.p, nullptr, Ik);
+    join_sorted_write(ctx, Dk, nD, Ik, Did, Iid, nI, soff.p, od.p, oi.p, nullptr);
     CF_HIP(hipStreamSynchronize(ctx->stream));
     *out_d = od.release();
     *out_i = oi.release();

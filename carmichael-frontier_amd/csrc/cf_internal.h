@@ -230,6 +230,8 @@ void scan_u32_to_u64_async(cf_ctx *ctx, const u32 *in, u64 *out, size_t n);   //
 
 // stable LSD radix sort of (key, val) by the low `bits` bits of key; in-place via scratch
 void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits);
+// the same stable sort from (kin, vin) into (kout, vout); both pairs may be one
+void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, size_t n, int bits);
 
 // tree expansion
 void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, int depth, u32 *cnt);
@@ -369,6 +371,9 @@ void wide_check_pairs(cf_ctx *ctx, const u64 *pd, const u64 *pi, const u64 *pj, 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
 
 // joins
-u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off);
-void join_sorted_write(cf_ctx *ctx, const u64 *Did, const u64 *Iid, size_t nI, const u64 *lo, const u64 *cnt,
-                       const u64 *off, u64 *od, u64 *oi, u64 *okey, const u64 *Ik);
+// equal-key join of key-sorted sides: pairs per span of I rows (scnt[join_spans(nI)]), their scan
+// (soff, one more word), then the pairs; returns the pair total
+size_t join_spans(size_t nI);
+u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *scnt, u64 *soff);
+void join_sorted_write(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, const u64 *Did, const u64 *Iid, size_t nI,
+                       const u64 *soff, u64 *od, u64 *oi, u64 *okey);

@@ -1,26 +1,29 @@
 // k_join.hip -- equal-key join of two key-sorted record sides on gfx950.
 //
 // Semantics of gpu/mitm_gpu2.py:107-119 (join_sorted): for every I row, in I order, the run
-// [lo, hi) of D rows with an equal key, emitted in D order. count -> scan -> write, so the
-// output order is exactly the reference's (I-major, then D position).
+// [lo, hi) of D rows with an equal key, emitted in D order. Pairs are counted per span of 512 I rows,
+// the span counts scanned, and each span with pairs recomputes its rows' runs and writes them from
+// its offset, so the output order is exactly the reference's (I-major, then D position) and no
+// per-row array is stored.
 #include <map>
 #include <mutex>
 
 #include "cf_internal.h"
+#include "tree_dev.h"
 
 namespace {
 constexpr int BLK = 256;
+constexpr int JW = 8;            // 64-key chunks per span: a span is 512 consecutive I rows, one wave's
+constexpr u64 SPAN = 64 * JW;
 
-__device__ __forceinline__ u64 lower_bound_u64(const u64 *__restrict__ a, u64 n, u64 v) {
-    u64 lo = 0, hi = n;
+__device__ __forceinline__ u64 lower_bound_in(const u64 *__restrict__ a, u64 lo, u64 hi, u64 v) {
     while (lo < hi) {
         const u64 mid = (lo + hi) >> 1;
         if (a[mid] < v) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
-__device__ __forceinline__ u64 upper_bound_from(const u64 *__restrict__ a, u64 lo, u64 n, u64 v) {
-    u64 hi = n;
+__device__ __forceinline__ u64 upper_bound_in(const u64 *__restrict__ a, u64 lo, u64 hi, u64 v) {
     while (lo < hi) {
         const u64 mid = (lo + hi) >> 1;
         if (a[mid] <= v) lo = mid + 1; else hi = mid;
@@ -28,60 +31,140 @@ __device__ __forceinline__ u64 upper_bound_from(const u64 *__restrict__ a, u64 l
     return lo;
 }
 
+// The D run of each key of one 64-row chunk (lane = row; `valid` marks rows inside the I side, lane
+// 0 always one): lb = first D row with key >= k, cnt = rows with key == k. Called by whole waves.
+// Ascending chunks (the sorted I side) find the first key's bound by a 64-ary search (one probe per
+// lane per step, 6 bits per step), load the 64 D keys from there, and place every lane's key in that
+// window by a 6-step search over lane shuffles; a lane whose key lies beyond the window searches the
+// rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order gets
+// exact bounds.
+__device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, bool valid, u64 &lb, u64 &cnt) {
+    const int lane = threadIdx.x & 63;
+    const u64 k0 = __shfl(k, 0, 64);
+    if (!valid) k = k0;
+    const u64 prev = __shfl(k, lane ? lane - 1 : 0, 64);
+    if (!__all(!valid || prev <= k)) {
+        lb = lower_bound_in(Dk, 0, nD, k);
+        cnt = valid ? upper_bound_in(Dk, lb, nD, k) - lb : 0;
+        return;
+    }
+    // invariant: D rows below lo hold keys < k0, rows at or above hi keys >= k0
+    u64 lo = 0, hi = nD;
+    while (hi - lo > 64) {
+        const u64 step = (hi - lo + 63) >> 6;
+        const u64 p = lo + (u64)lane * step;
+        const int c = __popcll(__ballot(p < hi && Dk[p] < k0));
+        if (c == 0) {
+            hi = lo;
+            break;
+        }
+        const u64 last = lo + (u64)(c - 1) * step;   // the last probe below k0
+        lo = last + 1;
+        if (last + step < hi) hi = last + step;      // the first probe at or above k0
+    }
+    if (hi > lo) lo += __popcll(__ballot(lo + lane < hi && Dk[lo + lane] < k0));
+    const u64 L0 = lo;
+    const u64 navail = nD - L0 < 64 ? nD - L0 : 64;
+    const u64 W = (u64)lane < navail ? Dk[L0 + lane] : 0;
+    int a = 0, b = 0;                                 // window keys < k, and <= k
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        const u64 wa = __shfl(W, a + s - 1, 64);
+        const u64 wb = __shfl(W, b + s - 1, 64);
+        if ((u64)(a + s - 1) < navail && wa < k) a += s;
+        if ((u64)(b + s - 1) < navail && wb <= k) b += s;
+    }
+    const u64 w63 = __shfl(W, 63, 64);
+    if (navail == 64 && a == 63 && w63 < k) a = 64;
+    if (navail == 64 && b == 63 && w63 <= k) b = 64;
+    lb = a < 64 ? L0 + a : lower_bound_in(Dk, L0 + 64, nD, k);
+    const u64 ub = b < 64 ? L0 + b : upper_bound_in(Dk, lb > L0 + 64 ? lb : L0 + 64, nD, k);
+    cnt = valid ? ub - lb : 0;
+}
+
+// Pairs per span: span s covers I rows [s * SPAN, (s + 1) * SPAN); one wave per span.
 __global__ void __launch_bounds__(BLK) k_join_count(const u64 *__restrict__ Dk, u64 nD, const u64 *__restrict__ Ik, u64 nI,
-                                                    u64 *__restrict__ lo, u64 *__restrict__ cnt) {
-    for (u64 i = (u64)blockIdx.x * BLK + threadIdx.x; i < nI; i += (u64)gridDim.x * BLK) {
-        const u64 k = Ik[i];
-        const u64 l = lower_bound_u64(Dk, nD, k);
-        u64 c = 0;
-        if (l < nD && Dk[l] == k) c = upper_bound_from(Dk, l, nD, k) - l;
-        lo[i] = l;
-        cnt[i] = c;
+                                                    u64 nspans, u64 *__restrict__ scnt) {
+    const int lane = threadIdx.x & 63;
+    const u64 w0 = ((u64)blockIdx.x * BLK + threadIdx.x) >> 6, nw = ((u64)gridDim.x * BLK) >> 6;
+    for (u64 sp = w0; sp < nspans; sp += nw) {
+        u64 tot = 0;
+        for (int j = 0; j < JW; ++j) {
+            const u64 i0 = sp * SPAN + (u64)j * 64;
+            if (i0 >= nI) break;
+            const u64 i = i0 + lane;
+            u64 lb, c;
+            chunk_bounds(Dk, nD, i < nI ? Ik[i] : 0, i < nI, lb, c);
+            tot += c;
+        }
+        tot = wave_incl_sum64(tot);
+        if (lane == 63) scnt[sp] = tot;
     }
 }
 
-// Writes nothing when the join's total (off[nI]) exceeds cap.
-__global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Did, const u64 *__restrict__ Iid, u64 nI,
-                                                    const u64 *__restrict__ lo, const u64 *__restrict__ cnt,
-                                                    const u64 *__restrict__ off, u64 *__restrict__ od,
-                                                    u64 *__restrict__ oi, u64 *__restrict__ okey,
-                                                    const u64 *__restrict__ Ik, u64 cap) {
-    if (off[nI] > cap) return;
-    for (u64 i = (u64)blockIdx.x * BLK + threadIdx.x; i < nI; i += (u64)gridDim.x * BLK) {
-        const u64 c = cnt[i];
-        if (!c) continue;
-        const u64 o = off[i], l = lo[i], iid = Iid[i];
-        for (u64 q = 0; q < c; ++q) {
-            od[o + q] = Did[l + q];
-            oi[o + q] = iid;
-            if (okey) okey[o + q] = Ik[i];
+// Writes span s's pairs from soff[s] on, in I order and D order within a row; spans without pairs
+// read nothing but their offsets. Writes nothing when the join's total (soff[nspans]) exceeds cap.
+__global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, u64 nD, const u64 *__restrict__ Ik,
+                                                    const u64 *__restrict__ Did, const u64 *__restrict__ Iid, u64 nI,
+                                                    u64 nspans, const u64 *__restrict__ soff, u64 *__restrict__ od,
+                                                    u64 *__restrict__ oi, u64 *__restrict__ okey, u64 cap) {
+    if (soff[nspans] > cap) return;
+    const int lane = threadIdx.x & 63;
+    const u64 w0 = ((u64)blockIdx.x * BLK + threadIdx.x) >> 6, nw = ((u64)gridDim.x * BLK) >> 6;
+    for (u64 sp = w0; sp < nspans; sp += nw) {
+        u64 base = soff[sp];
+        if (soff[sp + 1] == base) continue;
+        for (int j = 0; j < JW; ++j) {
+            const u64 i0 = sp * SPAN + (u64)j * 64;
+            if (i0 >= nI) break;
+            const u64 i = i0 + lane;
+            const bool valid = i < nI;
+            const u64 k = valid ? Ik[i] : 0;
+            u64 lb, c;
+            chunk_bounds(Dk, nD, k, valid, lb, c);
+            const u64 incl = wave_incl_sum64(c);
+            const u64 o = base + incl - c;
+            if (c) {
+                const u64 iid = Iid[i];
+                for (u64 q = 0; q < c; ++q) {
+                    od[o + q] = Did[lb + q];
+                    oi[o + q] = iid;
+                    if (okey) okey[o + q] = k;
+                }
+            }
+            base += __shfl(incl, 63, 64);
         }
     }
 }
 
-inline unsigned grid_for(u64 n) {
-    u64 g = (n + BLK - 1) / BLK;
+inline unsigned grid_for_spans(u64 nspans) {
+    u64 g = (nspans + BLK / 64 - 1) / (BLK / 64);
     if (g < 1) g = 1;
-    return (unsigned)(g > (1u << 20) ? (1u << 20) : g);
+    return (unsigned)(g > (1u << 18) ? (1u << 18) : g);
 }
 }  // namespace
 
-u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *lo, u64 *cnt, u64 *off) {
-    if (!nI) return 0;
-    CF_LAUNCH(ctx, "k_join_count", k_join_count, dim3(grid_for(nI)), dim3(BLK), 0, Dk, (u64)nD, Ik, (u64)nI, lo, cnt);
-    return scan_u64(ctx, cnt, off, nI);
+size_t join_spans(size_t nI) { return (nI + SPAN - 1) / SPAN; }
+
+u64 join_sorted_count(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, size_t nI, u64 *scnt, u64 *soff) {
+    if (!nI || !nD) return 0;
+    const u64 ns = join_spans(nI);
+    CF_LAUNCH(ctx, "k_join_count", k_join_count, dim3(grid_for_spans(ns)), dim3(BLK), 0, Dk, (u64)nD, Ik, (u64)nI,
+              ns, scnt);
+    return scan_u64(ctx, scnt, soff, ns);
 }
 
-void join_sorted_write(cf_ctx *ctx, const u64 *Did, const u64 *Iid, size_t nI, const u64 *lo, const u64 *cnt,
-                       const u64 *off, u64 *od, u64 *oi, u64 *okey, const u64 *Ik) {
-    if (!nI) return;
-    CF_LAUNCH(ctx, "k_join_write", k_join_write, dim3(grid_for(nI)), dim3(BLK), 0, Did, Iid, (u64)nI, lo, cnt, off, od,
-                       oi, okey, Ik, ~0ULL);
+void join_sorted_write(cf_ctx *ctx, const u64 *Dk, size_t nD, const u64 *Ik, const u64 *Did, const u64 *Iid, size_t nI,
+                       const u64 *soff, u64 *od, u64 *oi, u64 *okey) {
+    if (!nI || !nD) return;
+    const u64 ns = join_spans(nI);
+    CF_LAUNCH(ctx, "k_join_write", k_join_write, dim3(grid_for_spans(ns)), dim3(BLK), 0, Dk, (u64)nD, Ik, Did, Iid,
+              (u64)nI, ns, soff, od, oi, okey, ~0ULL);
 }
 
 // ---- the join without allocation ---------------------------------------------------------------
-// cf_join_sorted_noalloc keeps its per-row run starts, counts and offsets, and the scan's block
-// partials, in this file-scope array: one copy per device, resident while the library is loaded.
+// cf_join_sorted_noalloc keeps its per-span pair counts and offsets, and the scan's block partials,
+// in this file-scope array: one copy per device, resident while the library is loaded.
 // In host code `scratch` names a host-side stand-in, not device memory: its device address comes
 // from hipGetSymbolAddress, once per device (join_scratch_bind, called when a context is created),
 // and only that address is passed to kernels.
@@ -90,7 +173,8 @@ constexpr u64 NOALLOC_MAX_I = (u64)CF_JOIN_NOALLOC_MAX_I;
 constexpr size_t scan_words_bound(size_t n) {   // scan_ws_words, evaluated at compile time
     return (n + 2047) / 2048 <= 1 ? 0 : 2 * ((n + 2047) / 2048) + 1 + scan_words_bound((n + 2047) / 2048);
 }
-constexpr size_t SCRATCH_WORDS = 3 * NOALLOC_MAX_I + 1 + scan_words_bound(NOALLOC_MAX_I);
+constexpr size_t NOALLOC_SPANS = (NOALLOC_MAX_I + SPAN - 1) / SPAN;
+constexpr size_t SCRATCH_WORDS = 2 * NOALLOC_SPANS + 1 + scan_words_bound(NOALLOC_SPANS);
 static __device__ __align__(256) u64 scratch[SCRATCH_WORDS];
 
 std::mutex g_scratch_mu;
@@ -122,13 +206,14 @@ extern "C" int cf_join_sorted_noalloc(cf_ctx *ctx, const uint64_t *Dk, const uin
     static_assert(SCRATCH_WORDS * sizeof(u64) <= (64u << 20), "the join scratch stays modest");
     *npairs = 0;
     if (nD == 0 || nI == 0) return CF_OK;
-    u64 *lo = ctx->join_scratch, *cnt = lo + NOALLOC_MAX_I, *off = cnt + NOALLOC_MAX_I;
-    u64 *ws = off + NOALLOC_MAX_I + 1;
-    CF_LAUNCH(ctx, "k_join_count", k_join_count, dim3(grid_for(nI)), dim3(BLK), 0, Dk, (u64)nD, Ik, (u64)nI, lo, cnt);
-    scan_u64_ws(ctx, cnt, off, nI, ws);
-    CF_LAUNCH(ctx, "k_join_write", k_join_write, dim3(grid_for(nI)), dim3(BLK), 0, Did, Iid, (u64)nI, lo, cnt, off,
-              out_d, out_i, (u64 *)nullptr, Ik, (u64)cap);
-    const u64 tot = d2h_u64(ctx, off + nI);
+    const u64 ns = join_spans(nI);
+    u64 *scnt = ctx->join_scratch, *soff = scnt + NOALLOC_SPANS, *ws = soff + NOALLOC_SPANS + 1;
+    CF_LAUNCH(ctx, "k_join_count", k_join_count, dim3(grid_for_spans(ns)), dim3(BLK), 0, Dk, (u64)nD, Ik, (u64)nI, ns,
+              scnt);
+    scan_u64_ws(ctx, scnt, soff, ns, ws);
+    CF_LAUNCH(ctx, "k_join_write", k_join_write, dim3(grid_for_spans(ns)), dim3(BLK), 0, Dk, (u64)nD, Ik, Did, Iid,
+              (u64)nI, ns, (const u64 *)soff, out_d, out_i, (u64 *)nullptr, (u64)cap);
+    const u64 tot = d2h_u64(ctx, soff + ns);
     *npairs = tot;
     CF_REQUIRE(tot <= cap, "the output capacity is below the pair count (*npairs); nothing was written");
     API_END

@@ -1,9 +1,12 @@
 // k_sort.hip -- exclusive scans and a stable LSD radix sort of (key, id) records on gfx950.
 //
-// Radix sort: 8-bit digits over the significant bits of the keys only. Each pass is
+// Radix passes: 8-bit digits over the significant bits of the keys only. Each pass is
 //   histogram (per 2048-key tile, LDS atomics)  ->  exclusive scan in digit-major order
 //   ->  scatter, where each tile ranks its keys stably with 64-lane ballots (8 ballots give the
 //   lanes holding the same digit), stages them digit-grouped in LDS, and writes contiguous runs.
+// The sort takes the top bits off in one or two such passes (most significant first, the second
+// within each of the first's buckets) and finishes every bucket in LDS; a plain LSD sort covers
+// what that cannot take.
 // Stability matches the stable argsort of gpu/mitm_gpu2.py:101-104.
 #include "tree_dev.h"
 
@@ -119,27 +122,73 @@ void scan_impl(cf_ctx *ctx, const T *in, u64 *out, size_t n) {
     CF_LAUNCH(ctx, "k_scan_final", (k_scan_final<T>), dim3((unsigned)nb), dim3(SB), 0, in, n, (const u64 *)part_off.p, out);
 }
 
-// ---- radix sort -----------------------------------------------------------------------------
-__global__ void __launch_bounds__(RB) k_radix_hist(const u64 *__restrict__ keys, size_t n, int shift,
-                                                   u32 *__restrict__ hist, size_t ntiles) {
+// ---- radix passes ---------------------------------------------------------------------------
+// A pass splits its keys into tiles of at most RT keys; tile t's count of digit d sits at
+// c0 + d * cs in the digit-major count array, so one exclusive scan of that array gives every
+// (tile, digit) its output offset. Flat: tiles of the whole array. Segmented: tiles of each segment
+// (a run of keys equal in their higher digits, as a previous pass left them), laid out segment by
+// segment, so the scan keeps each segment inside its own range.
+struct FlatGeo {
+    size_t n, ntiles;
+    __device__ bool tile(u64 t, u64 &base, int &count, u64 &c0, u64 &cs) const {
+        base = t * RT;
+        count = (int)((n - base) < (size_t)RT ? (n - base) : (size_t)RT);
+        c0 = t;
+        cs = ntiles;
+        return true;
+    }
+};
+struct SegGeo {
+    const u64 *sstart, *send;
+    const u32 *sfirst, *sn;   // first tile and tile count of each segment; sfirst[S] = all tiles
+    int S;
+    __device__ bool tile(u64 t, u64 &base, int &count, u64 &c0, u64 &cs) const {
+        if (t >= sfirst[S]) {   // beyond the last tile: zero counts at c0 + d * cs
+            c0 = t * RADIX;
+            cs = 1;
+            return false;
+        }
+        int lo = 0, hi = S - 1;  // the last segment whose first tile is at or below t (it has tiles)
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sfirst[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const u64 F = sfirst[lo], local = t - F;
+        base = sstart[lo] + local * RT;
+        const u64 left = send[lo] - base;
+        count = (int)(left < (u64)RT ? left : (u64)RT);
+        c0 = (u64)RADIX * F + local;
+        cs = sn[lo];
+        return true;
+    }
+};
+
+template <class G>
+__global__ void __launch_bounds__(RB) k_radix_hist(const u64 *__restrict__ keys, int shift, u32 *__restrict__ hist,
+                                                   size_t ntiles, G g) {
     __shared__ u32 h[RADIX];
     for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        u64 base, c0, cs;
+        int tcount = 0;
+        const bool live = g.tile(tile, base, tcount, c0, cs);
         h[threadIdx.x] = 0;
         __syncthreads();
-        const size_t base = tile * RT;
+        if (live) {
 #pragma unroll
-        for (int k = 0; k < RI; ++k) {
-            const size_t e = base + (size_t)k * RB + threadIdx.x;
-            if (e < n) atomicAdd(&h[(keys[e] >> shift) & (RADIX - 1)], 1u);
+            for (int k = 0; k < RI; ++k) {
+                const int le = k * RB + threadIdx.x;
+                if (le < tcount) atomicAdd(&h[(keys[base + le] >> shift) & (RADIX - 1)], 1u);
+            }
         }
         __syncthreads();
-        hist[(size_t)threadIdx.x * ntiles + tile] = h[threadIdx.x];
+        hist[c0 + (u64)threadIdx.x * cs] = h[threadIdx.x];
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(RB) k_radix_scatter(const u64 *__restrict__ kin, const u64 *__restrict__ vin, size_t n,
-                                                      int shift, const u64 *__restrict__ off, size_t ntiles,
+template <class G>
+__global__ void __launch_bounds__(RB) k_radix_scatter(const u64 *__restrict__ kin, const u64 *__restrict__ vin,
+                                                      int shift, const u64 *__restrict__ off, size_t ntiles, G g,
                                                       u64 *__restrict__ kout, u64 *__restrict__ vout) {
     __shared__ u32 s_cnt[RADIX];
     __shared__ u32 s_wc[RB / 64][RADIX];
@@ -150,9 +199,10 @@ __global__ void __launch_bounds__(RB) k_radix_scatter(const u64 *__restrict__ ki
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const u64 lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
     for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const size_t base = tile * RT;
-        const int tcount = (int)((n - base) < (size_t)RT ? (n - base) : (size_t)RT);
-        s_goff[threadIdx.x] = off[(size_t)threadIdx.x * ntiles + tile];
+        u64 base, c0, cs;
+        int tcount = 0;
+        if (!g.tile(tile, base, tcount, c0, cs)) continue;   // uniform over the block
+        s_goff[threadIdx.x] = off[c0 + (u64)threadIdx.x * cs];
         s_cnt[threadIdx.x] = 0;
         u64 k[RI], v[RI];
         u32 rk[RI];
@@ -219,6 +269,169 @@ __global__ void __launch_bounds__(RB) k_radix_scatter(const u64 *__restrict__ ki
     }
 }
 
+// The segments of the next pass: segment b = the keys whose previous-pass digit is b (offA in the
+// flat pass's layout, ntA tiles), or the whole array when offA is null (S = 1).
+__global__ void __launch_bounds__(SB) k_seg_table(const u64 *__restrict__ offA, size_t ntA, int S, size_t n,
+                                                  u64 *__restrict__ sstart, u64 *__restrict__ send,
+                                                  u32 *__restrict__ sfirst, u32 *__restrict__ sn) {
+    const int b = threadIdx.x;
+    u64 st = 0, en = 0;
+    if (b < S) {
+        st = offA ? offA[(size_t)b * ntA] : 0;
+        en = offA ? offA[(size_t)(b + 1) * ntA] : n;
+    }
+    const u64 nt = (en - st + RT - 1) / RT;
+    u64 tot;
+    const u64 f = block_excl_scan(nt, &tot);
+    if (b < S) {
+        sstart[b] = st;
+        send[b] = en;
+        sfirst[b] = (u32)f;
+        sn[b] = (u32)nt;
+    }
+    if (b == 0) sfirst[S] = (u32)tot;
+}
+
+// ---- per-bucket sort in LDS ------------------------------------------------------------------
+// After the MSD passes, bucket q holds the keys whose top m bits equal q, in input order. One
+// block sorts a bucket of at most BCAP keys by its remaining `lowbits` bits: each key is packed
+// with its position in the bucket as (low bits << BIB) | position, the packed words go through
+// 8-bit LSD passes in LDS, and the position then gathers the id. Wave w owns positions
+// [w * R * 64, (w + 1) * R * 64): its lanes rank equal digits with ballots and keep a running count
+// per digit across their rounds, so one block-wide scan per pass orders the waves and the result
+// is stable. A bucket above BCAP raises *flag and is copied through unsorted.
+constexpr int BT = 512;
+constexpr int BW = BT / 64;
+constexpr int BI = 16;
+constexpr int BCAP = BT * BI;   // 8192
+constexpr int BIB = 13;         // 2^BIB = BCAP
+static_assert((1 << BIB) == BCAP, "positions fill BIB bits");
+
+__global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
+                                                    const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+                                                    const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
+                                                    unsigned long long *__restrict__ flag) {
+    __shared__ u64 s_v[BCAP];
+    __shared__ u32 s_wc[BW][RADIX];
+    __shared__ u32 s_ws[BW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const u32 q = blockIdx.x;
+    u64 start = 0, end = n;
+    if (off) {
+        const u32 b1 = q >> m2, d2 = q & ((1u << m2) - 1);
+        const u64 F = sfirst[b1], N = sn[b1];
+        if (N == 0) return;
+        start = off[(u64)RADIX * F + (u64)d2 * N];
+        end = off[(u64)RADIX * F + (u64)(d2 + 1) * N];
+    }
+    const u64 len = end - start;
+    if (len == 0) return;
+    if (len > (u64)BCAP) {
+        if (tid == 0) atomicOr(flag, 1ULL);
+        if (kin != kout)
+            for (u64 e = tid; e < len; e += BT) {
+                kout[start + e] = kin[start + e];
+                vout[start + e] = vin[start + e];
+            }
+        return;
+    }
+    const int R = (int)((len + BT - 1) / BT);
+    const u64 wbase = (u64)w * R * 64;
+    const u64 lowmask = lowbits ? (~0ULL >> (64 - lowbits)) : 0;
+    const u64 lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
+    u64 v[BI];
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u64 e = wbase + (u64)it * 64 + lane;
+        v[it] = (it < R && e < len) ? (((kin[start + e] & lowmask) << BIB) | e) : 0;
+    }
+    const int passes = (lowbits + 7) / 8;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int sh = BIB + 8 * ps;
+        for (int x = tid; x < BW * RADIX; x += BT) (&s_wc[0][0])[x] = 0;
+        __syncthreads();
+        u32 rk[BI];
+#pragma unroll
+        for (int it = 0; it < BI; ++it) {
+            rk[it] = 0;
+            if (it < R) {
+                const u64 e = wbase + (u64)it * 64 + lane;
+                const bool valid = e < len;
+                const u32 d = (u32)(v[it] >> sh) & (RADIX - 1);
+                u64 peers = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const u64 bb = __ballot(bit);
+                    peers &= bit ? bb : ~bb;
+                }
+                const u32 rw = (u32)__popcll(peers & lt), cw = (u32)__popcll(peers);
+                // every lane of the wave reads the digit's running count before its last holder adds
+                // the round's run: LDS operations of one wave complete in issue order
+                const u32 run = valid ? s_wc[w][d] : 0;
+                rk[it] = run + rw;
+                __builtin_amdgcn_wave_barrier();
+                if (valid && rw == cw - 1) s_wc[w][d] = run + cw;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+        // digit d: the waves' counts become their offsets within the digit, then the digit's base
+        u32 tot = 0;
+        if (tid < RADIX) {
+#pragma unroll
+            for (int x = 0; x < BW; ++x) {
+                const u32 c = s_wc[x][tid];
+                s_wc[x][tid] = tot;
+                tot += c;
+            }
+        }
+        const u32 inc = wave_incl_sum(tot);
+        if (lane == 63) s_ws[w] = inc;
+        __syncthreads();
+        if (tid < RADIX) {
+            u32 base = inc - tot;
+            for (int x = 0; x < w; ++x) base += s_ws[x];
+#pragma unroll
+            for (int x = 0; x < BW; ++x) s_wc[x][tid] += base;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < BI; ++it) {
+            if (it < R && wbase + (u64)it * 64 + lane < len) {
+                const u32 d = (u32)(v[it] >> sh) & (RADIX - 1);
+                s_v[s_wc[w][d] + rk[it]] = v[it];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < BI; ++it) {
+            const u64 e = wbase + (u64)it * 64 + lane;
+            if (it < R && e < len) v[it] = s_v[e];
+        }
+        __syncthreads();
+    }
+    u64 idv[BI];
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u64 e = wbase + (u64)it * 64 + lane;
+        idv[it] = 0;
+        if (it < R && e < len) {
+            idv[it] = vin[start + (v[it] & (BCAP - 1))];
+            v[it] = ((u64)q << lowbits) | (v[it] >> BIB);
+        }
+    }
+    __syncthreads();   // every id is read before any is written (vin and vout may be one array)
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u64 e = wbase + (u64)it * 64 + lane;
+        if (it < R && e < len) {
+            kout[start + e] = v[it];
+            vout[start + e] = idv[it];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(SB) k_max(const u64 *__restrict__ a, size_t n, unsigned long long *__restrict__ out) {
     u64 m = 0;
     for (size_t e = (size_t)blockIdx.x * SB + threadIdx.x; e < n; e += (size_t)gridDim.x * SB) m = a[e] > m ? a[e] : m;
@@ -274,7 +487,9 @@ u64 scan_u64(cf_ctx *ctx, const u64 *in, u64 *out, size_t n) {
 
 void scan_u32_to_u64_async(cf_ctx *ctx, const u32 *in, u64 *out, size_t n) { scan_impl<u32>(ctx, in, out, n); }
 
-void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
+namespace {
+// Stable LSD radix sort in place: one flat pass per 8 bits of `bits`.
+void lsd_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
     if (n <= 1 || bits <= 0) return;
     const int passes = (bits + 7) / 8;
     const size_t ntiles = (n + RT - 1) / RT;
@@ -283,11 +498,13 @@ void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
     DBuf<u64> off(ctx, ntiles * RADIX + 1);
     u64 *ka = keys, *va = vals, *kb = k2.p, *vb = v2.p;
     const unsigned grid = (unsigned)(ntiles < 65536 ? ntiles : 65536);
+    const FlatGeo g{n, ntiles};
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = ps * 8;
-        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist, dim3(grid), dim3(RB), 0, ka, n, shift, hist.p, ntiles);
+        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<FlatGeo>, dim3(grid), dim3(RB), 0, ka, shift, hist.p, ntiles, g);
         scan_impl<u32>(ctx, hist.p, off.p, ntiles * RADIX);
-        CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter, dim3(grid), dim3(RB), 0, ka, va, n, shift, off.p, ntiles, kb, vb);
+        CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter<FlatGeo>, dim3(grid), dim3(RB), 0, ka, va, shift, off.p,
+                  ntiles, g, kb, vb);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -295,4 +512,103 @@ void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
         CF_HIP(hipMemcpyAsync(keys, ka, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
         CF_HIP(hipMemcpyAsync(vals, va, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
     }
+}
+
+constexpr size_t MSD_MEAN = 4096;   // mean bucket the MSD bits aim at (half of BCAP)
+constexpr int MSD_MAX_BITS = 16;    // two passes of at most 8 bits
+}  // namespace
+
+// Stable sort of (kin, vin) by key into (kout, vout) (which may be the inputs themselves), for keys
+// below 2^bits. The top m bits (mean bucket at most MSD_MEAN) are split off by one or two stable
+// MSD passes, the second segmented by the first's digit, and each bucket is then sorted in LDS by
+// k_bucket_sort: three passes over HBM. Stability makes the result the same as the LSD sort's, which
+// takes over when the MSD bits would exceed two passes or a bucket overflows the LDS.
+void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, size_t n, int bits) {
+    const bool inplace = kin == kout;
+    CF_REQUIRE(inplace == (vin == vout), "keys and ids are both sorted in place or both into new arrays");
+    if (!inplace && n) {
+        if (n == 1 || bits <= 0) {
+            CF_HIP(hipMemcpyAsync(kout, kin, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(vout, vin, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+            return;
+        }
+    }
+    if (n <= 1 || bits <= 0) return;
+    int m = 0;
+    while ((n >> m) > MSD_MEAN) ++m;
+    if (m > MSD_MAX_BITS && (n >> MSD_MAX_BITS) <= MSD_MEAN + MSD_MEAN / 2) m = MSD_MAX_BITS;
+    m = std::max(m, bits + BIB - 64);   // the low bits and a position fit one word
+    m = std::min(m, bits);
+    if (m > MSD_MAX_BITS) {
+        if (!inplace) {
+            CF_HIP(hipMemcpyAsync(kout, kin, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(vout, vin, n * sizeof(u64), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        lsd_sort_pairs(ctx, kout, vout, n, bits);
+        return;
+    }
+    const int m2 = std::min(m, 8), m1 = m - m2, S = 1 << m1;
+    const int lowbits = bits - m;
+    DBuf<u64> flag(ctx, 1);
+    CF_HIP(hipMemsetAsync(flag.p, 0, sizeof(u64), ctx->stream));
+    const u64 *bk = kin, *bv = vin;   // the bucket sort's input
+    DBuf<u64> sstart(ctx, S), send(ctx, S);
+    DBuf<u32> sfirst(ctx, S + 1), sn(ctx, S);
+    size_t ntB = 0;
+    DBuf<u64> offB;
+    if (m > 0) {
+        const bool need_tmp = m1 > 0 || inplace;
+        DBuf<u64> tk, tv;
+        if (need_tmp) {
+            tk = DBuf<u64>(ctx, n);
+            tv = DBuf<u64>(ctx, n);
+        }
+        const u64 *ak = kin, *av = vin;   // pass B's input
+        DBuf<u64> offA;
+        size_t ntA = 0;
+        if (m1 > 0) {   // pass A: flat, top m1 bits, into the scratch pair
+            ntA = (n + RT - 1) / RT;
+            DBuf<u32> histA(ctx, ntA * RADIX);
+            offA = DBuf<u64>(ctx, ntA * RADIX + 1);
+            const unsigned grid = (unsigned)(ntA < 65536 ? ntA : 65536);
+            const FlatGeo g{n, ntA};
+            CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<FlatGeo>, dim3(grid), dim3(RB), 0, kin, bits - m1, histA.p,
+                      ntA, g);
+            scan_impl<u32>(ctx, histA.p, offA.p, ntA * RADIX);
+            CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter<FlatGeo>, dim3(grid), dim3(RB), 0, kin, vin, bits - m1,
+                      (const u64 *)offA.p, ntA, g, tk.p, tv.p);
+            ak = tk.p;
+            av = tv.p;
+        }
+        CF_LAUNCH(ctx, "k_seg_table", k_seg_table, dim3(1), dim3(SB), 0, (const u64 *)(m1 > 0 ? offA.p : nullptr), ntA,
+                  S, n, sstart.p, send.p, sfirst.p, sn.p);
+        // pass B: segmented, the next m2 bits; into the outputs, or the scratch pair when sorting in
+        // place with one MSD pass (pass B cannot write over its own input)
+        ntB = (n + RT - 1) / RT + (size_t)S;
+        u64 *ok = kout, *ov = vout;
+        if (m1 == 0 && inplace) {
+            ok = tk.p;
+            ov = tv.p;
+        }
+        DBuf<u32> histB(ctx, ntB * RADIX);
+        offB = DBuf<u64>(ctx, ntB * RADIX + 1);
+        const unsigned grid = (unsigned)(ntB < 65536 ? ntB : 65536);
+        const SegGeo g{sstart.p, send.p, sfirst.p, sn.p, S};
+        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<SegGeo>, dim3(grid), dim3(RB), 0, ak, lowbits, histB.p, ntB, g);
+        scan_impl<u32>(ctx, histB.p, offB.p, ntB * RADIX);
+        CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter<SegGeo>, dim3(grid), dim3(RB), 0, ak, av, lowbits,
+                  (const u64 *)offB.p, ntB, g, ok, ov);
+        bk = ok;
+        bv = ov;
+        CF_LAUNCH(ctx, "k_bucket_sort", k_bucket_sort, dim3(1u << m), dim3(BT), 0, bk, bv, kout, vout,
+                  (const u64 *)offB.p, (const u32 *)sfirst.p, (const u32 *)sn.p, (u64)n, m2, lowbits, (unsigned long long *)flag.p);
+    } else {
+        CF_LAUNCH(ctx, "k_bucket_sort", k_bucket_sort, dim3(1), dim3(BT), 0, bk, bv, kout, vout, (const u64 *)nullptr,
+                  (const u32 *)nullptr, (const u32 *)nullptr, (u64)n, 0, lowbits, (unsigned long long *)flag.p);
+    }
+    if (d2h_u64(ctx, flag.p)) lsd_sort_pairs(ctx, kout, vout, n, bits);
+}
+
+void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
+    sort_pairs_into(ctx, keys, vals, keys, vals, n, bits);
 }

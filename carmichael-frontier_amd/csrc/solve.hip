@@ -940,7 +940,7 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
     if (nI && nD) {
         const u64 chunk = max_chunk ? max_chunk : (u64)1 << 28;
         const u64 cs = std::min<u64>(chunk, nI);
-        DBuf<u64> Ik(ctx, cs), Iid(ctx, cs), lo(ctx, cs), cnt(ctx, cs), off(ctx, cs + 1);
+        DBuf<u64> Ik(ctx, cs), Iid(ctx, cs), scnt(ctx, join_spans(cs)), soff(ctx, join_spans(cs) + 1);
         hipEvent_t a, b, c, d;
         CF_HIP(hipEventCreate(&a));
         CF_HIP(hipEventCreate(&b));
@@ -953,10 +953,10 @@ int cf_solve(cf_ctx *ctx, const cf_instance *inst, int mode, uint64_t max_chunk,
             CF_HIP(hipEventRecord(b, ctx->stream));
             radix_sort_pairs(ctx, Ik.p, Iid.p, m, kbits);
             CF_HIP(hipEventRecord(c, ctx->stream));
-            const u64 tot = join_sorted_count(ctx, Dk.p, nD, Ik.p, m, lo.p, cnt.p, off.p);
+            const u64 tot = join_sorted_count(ctx, Dk.p, nD, Ik.p, m, scnt.p, soff.p);
             if (tot) {
                 DBuf<u64> od(ctx, tot), oi(ctx, tot);
-                join_sorted_write(ctx, Did.p, Iid.p, m, lo.p, cnt.p, off.p, od.p, oi.p, nullptr, Ik.p);
+                join_sorted_write(ctx, Dk.p, nD, Ik.p, Did.p, Iid.p, m, soff.p, od.p, oi.p, nullptr);
                 parts.emplace_back(std::move(od), std::move(oi));
                 total_pairs += tot;
             }

@@ -47,11 +47,12 @@ def generate(primes, r: int, M: int, mult: int = 1, bits: int = 10, lo: int = 0,
 
 def sort_side(keys, ids):
     """Sort one side by key (stable). Returns new (sorted_keys, sorted_ids); inputs are unchanged."""
-    k = keys.copy()
-    i = ids.copy()
-    if k.size > 1:
-        _lib.check(_lib.load().cf_sort_pairs(k.ctx.handle, _lib.c_void_p(k.ptr), _lib.c_void_p(i.ptr), k.size),
-                   "sort_side")
+    if keys.size <= 1:
+        return keys.copy(), ids.copy()
+    k = cp.ndarray(keys.size, keys.dtype, ctx=keys.ctx)
+    i = cp.ndarray(ids.size, ids.dtype, ctx=keys.ctx)
+    _lib.check(_lib.load().cf_sort_pairs_into(keys.ctx.handle, _lib.c_void_p(keys.ptr), _lib.c_void_p(ids.ptr),
+                                              _lib.c_void_p(k.ptr), _lib.c_void_p(i.ptr), keys.size), "sort_side")
     return k, i
 
 

@@ -3,6 +3,7 @@
 Run on an MI355X:  python -u -m pytest tests -m gpu -x -v --timeout 120 --timeout-method thread
 Each test prints the case it is about to run (instance, sizes) before touching the device.
 """
+import ctypes
 import itertools
 import os
 from math import comb, prod
@@ -144,7 +145,10 @@ def test_plan_deletion_bitmask_and_edges():
 
 
 # ---- sort and join ------------------------------------------------------------------------
-@pytest.mark.parametrize("n,M", [(1, 97), (1000, 97), (100000, 1 << 40), (300001, 1768248177696000), (70000, (1 << 64) - 59)])
+# 3e6 keys take two MSD passes before the per-bucket sort; 1e5 keys below 7 fill buckets past the
+# LDS capacity, so the LSD sort finishes them
+@pytest.mark.parametrize("n,M", [(1, 97), (1000, 97), (100000, 1 << 40), (300001, 1768248177696000), (70000, (1 << 64) - 59),
+                                 (3000000, 1768248177696000), (100000, 7), (5000, 1 << 52)])
 def test_sort_side_stable(n, M):
     import mitm_gpu2 as G
     rng = np.random.default_rng(n)
@@ -156,13 +160,31 @@ def test_sort_side_stable(n, M):
     assert np.array_equal(ks.get(), k[o]) and np.array_equal(is_.get(), i[o])
 
 
+@pytest.mark.parametrize("n,M", [(3000000, 1 << 51), (100000, 1 << 40), (100000, 7), (4097, 1 << 63)])
+def test_sort_pairs_in_place(n, M):
+    """cf_sort_pairs (in place) against the stable argsort: with one MSD pass the in-place sort
+    goes through a scratch pair, with two it returns to the input arrays."""
+    rng = np.random.default_rng(n + 1)
+    k = rng.integers(0, M, n, dtype=np.uint64)
+    i = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    say("sort in place", n, M)
+    dk, di = dev(k), dev(i)
+    L.check(L.load().cf_sort_pairs(dk.ctx.handle, ctypes.c_void_p(dk.ptr), ctypes.c_void_p(di.ptr), n), "sort")
+    o = np.argsort(k, kind="stable")
+    assert np.array_equal(dk.get(), k[o]) and np.array_equal(di.get(), i[o])
+
+
 def test_join_sorted_matches_oracle():
     import mitm_gpu2 as G
     rng = np.random.default_rng(5)
-    for nD, nI, M in [(5000, 20000, 3000), (100, 100, 50), (200000, 300000, 1 << 30), (1, 1, 2)]:
-        say("join", nD, nI, M)
+    # the last case leaves the I keys unsorted: each thread's bound then also moves back
+    for nD, nI, M, sort_i in [(5000, 20000, 3000, True), (100, 100, 50, True), (200000, 300000, 1 << 30, True),
+                              (1, 1, 2, True), (20, 7, 9, True), (4000, 9001, 2500, False)]:
+        say("join", nD, nI, M, sort_i)
         Dk = np.sort(rng.integers(0, M, nD, dtype=np.uint64))
-        Ik = np.sort(rng.integers(0, M, nI, dtype=np.uint64))
+        Ik = rng.integers(0, M, nI, dtype=np.uint64)
+        if sort_i:
+            Ik = np.sort(Ik)
         Did = rng.integers(0, 1 << 60, nD, dtype=np.uint64)
         Iid = rng.integers(0, 1 << 60, nI, dtype=np.uint64)
         pd, pi = G.join_sorted(dev(Dk), dev(Did), dev(Ik), dev(Iid))
@@ -426,6 +448,27 @@ def test_server_results_equal_alone(workloads):
         lg = x.meta.get("logged")
         if lg:
             assert (r.nD, r.nI, len(r.pairs_d)) == (lg["nD"], lg["nI"], lg["pairs"]), x.tag
+
+
+def test_server_tags_are_unique_until_released():
+    """A tag names one request from its submission until its result is released: a second
+    submission under it is refused while the first is queued or held (its result's arrays stay
+    valid), and the tag is free again once released."""
+    from cfjoin import serve
+    inst = I.c1_instance(0, 100)
+    say("serve tags")
+    srv = serve.Server(workers=1, max_batch=4)
+    try:
+        srv.submit(inst, 5)
+        with pytest.raises(AssertionError, match="tag already in use"):
+            srv.submit(inst, 5)
+        r = srv.next(30000)              # copies the pairs and releases the tag
+        assert r is not None and r.status == 0 and r.tag == 5
+        srv.submit(inst, 5)
+        r2 = srv.next(30000)
+        assert r2 is not None and r2.tag == 5 and np.array_equal(r2.pair_array(), r.pair_array())
+    finally:
+        srv.close()
 
 
 def test_r8_anchor_d_filtered_matches_logged(workloads):

@@ -294,37 +294,71 @@ __global__ void __launch_bounds__(SB) k_seg_table(const u64 *__restrict__ offA, 
 
 // ---- per-bucket sort in LDS ------------------------------------------------------------------
 // After the MSD passes, bucket q holds the keys whose top m bits equal q, in input order. One
-// block sorts a bucket of at most BCAP keys by its remaining `lowbits` bits: each key is packed
-// with its position in the bucket as (low bits << BIB) | position, the packed words go through
-// 8-bit LSD passes in LDS, and the position then gathers the id. Wave w owns positions
-// [w * R * 64, (w + 1) * R * 64): its lanes rank equal digits with ballots and keep a running count
-// per digit across their rounds, so one block-wide scan per pass orders the waves and the result
-// is stable. A bucket above BCAP raises *flag and is copied through unsorted.
+// block sorts a bucket of at most BCAP keys by its remaining `lowbits` bits. Each key is packed
+// with its position in the bucket as (low bits << BIB) | position: the packed words are distinct
+// and their order is the stable order, so any exact sort of them is the stable sort, and the
+// position then gathers the id.
+//   k_bucket_count: the top GB bits of the low bits pick a group; LDS atomics give each word a
+//   slot, a scan of the group counts places the groups, and each word's rank inside its group is
+//   the number of smaller words there (groups hold a few words when the keys spread). A bucket with
+//   a group above GMAX words (keys bunched on a few values) is marked and left to
+//   k_bucket_lsd: 8-bit LSD passes in LDS, where wave w owns positions [w * R * 64, (w + 1) * R * 64),
+//   its lanes rank equal digits with ballots and keep a running count per digit across their
+//   rounds, and one block scan per pass orders the waves.
+// A bucket above BCAP sets bit 0 of *flag and is copied through unsorted; a marked bucket sets bit 1.
 constexpr int BT = 512;
 constexpr int BW = BT / 64;
 constexpr int BI = 16;
 constexpr int BCAP = BT * BI;   // 8192
 constexpr int BIB = 13;         // 2^BIB = BCAP
+constexpr int GB = 11;          // group bits of the counting pass
+constexpr int NG = 1 << GB;
+constexpr u32 GMAX = 64;
 static_assert((1 << BIB) == BCAP, "positions fill BIB bits");
+static_assert(NG % BT == 0, "each thread scans NG / BT group counts");
 
-__global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
-                                                    const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
-                                                    const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
-                                                    unsigned long long *__restrict__ flag) {
+// Bucket q's range [start, start + len) in the MSD output (the whole array when off is null).
+__device__ __forceinline__ void bucket_range(u32 q, const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+                                             const u32 *__restrict__ sn, u64 n, int m2, u64 &start, u64 &len) {
+    start = 0;
+    len = n;
+    if (!off) return;
+    const u32 b1 = q >> m2, d2 = q & ((1u << m2) - 1);
+    const u64 F = sfirst[b1], N = sn[b1];
+    if (N == 0) {
+        len = 0;
+        return;
+    }
+    start = off[(u64)RADIX * F + (u64)d2 * N];
+    len = off[(u64)RADIX * F + (u64)(d2 + 1) * N] - start;
+}
+
+__global__ void __launch_bounds__(SB) k_bucket_max(const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+                                                   const u32 *__restrict__ sn, u32 nb, int m2,
+                                                   unsigned long long *__restrict__ out) {
+    const u32 q = blockIdx.x * SB + threadIdx.x;
+    u64 start = 0, len = 0;
+    if (q < nb) bucket_range(q, off, sfirst, sn, 0, m2, start, len);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const u64 o = __shfl_xor(len, d, 64);
+        len = o > len ? o : len;
+    }
+    if ((threadIdx.x & 63) == 0 && len) atomicMax(out, (unsigned long long)len);
+}
+
+__global__ void __launch_bounds__(BT) k_bucket_count(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
+                                                     const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+                                                     const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
+                                                     unsigned long long *__restrict__ flag, u32 *__restrict__ mark) {
     __shared__ u64 s_v[BCAP];
-    __shared__ u32 s_wc[BW][RADIX];
+    __shared__ u32 s_c[NG + 1];   // group counts, then group starts
     __shared__ u32 s_ws[BW];
+    __shared__ u32 s_big;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const u32 q = blockIdx.x;
-    u64 start = 0, end = n;
-    if (off) {
-        const u32 b1 = q >> m2, d2 = q & ((1u << m2) - 1);
-        const u64 F = sfirst[b1], N = sn[b1];
-        if (N == 0) return;
-        start = off[(u64)RADIX * F + (u64)d2 * N];
-        end = off[(u64)RADIX * F + (u64)(d2 + 1) * N];
-    }
-    const u64 len = end - start;
+    u64 start, len;
+    bucket_range(q, off, sfirst, sn, n, m2, start, len);
     if (len == 0) return;
     if (len > (u64)BCAP) {
         if (tid == 0) atomicOr(flag, 1ULL);
@@ -335,15 +369,130 @@ __global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *v
             }
         return;
     }
-    const int R = (int)((len + BT - 1) / BT);
-    const u64 wbase = (u64)w * R * 64;
+    // 32-bit positions against bucket-based pointers
+    const u32 ln = (u32)len;
+    const u64 *kb = kin + start, *vb = vin + start;
+    u64 *kob = kout + start, *vob = vout + start;
+    const int R = (int)((ln + BT - 1) / BT);
+    const u32 wbase = (u32)(w * R * 64);
+    const u64 lowmask = lowbits ? (~0ULL >> (64 - lowbits)) : 0;
+    const int gb = lowbits < GB ? lowbits : GB;
+    const int gsh = BIB + lowbits - gb;
+    const u32 gmask = (1u << gb) - 1;
+    for (int x = tid; x <= NG; x += BT) s_c[x] = 0;
+    if (tid == 0) s_big = 0;
+    __syncthreads();
+    u64 v[BI];
+    u32 slot[BI];
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        if (it < R && e < ln) {
+            v[it] = ((kb[e] & lowmask) << BIB) | e;
+            slot[it] = atomicAdd(&s_c[(u32)(v[it] >> gsh) & gmask], 1u);
+        }
+    }
+    __syncthreads();
+    {   // group starts: thread t scans groups [t * NG / BT, (t + 1) * NG / BT)
+        constexpr int PER = NG / BT;
+        u32 c[PER], sum = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            c[j] = s_c[tid * PER + j];
+            sum += c[j];
+        }
+        const u32 inc = wave_incl_sum(sum);
+        if (lane == 63) s_ws[w] = inc;
+        __syncthreads();
+        u32 pre = inc - sum;
+        for (int x = 0; x < w; ++x) pre += s_ws[x];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            s_c[tid * PER + j] = pre;
+            pre += c[j];
+        }
+        if (tid == BT - 1) s_c[NG] = pre;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < BI; ++it)
+        if (it < R && wbase + (u32)(it * 64 + lane) < ln) s_v[s_c[(u32)(v[it] >> gsh) & gmask] + slot[it]] = v[it];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        if (it < R && wbase + (u32)(it * 64 + lane) < ln) {
+            const u32 g = (u32)(v[it] >> gsh) & gmask, g0 = s_c[g], g1 = s_c[g + 1];
+            if (g1 - g0 > GMAX) {
+                s_big = 1;
+            } else {
+                u32 rank = 0;
+                for (u32 j = g0; j < g1; ++j) rank += s_v[j] < v[it];
+                slot[it] = g0 + rank;
+            }
+        }
+    }
+    __syncthreads();
+    if (s_big) {
+        if (tid == 0) {
+            mark[q] = 1;
+            atomicOr(flag, 2ULL);
+        }
+        return;
+    }
+#pragma unroll
+    for (int it = 0; it < BI; ++it)
+        if (it < R && wbase + (u32)(it * 64 + lane) < ln) s_v[slot[it]] = v[it];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        if (it < R && e < ln) {
+            const u64 x = s_v[e];
+            slot[it] = (u32)x & (BCAP - 1);
+            v[it] = ((u64)q << lowbits) | (x >> BIB);
+        }
+    }
+    u64 idv[BI];
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        if (it < R && e < ln) idv[it] = vb[slot[it]];
+    }
+    __syncthreads();   // every id is read before any is written (vin and vout may be one array)
+#pragma unroll
+    for (int it = 0; it < BI; ++it) {
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        if (it < R && e < ln) {
+            kob[e] = v[it];
+            vob[e] = idv[it];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_bucket_lsd(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
+                                                   const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+                                                   const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
+                                                   const u32 *__restrict__ mark) {
+    __shared__ u64 s_v[BCAP];
+    __shared__ u32 s_wc[BW][RADIX];
+    __shared__ u32 s_ws[BW];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const u32 q = blockIdx.x;
+    if (!mark[q]) return;
+    u64 start, len;
+    bucket_range(q, off, sfirst, sn, n, m2, start, len);
+    const u32 ln = (u32)len;   // marked buckets hold at most BCAP keys
+    const u64 *kb = kin + start, *vb = vin + start;
+    u64 *kob = kout + start, *vob = vout + start;
+    const int R = (int)((ln + BT - 1) / BT);
+    const u32 wbase = (u32)(w * R * 64);
     const u64 lowmask = lowbits ? (~0ULL >> (64 - lowbits)) : 0;
     const u64 lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
     u64 v[BI];
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
-        const u64 e = wbase + (u64)it * 64 + lane;
-        v[it] = (it < R && e < len) ? (((kin[start + e] & lowmask) << BIB) | e) : 0;
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        v[it] = (it < R && e < ln) ? (((kb[e] & lowmask) << BIB) | e) : 0;
     }
     const int passes = (lowbits + 7) / 8;
     for (int ps = 0; ps < passes; ++ps) {
@@ -355,8 +504,7 @@ __global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *v
         for (int it = 0; it < BI; ++it) {
             rk[it] = 0;
             if (it < R) {
-                const u64 e = wbase + (u64)it * 64 + lane;
-                const bool valid = e < len;
+                const bool valid = wbase + (u32)(it * 64 + lane) < ln;
                 const u32 d = (u32)(v[it] >> sh) & (RADIX - 1);
                 u64 peers = __ballot(valid);
 #pragma unroll
@@ -398,7 +546,7 @@ __global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *v
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < BI; ++it) {
-            if (it < R && wbase + (u64)it * 64 + lane < len) {
+            if (it < R && wbase + (u32)(it * 64 + lane) < ln) {
                 const u32 d = (u32)(v[it] >> sh) & (RADIX - 1);
                 s_v[s_wc[w][d] + rk[it]] = v[it];
             }
@@ -406,28 +554,29 @@ __global__ void __launch_bounds__(BT) k_bucket_sort(const u64 *kin, const u64 *v
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < BI; ++it) {
-            const u64 e = wbase + (u64)it * 64 + lane;
-            if (it < R && e < len) v[it] = s_v[e];
+            const u32 e = wbase + (u32)(it * 64 + lane);
+            if (it < R && e < ln) v[it] = s_v[e];
         }
         __syncthreads();
     }
+    // with no pass every key of the bucket is equal and the input order is the order
     u64 idv[BI];
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
-        const u64 e = wbase + (u64)it * 64 + lane;
+        const u32 e = wbase + (u32)(it * 64 + lane);
         idv[it] = 0;
-        if (it < R && e < len) {
-            idv[it] = vin[start + (v[it] & (BCAP - 1))];
+        if (it < R && e < ln) {
+            idv[it] = vb[(u32)v[it] & (BCAP - 1)];
             v[it] = ((u64)q << lowbits) | (v[it] >> BIB);
         }
     }
     __syncthreads();   // every id is read before any is written (vin and vout may be one array)
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
-        const u64 e = wbase + (u64)it * 64 + lane;
-        if (it < R && e < len) {
-            kout[start + e] = v[it];
-            vout[start + e] = idv[it];
+        const u32 e = wbase + (u32)(it * 64 + lane);
+        if (it < R && e < ln) {
+            kob[e] = v[it];
+            vob[e] = idv[it];
         }
     }
 }
@@ -515,13 +664,37 @@ void lsd_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
 }
 
 constexpr size_t MSD_MEAN = 4096;   // mean bucket the MSD bits aim at (half of BCAP)
+
+// The largest of the 2^m buckets a segmented pass's offsets describe.
+u64 max_bucket(cf_ctx *ctx, const u64 *off, const u32 *sfirst, const u32 *sn, int m, int m2) {
+    DBuf<u64> mx(ctx, 1);
+    CF_HIP(hipMemsetAsync(mx.p, 0, sizeof(u64), ctx->stream));
+    const u32 nb = 1u << m;
+    CF_LAUNCH(ctx, "k_bucket_max", k_bucket_max, dim3((nb + SB - 1) / SB), dim3(SB), 0, off, sfirst, sn, nb, m2,
+              (unsigned long long *)mx.p);
+    return d2h_u64(ctx, mx.p);
+}
+
+// The 2^m buckets sorted from (kin, vin) into (kout, vout); *flag gets bit 0 for a bucket above
+// BCAP (copied through). Buckets the counting pass marks are finished by the LSD kernel here.
+void bucket_sort(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, const u64 *off, const u32 *sfirst,
+                 const u32 *sn, size_t n, int m, int m2, int lowbits, u64 *flag) {
+    const unsigned nb = 1u << m;
+    DBuf<u32> mark(ctx, nb);
+    CF_HIP(hipMemsetAsync(mark.p, 0, nb * sizeof(u32), ctx->stream));
+    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn, (u64)n,
+              m2, lowbits, (unsigned long long *)flag, mark.p);
+    if (d2h_u64(ctx, flag) & 2)
+        CF_LAUNCH(ctx, "k_bucket_lsd", k_bucket_lsd, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn,
+                  (u64)n, m2, lowbits, (const u32 *)mark.p);
+}
 constexpr int MSD_MAX_BITS = 16;    // two passes of at most 8 bits
 }  // namespace
 
 // Stable sort of (kin, vin) by key into (kout, vout) (which may be the inputs themselves), for keys
 // below 2^bits. The top m bits (mean bucket at most MSD_MEAN) are split off by one or two stable
 // MSD passes, the second segmented by the first's digit, and each bucket is then sorted in LDS by
-// k_bucket_sort: three passes over HBM. Stability makes the result the same as the LSD sort's, which
+// k_bucket_count: three passes over HBM. Stability makes the result the same as the LSD sort's, which
 // takes over when the MSD bits would exceed two passes or a bucket overflows the LDS.
 void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, size_t n, int bits) {
     const bool inplace = kin == kout;
@@ -547,19 +720,21 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
         lsd_sort_pairs(ctx, kout, vout, n, bits);
         return;
     }
-    const int m2 = std::min(m, 8), m1 = m - m2, S = 1 << m1;
-    const int lowbits = bits - m;
     DBuf<u64> flag(ctx, 1);
     CF_HIP(hipMemsetAsync(flag.p, 0, sizeof(u64), ctx->stream));
-    const u64 *bk = kin, *bv = vin;   // the bucket sort's input
-    DBuf<u64> sstart(ctx, S), send(ctx, S);
-    DBuf<u32> sfirst(ctx, S + 1), sn(ctx, S);
-    size_t ntB = 0;
-    DBuf<u64> offB;
-    if (m > 0) {
-        const bool need_tmp = m1 > 0 || inplace;
+    if (m == 0) {   // one bucket of at most MSD_MEAN keys
+        bucket_sort(ctx, kin, vin, kout, vout, nullptr, nullptr, nullptr, n, 0, 0, bits, flag.p);
+        if (d2h_u64(ctx, flag.p) & 1) lsd_sort_pairs(ctx, kout, vout, n, bits);
+        return;
+    }
+    const int mcap = std::min(MSD_MAX_BITS, bits);
+    for (int attempt = 0;; ++attempt) {
+        const int m2 = std::min(m, 8), m1 = m - m2, S = 1 << m1;
+        const int lowbits = bits - m;
+        DBuf<u64> sstart(ctx, S), send(ctx, S);
+        DBuf<u32> sfirst(ctx, S + 1), sn(ctx, S);
         DBuf<u64> tk, tv;
-        if (need_tmp) {
+        if (m1 > 0 || inplace) {
             tk = DBuf<u64>(ctx, n);
             tv = DBuf<u64>(ctx, n);
         }
@@ -584,29 +759,34 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
                   S, n, sstart.p, send.p, sfirst.p, sn.p);
         // pass B: segmented, the next m2 bits; into the outputs, or the scratch pair when sorting in
         // place with one MSD pass (pass B cannot write over its own input)
-        ntB = (n + RT - 1) / RT + (size_t)S;
+        const size_t ntB = (n + RT - 1) / RT + (size_t)S;
+        DBuf<u32> histB(ctx, ntB * RADIX);
+        DBuf<u64> offB(ctx, ntB * RADIX + 1);
+        const unsigned grid = (unsigned)(ntB < 65536 ? ntB : 65536);
+        const SegGeo g{sstart.p, send.p, sfirst.p, sn.p, S};
+        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<SegGeo>, dim3(grid), dim3(RB), 0, ak, lowbits, histB.p, ntB, g);
+        scan_impl<u32>(ctx, histB.p, offB.p, ntB * RADIX);
+        if (m < mcap && attempt < 2) {
+            // keys bunched at this depth (a bucket past the LDS): take more top bits and start again
+            const u64 big = max_bucket(ctx, offB.p, sfirst.p, sn.p, m, m2);
+            if (big > (u64)BCAP) {
+                int grow = 1;
+                while ((big >> grow) > MSD_MEAN) ++grow;
+                m = std::min(mcap, m + grow);
+                continue;
+            }
+        }
         u64 *ok = kout, *ov = vout;
         if (m1 == 0 && inplace) {
             ok = tk.p;
             ov = tv.p;
         }
-        DBuf<u32> histB(ctx, ntB * RADIX);
-        offB = DBuf<u64>(ctx, ntB * RADIX + 1);
-        const unsigned grid = (unsigned)(ntB < 65536 ? ntB : 65536);
-        const SegGeo g{sstart.p, send.p, sfirst.p, sn.p, S};
-        CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<SegGeo>, dim3(grid), dim3(RB), 0, ak, lowbits, histB.p, ntB, g);
-        scan_impl<u32>(ctx, histB.p, offB.p, ntB * RADIX);
         CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter<SegGeo>, dim3(grid), dim3(RB), 0, ak, av, lowbits,
                   (const u64 *)offB.p, ntB, g, ok, ov);
-        bk = ok;
-        bv = ov;
-        CF_LAUNCH(ctx, "k_bucket_sort", k_bucket_sort, dim3(1u << m), dim3(BT), 0, bk, bv, kout, vout,
-                  (const u64 *)offB.p, (const u32 *)sfirst.p, (const u32 *)sn.p, (u64)n, m2, lowbits, (unsigned long long *)flag.p);
-    } else {
-        CF_LAUNCH(ctx, "k_bucket_sort", k_bucket_sort, dim3(1), dim3(BT), 0, bk, bv, kout, vout, (const u64 *)nullptr,
-                  (const u32 *)nullptr, (const u32 *)nullptr, (u64)n, 0, lowbits, (unsigned long long *)flag.p);
+        bucket_sort(ctx, ok, ov, kout, vout, offB.p, sfirst.p, sn.p, n, m, m2, lowbits, flag.p);
+        break;
     }
-    if (d2h_u64(ctx, flag.p)) lsd_sort_pairs(ctx, kout, vout, n, bits);
+    if (d2h_u64(ctx, flag.p) & 1) lsd_sort_pairs(ctx, kout, vout, n, bits);
 }
 
 void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {

@@ -160,6 +160,35 @@ def test_sort_side_stable(n, M):
     assert np.array_equal(ks.get(), k[o]) and np.array_equal(is_.get(), i[o])
 
 
+def _bunched(kind, rng):
+    if kind == "half_in_one_range":     # half the keys below 2^40 of a 51-bit range
+        k = rng.integers(0, 1 << 51, 300000, dtype=np.uint64)
+        k[::2] = rng.integers(0, 1 << 40, 150000, dtype=np.uint64)
+        return k
+    if kind == "values_repeated":       # 1000 values, each 100 times
+        return np.repeat(rng.integers(0, 1 << 40, 1000, dtype=np.uint64), 100)[rng.permutation(100000)]
+    if kind == "all_equal":
+        return np.full(50000, 123456789, np.uint64)
+    if kind == "two_values":
+        return rng.integers(0, 2, 200000, dtype=np.uint64) * np.uint64(1 << 45)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["half_in_one_range", "values_repeated", "all_equal", "two_values"])
+def test_sort_side_bunched_keys(kind):
+    """Keys bunched on a few ranges or values: buckets past the LDS take more top bits, groups of
+    many equal keys go to the LSD passes in LDS, and a bucket of equal keys past the LDS to the LSD
+    sort; each result is the stable argsort."""
+    import mitm_gpu2 as G
+    rng = np.random.default_rng(11)
+    k = _bunched(kind, rng)
+    i = np.arange(len(k), dtype=np.uint64)
+    say("sort bunched", kind, len(k))
+    ks, is_ = G.sort_side(dev(k), dev(i))
+    o = np.argsort(k, kind="stable")
+    assert np.array_equal(ks.get(), k[o]) and np.array_equal(is_.get(), i[o])
+
+
 @pytest.mark.parametrize("n,M", [(3000000, 1 << 51), (100000, 1 << 40), (100000, 7), (4097, 1 << 63)])
 def test_sort_pairs_in_place(n, M):
     """cf_sort_pairs (in place) against the stable argsort: with one MSD pass the in-place sort

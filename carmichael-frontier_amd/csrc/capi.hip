@@ -243,8 +243,7 @@ static void plan_first_counts(cf_plan *pl) {
 
 void plan_write_range(cf_plan *pl, u64 L0, u64 L1, u64 *keys, u64 *ids) {
     if (L1 <= L0) return;
-    launch_leaf_write(pl->ctx, pl->d_desc.p, pl->leafp.p, pl->nleafp, pl->off.p, L0, L1 - L0, pl->tab.desc.r - 1, keys,
-                      ids);
+    leaf_records(pl->ctx, pl->d_desc.p, pl->leafp.p, pl->nleafp, pl->off.p, L0, L1 - L0, pl->tab.desc.r - 1, keys, ids);
 }
 
 extern "C" {

@@ -173,6 +173,8 @@ struct SideDesc {
     int leaf32;         // 1: every prime of the instance (both sides) is below 2^32, so join leaves take the
                         // 32-bit scaled step (cf_redc32_mul); 0: the plain 64-bit step
     u64 xm0, x20;       // initial residue state (the side multiplier)
+    u64 s32;            // 2^96 mod m: cf_mulmod_m(x, s32) = x * 2^32 mod m, which the 32-bit step
+                        // (x * p * 2^-32) turns into the exact x * p
 };
 
 // Host-owned copy of one side's device tables (one allocation holding all four arrays).
@@ -242,8 +244,10 @@ void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
 // own child count (cnt, nout entries)
 void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                                size_t nout, int depth, Node *out, u32 *cnt);
-void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
-                       u64 leaf_lo, size_t nleaf, int depth, u64 *keys, u64 *ids);
+// records (key, id) of leaves [lo, lo + n) of a last level, record j written at j - lo (leaf tiles
+// with their parents staged in LDS)
+void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                  int depth, u64 *keys, u64 *ids);
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
 // the same records with runs sharing their first r - 1 indices computed once (one wave per 64
 // consecutive prefixes); rank range checked against C(n, r)

@@ -267,32 +267,6 @@ __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__re
     }
 }
 
-// Leaves [leaf_lo, leaf_lo + nleaf) of the depth-(r-1) nodes; writes record j - leaf_lo.
-__global__ void __launch_bounds__(BLK) k_leaf_write(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
-                                                    size_t nn, const u64 *__restrict__ off, u64 leaf_lo, size_t nleaf,
-                                                    int depth, u64 *__restrict__ keys, u64 *__restrict__ ids) {
-    const size_t ntiles = (nleaf + BLK - 1) / BLK;
-    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const u64 j0 = leaf_lo + (u64)tile * BLK;
-        const u64 jend = leaf_lo + nleaf;
-        const u64 j1 = j0 + BLK < jend ? j0 + BLK : jend;
-        size_t p0, p1;
-        block_parent_range(off, nn, j0, j1, &p0, &p1);
-        const u64 j = j0 + threadIdx.x;
-        if (j < j1) {
-            const size_t p = find_parent(off, p0, p1, j);
-            const Node nd = nodes[p];
-            const SideDesc &s = descs[nd.meta >> 8];
-            const int k = (int)(j - off[p]);
-            const int x = (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
-            const u64 xm = cf_mulmod_m(nd.xm, s.pR[x], s.mc);
-            const u64 x2 = nd.x2 * s.p[x];
-            keys[j - leaf_lo] = cf_key(xm, x2, s.mc) + s.koff;
-            ids[j - leaf_lo] = id_update(s, nd.id, x, depth);
-        }
-    }
-}
-
 // Per first index: the leaf offsets where its nodes start and end. The nodes of one first index
 // are one contiguous run (depth-first order), so the node opening a run writes lo[first] = off[j]
 // and the node closing it writes hi[first] = off[j + 1]; its leaves are hi - lo (0 - 0 for a first
@@ -444,13 +418,6 @@ void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *
                   nodes, nn, off, tp.p, nout, depth, out, cnt);
 }
 
-void launch_leaf_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off, u64 leaf_lo,
-                       size_t nleaf, int depth, u64 *keys, u64 *ids) {
-    if (!nleaf) return;
-    CF_LAUNCH(ctx, "k_leaf_write", k_leaf_write, dim3(grid_for(nleaf, BLK)), dim3(BLK), 0, d_desc, nodes, nn, off,
-                       leaf_lo, nleaf, depth, keys, ids);
-}
-
 void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi) {
     if (!nn) return;
     CF_LAUNCH(ctx, "k_first_bounds", k_first_bounds, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, nodes, off, nn, lo,
@@ -543,6 +510,11 @@ void side_tables_host(const SideSpec &s, SideDesc &d, std::vector<u64> &buf, siz
     const u64 mult = s.mult % s.M;
     d.xm0 = d.mc.m == 1 ? 0 : mult % d.mc.m;
     d.x20 = mult;
+    d.s32 = d.mc.m == 1 ? 0 : (u64)(((unsigned __int128)1 << 96) % d.mc.m);
+    // one side's records take the 32-bit leaf step when its primes are below 2^32 (a join sets this
+    // again from both sides: the scaled residues it compares must share one step)
+    d.leaf32 = 1;
+    for (u64 q : s.primes) d.leaf32 &= q < (1ULL << 32);
     const int nn = n ? n : 1;
     auto take = [&](size_t words) {
         size_t at = (buf.size() + 1) & ~(size_t)1;   // 16-byte alignment for every array

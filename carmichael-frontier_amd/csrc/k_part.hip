@@ -434,6 +434,64 @@ __global__ void __launch_bounds__(PB) k_tile_parents(const u64 *__restrict__ off
     }
 }
 
+// ---- plan records ------------------------------------------------------------------------------
+// Records (key, id) of leaves [lo, lo + n) of a plan's last level, record j at j - lo: the leaf tiles
+// above, with the tile's parents' partial ids staged beside their residue state. On a fast tile a
+// leaf is a few LDS reads, one 32-bit Montgomery step and the exact key's CRT step; each lane writes
+// neighbouring leaves, so the stores are whole lines.
+template <bool NARROW>
+__device__ __forceinline__ void fast_records(const LeafSrc &L, const Tile &T, const TileLds &S, const u64 *s_id,
+                                             const SideDesc &sd, u64 *__restrict__ keys, u64 *__restrict__ ids) {
+    const FastCtx c(S);
+    const u64 jb = T.j0 - L.lo;
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        const u32 i = threadIdx.x + LPB * k;
+        if (i < T.cnt) {
+            u64 xm, x2;
+            int x;
+            u32 q;
+            leaf_fast<NARROW>(S, c, i, xm, x2, x, q);
+            keys[jb + i] = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
+            ids[jb + i] = id_update(sd, s_id[q], x, L.depth);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(LPB) k_leaf_records(LeafSrc L, u64 *__restrict__ keys, u64 *__restrict__ ids) {
+    __shared__ TileLds S;
+    __shared__ u64 s_id[NCAP];
+    tile_lds_init(S);
+    const u64 ntiles = (L.n + LT - 1) / LT;
+    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile T = tile_setup(L, t, S);
+        if (T.fast) {
+            // parents' ids, and their residues times 2^32: the leaf's 32-bit step then gives the
+            // exact residue (the join compares residues all scaled by 2^-32; records carry keys)
+            const SideDesc &sd = L.descs[S.inst];
+            for (u32 q = threadIdx.x; q < T.np; q += LPB) {
+                s_id[q] = L.nodes[T.p0 + q].id;
+                S.xm[q] = cf_mulmod_m(S.xm[q], sd.s32, sd.mc);
+            }
+            lds_sync();
+            if (S.narrow) fast_records<true>(L, T, S, s_id, sd, keys, ids);
+            else fast_records<false>(L, T, S, s_id, sd, keys, ids);
+        } else {
+            const u64 jb = T.j0 - L.lo;
+            for (int k = 0; k < LPT; ++k) {
+                const u32 i = threadIdx.x + LPB * k;
+                if (i < T.cnt) {
+                    const LeafRef r = leaf_ref(L, T, S, i);
+                    const u64 xm = cf_mulmod_m(r.nd.xm, r.s->pR[r.x], r.s->mc), x2 = r.nd.x2 * r.s->p[r.x];
+                    keys[jb + i] = cf_key(xm, x2, r.s->mc) + r.s->koff;
+                    ids[jb + i] = id_update(*r.s, r.nd.id, r.x, L.depth);
+                }
+            }
+        }
+        lds_sync();   // the tile's LDS is read before the next set-up rewrites it
+    }
+}
+
 // ---- D side: counting sort by bucket --------------------------------------------------------
 __global__ void __launch_bounds__(LPB) k_bucket_count(LeafSrc L, KeyHash kh, u32 *__restrict__ bcount) {
     __shared__ TileLds S;
@@ -1327,6 +1385,16 @@ static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     CF_LAUNCH(ctx, "k_tile_parents", k_tile_parents, dim3(grid_for(2 * ntiles, PB, 65536)), dim3(PB), 0, off, nn, lo, n,
               tp.p);
     return LeafSrc{descs, nodes, nn, off, lo, n, depth, tp.p};
+}
+
+void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+                  int depth, u64 *keys, u64 *ids) {
+    if (!n) return;
+    DBuf<u64> tp;
+    LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
+    const u64 ntiles = (n + LT - 1) / LT;
+    CF_LAUNCH(ctx, "k_leaf_records", k_leaf_records, dim3((unsigned)(ntiles < 8192 ? ntiles : 8192)), dim3(LPB), 0, L,
+              keys, ids);
 }
 
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,

@@ -298,11 +298,11 @@ __global__ void __launch_bounds__(SB) k_seg_table(const u64 *__restrict__ offA, 
 // with its position in the bucket as (low bits << BIB) | position: the packed words are distinct
 // and their order is the stable order, so any exact sort of them is the stable sort, and the
 // position then gathers the id.
-//   k_bucket_count: the top GB bits of the low bits pick a group; LDS atomics give each word a
+//   k_lds_sort: the top GB bits of the low bits pick a group; LDS atomics give each word a
 //   slot, a scan of the group counts places the groups, and each word's rank inside its group is
 //   the number of smaller words there (groups hold a few words when the keys spread). A bucket with
 //   a group above GMAX words (keys bunched on a few values) is marked and left to
-//   k_bucket_lsd: 8-bit LSD passes in LDS, where wave w owns positions [w * R * 64, (w + 1) * R * 64),
+//   k_lds_sort_lsd: 8-bit LSD passes in LDS, where wave w owns positions [w * R * 64, (w + 1) * R * 64),
 //   its lanes rank equal digits with ballots and keep a running count per digit across their
 //   rounds, and one block scan per pass orders the waves.
 // A bucket above BCAP sets bit 0 of *flag and is copied through unsorted; a marked bucket sets bit 1.
@@ -333,7 +333,7 @@ __device__ __forceinline__ void bucket_range(u32 q, const u64 *__restrict__ off,
     len = off[(u64)RADIX * F + (u64)(d2 + 1) * N] - start;
 }
 
-__global__ void __launch_bounds__(SB) k_bucket_max(const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
+__global__ void __launch_bounds__(SB) k_msd_bucket_max(const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
                                                    const u32 *__restrict__ sn, u32 nb, int m2,
                                                    unsigned long long *__restrict__ out) {
     const u32 q = blockIdx.x * SB + threadIdx.x;
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(SB) k_bucket_max(const u64 *__restrict__ off, 
     if ((threadIdx.x & 63) == 0 && len) atomicMax(out, (unsigned long long)len);
 }
 
-__global__ void __launch_bounds__(BT) k_bucket_count(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
+__global__ void __launch_bounds__(BT) k_lds_sort(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
                                                      const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
                                                      const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
                                                      unsigned long long *__restrict__ flag, u32 *__restrict__ mark) {
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(BT) k_bucket_count(const u64 *kin, const u64 *
     }
 }
 
-__global__ void __launch_bounds__(BT) k_bucket_lsd(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
+__global__ void __launch_bounds__(BT) k_lds_sort_lsd(const u64 *kin, const u64 *vin, u64 *kout, u64 *vout,
                                                    const u64 *__restrict__ off, const u32 *__restrict__ sfirst,
                                                    const u32 *__restrict__ sn, u64 n, int m2, int lowbits,
                                                    const u32 *__restrict__ mark) {
@@ -670,7 +670,7 @@ u64 max_bucket(cf_ctx *ctx, const u64 *off, const u32 *sfirst, const u32 *sn, in
     DBuf<u64> mx(ctx, 1);
     CF_HIP(hipMemsetAsync(mx.p, 0, sizeof(u64), ctx->stream));
     const u32 nb = 1u << m;
-    CF_LAUNCH(ctx, "k_bucket_max", k_bucket_max, dim3((nb + SB - 1) / SB), dim3(SB), 0, off, sfirst, sn, nb, m2,
+    CF_LAUNCH(ctx, "k_msd_bucket_max", k_msd_bucket_max, dim3((nb + SB - 1) / SB), dim3(SB), 0, off, sfirst, sn, nb, m2,
               (unsigned long long *)mx.p);
     return d2h_u64(ctx, mx.p);
 }
@@ -682,10 +682,10 @@ void bucket_sort(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vo
     const unsigned nb = 1u << m;
     DBuf<u32> mark(ctx, nb);
     CF_HIP(hipMemsetAsync(mark.p, 0, nb * sizeof(u32), ctx->stream));
-    CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn, (u64)n,
+    CF_LAUNCH(ctx, "k_lds_sort", k_lds_sort, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn, (u64)n,
               m2, lowbits, (unsigned long long *)flag, mark.p);
     if (d2h_u64(ctx, flag) & 2)
-        CF_LAUNCH(ctx, "k_bucket_lsd", k_bucket_lsd, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn,
+        CF_LAUNCH(ctx, "k_lds_sort_lsd", k_lds_sort_lsd, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn,
                   (u64)n, m2, lowbits, (const u32 *)mark.p);
 }
 constexpr int MSD_MAX_BITS = 16;    // two passes of at most 8 bits
@@ -694,7 +694,7 @@ constexpr int MSD_MAX_BITS = 16;    // two passes of at most 8 bits
 // Stable sort of (kin, vin) by key into (kout, vout) (which may be the inputs themselves), for keys
 // below 2^bits. The top m bits (mean bucket at most MSD_MEAN) are split off by one or two stable
 // MSD passes, the second segmented by the first's digit, and each bucket is then sorted in LDS by
-// k_bucket_count: three passes over HBM. Stability makes the result the same as the LSD sort's, which
+// k_lds_sort: three passes over HBM. Stability makes the result the same as the LSD sort's, which
 // takes over when the MSD bits would exceed two passes or a bucket overflows the LDS.
 void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, size_t n, int bits) {
     const bool inplace = kin == kout;

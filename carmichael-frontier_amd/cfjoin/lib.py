@@ -83,7 +83,7 @@ SIGNATURES = {
     "cf_prof_dump": (c_int, [c_void_p, ctypes.c_char_p, c_u64]),
     "cf_gen_subsets": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_u64, c_u64, c_u64, c_u64, c_void_p, c_void_p]),
     "cf_sort_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_u64]),
-    "cf_sort_pairs_into": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_u64]),
+    "cf_sort_pairs_into": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_u64, c_u64]),
     "cf_join_sorted": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64,
                                ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), P_u64]),
     "cf_join_sorted_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_u64, c_void_p,

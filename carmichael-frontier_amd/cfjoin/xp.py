@@ -44,7 +44,7 @@ class _Buffer:
 
 class ndarray:
     """1-D device array of 8-byte integers."""
-    __slots__ = ("_buf", "_off", "size", "dtype", "__weakref__")
+    __slots__ = ("_buf", "_off", "size", "dtype", "key_bound", "__weakref__")
 
     def __init__(self, size: int, dtype=np.uint64, _buf: Optional[_Buffer] = None, _off: int = 0,
                  ctx: Optional[_lib.Context] = None):
@@ -57,6 +57,9 @@ class ndarray:
             _buf = _Buffer(max(self.size * 8, 8), ctx or _lib.default_context())
         self._buf = _buf
         self._off = _off
+        # every value is below this bound when the library's generator that wrote them says so
+        # (keys reduced mod M); None when unknown. Slices and concatenations do not carry it.
+        self.key_bound = None
 
     @classmethod
     def _adopt(cls, ptr: int, size: int, dtype=np.uint64, ctx: Optional[_lib.Context] = None) -> "ndarray":
@@ -102,6 +105,7 @@ class ndarray:
         out = ndarray(self.size, self.dtype, ctx=self.ctx)
         if self.size:
             self.ctx.d2d(out.ptr, self.ptr, self.nbytes)
+        out.key_bound = self.key_bound
         return out
 
     def astype(self, dtype, copy: bool = True) -> "ndarray":

@@ -440,13 +440,14 @@ int cf_sort_pairs(cf_ctx *ctx, uint64_t *keys_dev, uint64_t *ids_dev, uint64_t n
 }
 
 int cf_sort_pairs_into(cf_ctx *ctx, const uint64_t *keys_dev, const uint64_t *ids_dev, uint64_t *keys_out_dev,
-                       uint64_t *ids_out_dev, uint64_t n) {
+                       uint64_t *ids_out_dev, uint64_t n, uint64_t key_bound) {
     API_BEGIN
     CF_REQUIRE(ctx, "NULL context");
     if (n == 0) return CF_OK;
     CF_REQUIRE(keys_dev && ids_dev && keys_out_dev && ids_out_dev, "NULL array");
     CF_REQUIRE((keys_dev == keys_out_dev) == (ids_dev == ids_out_dev), "sort both arrays in place or neither");
-    sort_pairs_into(ctx, keys_dev, ids_dev, keys_out_dev, ids_out_dev, n, bitlen(max_u64(ctx, keys_dev, n)));
+    const int bits = key_bound ? bitlen(key_bound - 1) : bitlen(max_u64(ctx, keys_dev, n));
+    sort_pairs_into(ctx, keys_dev, ids_dev, keys_out_dev, ids_out_dev, n, bits);
     CF_HIP(hipStreamSynchronize(ctx->stream));
     API_END
 }

@@ -38,7 +38,11 @@ __device__ __forceinline__ u64 upper_bound_in(const u64 *__restrict__ a, u64 lo,
 // window by a 6-step search over lane shuffles; a lane whose key lies beyond the window searches the
 // rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order gets
 // exact bounds.
-__device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, bool valid, u64 &lb, u64 &cnt) {
+// `from` (wave-uniform): a D row known to be at or below the first key's bound (the previous chunk's
+// last bound when its last key is at most this chunk's first); the 64 rows from there are tried
+// before the 64-ary search.
+__device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, bool valid, u64 from, u64 &lb,
+                                             u64 &cnt) {
     const int lane = threadIdx.x & 63;
     const u64 k0 = __shfl(k, 0, 64);
     if (!valid) k = k0;
@@ -49,7 +53,12 @@ __device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD,
         return;
     }
     // invariant: D rows below lo hold keys < k0, rows at or above hi keys >= k0
-    u64 lo = 0, hi = nD;
+    u64 lo = from, hi = nD;
+    if (hi - lo > 64) {
+        const int c = __popcll(__ballot(Dk[lo + lane] < k0));
+        if (c < 64) hi = lo + c;
+        lo += c;
+    }
     while (hi - lo > 64) {
         const u64 step = (hi - lo + 63) >> 6;
         const u64 p = lo + (u64)lane * step;
@@ -88,14 +97,18 @@ __global__ void __launch_bounds__(BLK) k_join_count(const u64 *__restrict__ Dk, 
     const int lane = threadIdx.x & 63;
     const u64 w0 = ((u64)blockIdx.x * BLK + threadIdx.x) >> 6, nw = ((u64)gridDim.x * BLK) >> 6;
     for (u64 sp = w0; sp < nspans; sp += nw) {
-        u64 tot = 0;
+        u64 tot = 0, from = 0, last = 0;
         for (int j = 0; j < JW; ++j) {
             const u64 i0 = sp * SPAN + (u64)j * 64;
             if (i0 >= nI) break;
             const u64 i = i0 + lane;
+            const u64 k = i < nI ? Ik[i] : 0;
             u64 lb, c;
-            chunk_bounds(Dk, nD, i < nI ? Ik[i] : 0, i < nI, lb, c);
+            chunk_bounds(Dk, nD, k, i < nI, __shfl(k, 0, 64) >= last ? from : 0, lb, c);
             tot += c;
+            const int lv = nI - i0 < 64 ? (int)(nI - i0) - 1 : 63;   // the chunk's last row
+            from = __shfl(lb, lv, 64);
+            last = __shfl(k, lv, 64);
         }
         tot = wave_incl_sum64(tot);
         if (lane == 63) scnt[sp] = tot;
@@ -114,6 +127,7 @@ __global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, 
     for (u64 sp = w0; sp < nspans; sp += nw) {
         u64 base = soff[sp];
         if (soff[sp + 1] == base) continue;
+        u64 from = 0, last = 0;
         for (int j = 0; j < JW; ++j) {
             const u64 i0 = sp * SPAN + (u64)j * 64;
             if (i0 >= nI) break;
@@ -121,7 +135,10 @@ __global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, 
             const bool valid = i < nI;
             const u64 k = valid ? Ik[i] : 0;
             u64 lb, c;
-            chunk_bounds(Dk, nD, k, valid, lb, c);
+            chunk_bounds(Dk, nD, k, valid, __shfl(k, 0, 64) >= last ? from : 0, lb, c);
+            const int lv = nI - i0 < 64 ? (int)(nI - i0) - 1 : 63;
+            from = __shfl(lb, lv, 64);
+            last = __shfl(k, lv, 64);
             const u64 incl = wave_incl_sum64(c);
             const u64 o = base + incl - c;
             if (c) {

@@ -78,6 +78,7 @@ def generate_chunks(INS, r, M, P0, Icap, split=None, hmax=None, max_records=200_
                     ids = cp.empty(acc, dtype=cp.uint64)
                     wrote = plan.write(a2, b2, keys.ptr, ids.ptr)
                     assert wrote == acc, ("per-prefix count/write mismatch", a2, b2)
+                    keys.key_bound = M
                     total += acc
                     yield keys, ids
                 a2 = b2

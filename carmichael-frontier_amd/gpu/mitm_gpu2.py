@@ -42,6 +42,7 @@ def generate(primes, r: int, M: int, mult: int = 1, bits: int = 10, lo: int = 0,
     ctx = keys.ctx
     _lib.check(_lib.load().cf_gen_subsets(ctx.handle, pr.ctypes.data_as(_lib.c_void_p), n, r, bits, M, mult % (1 << 64),
                                           lo, count, _lib.c_void_p(keys.ptr), _lib.c_void_p(ids.ptr)), "generate")
+    keys.key_bound = M
     return keys, ids
 
 
@@ -51,8 +52,11 @@ def sort_side(keys, ids):
         return keys.copy(), ids.copy()
     k = cp.ndarray(keys.size, keys.dtype, ctx=keys.ctx)
     i = cp.ndarray(ids.size, ids.dtype, ctx=keys.ctx)
+    # keys made by this library's generators carry their modulus: no pass for the largest key
+    bound = keys.key_bound or 0
     _lib.check(_lib.load().cf_sort_pairs_into(keys.ctx.handle, _lib.c_void_p(keys.ptr), _lib.c_void_p(ids.ptr),
-                                              _lib.c_void_p(k.ptr), _lib.c_void_p(i.ptr), keys.size), "sort_side")
+                                              _lib.c_void_p(k.ptr), _lib.c_void_p(i.ptr), keys.size, bound), "sort_side")
+    k.key_bound = keys.key_bound
     return k, i
 
 

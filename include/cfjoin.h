@@ -86,9 +86,11 @@ int cf_gen_subsets(cf_ctx *ctx, const uint64_t *primes, int n, int r, int bits, 
  * Replaces sort_side() (gpu/mitm_gpu2.py:101-104): stable sort of (keys, ids) by key, in place. */
 int cf_sort_pairs(cf_ctx *ctx, uint64_t *keys_dev, uint64_t *ids_dev, uint64_t n);
 /* The same sort into new arrays (sort_side returns new arrays, gpu/mitm_gpu2.py:101-104): the
- * inputs are left unchanged; the output pair may also be the input pair. */
+ * inputs are left unchanged; the output pair may also be the input pair. key_bound: 0, or a bound
+ * every key is below (the modulus the keys were reduced by), which spares the pass that finds the
+ * largest key; a key at or above a non-zero bound leaves the order undefined. */
 int cf_sort_pairs_into(cf_ctx *ctx, const uint64_t *keys_dev, const uint64_t *ids_dev, uint64_t *keys_out_dev,
-                       uint64_t *ids_out_dev, uint64_t n);
+                       uint64_t *ids_out_dev, uint64_t n, uint64_t key_bound);
 
 /* ---- join two sorted sides --------------------------------------------------------------
  * Replaces join_sorted() (gpu/mitm_gpu2.py:107-119): every (D, I) with equal key, I-major then

@@ -198,10 +198,15 @@ __global__ void __launch_bounds__(RB) k_radix_scatter(const u64 *__restrict__ ki
     __shared__ u64 s_v[RT];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const u64 lt = lane ? (~0ULL >> (64 - lane)) : 0ULL;
-    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // XCD-aware order: blocks are dealt to the 8 dies round-robin, so die x takes the tiles
+    // [x * per, (x + 1) * per) in order, and a digit's runs from neighbouring tiles (adjacent in the
+    // output) meet in that die's L2 instead of reaching memory as partial lines from two dies
+    const size_t per = (ntiles + 7) / 8;
+    for (size_t b = blockIdx.x; b < 8 * per; b += gridDim.x) {
+        const size_t tile = (b & 7) * per + (b >> 3);
         u64 base, c0, cs;
         int tcount = 0;
-        if (!g.tile(tile, base, tcount, c0, cs)) continue;   // uniform over the block
+        if (tile >= ntiles || !g.tile(tile, base, tcount, c0, cs)) continue;   // uniform over the block
         s_goff[threadIdx.x] = off[c0 + (u64)threadIdx.x * cs];
         s_cnt[threadIdx.x] = 0;
         u64 k[RI], v[RI];
@@ -637,6 +642,12 @@ u64 scan_u64(cf_ctx *ctx, const u64 *in, u64 *out, size_t n) {
 void scan_u32_to_u64_async(cf_ctx *ctx, const u32 *in, u64 *out, size_t n) { scan_impl<u32>(ctx, in, out, n); }
 
 namespace {
+// Blocks of a pass over ntiles tiles: a multiple of 8 (one tile per block up to 65536 blocks).
+unsigned scatter_grid(size_t ntiles) {
+    const size_t g = 8 * ((ntiles + 7) / 8);
+    return (unsigned)(g < 65536 ? g : 65536);
+}
+
 // Stable LSD radix sort in place: one flat pass per 8 bits of `bits`.
 void lsd_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
     if (n <= 1 || bits <= 0) return;
@@ -646,7 +657,7 @@ void lsd_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {
     DBuf<u32> hist(ctx, ntiles * RADIX);
     DBuf<u64> off(ctx, ntiles * RADIX + 1);
     u64 *ka = keys, *va = vals, *kb = k2.p, *vb = v2.p;
-    const unsigned grid = (unsigned)(ntiles < 65536 ? ntiles : 65536);
+    const unsigned grid = scatter_grid(ntiles);
     const FlatGeo g{n, ntiles};
     for (int ps = 0; ps < passes; ++ps) {
         const int shift = ps * 8;
@@ -745,7 +756,7 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
             ntA = (n + RT - 1) / RT;
             DBuf<u32> histA(ctx, ntA * RADIX);
             offA = DBuf<u64>(ctx, ntA * RADIX + 1);
-            const unsigned grid = (unsigned)(ntA < 65536 ? ntA : 65536);
+            const unsigned grid = scatter_grid(ntA);
             const FlatGeo g{n, ntA};
             CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<FlatGeo>, dim3(grid), dim3(RB), 0, kin, bits - m1, histA.p,
                       ntA, g);
@@ -762,7 +773,7 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
         const size_t ntB = (n + RT - 1) / RT + (size_t)S;
         DBuf<u32> histB(ctx, ntB * RADIX);
         DBuf<u64> offB(ctx, ntB * RADIX + 1);
-        const unsigned grid = (unsigned)(ntB < 65536 ? ntB : 65536);
+        const unsigned grid = scatter_grid(ntB);
         const SegGeo g{sstart.p, send.p, sfirst.p, sn.p, S};
         CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<SegGeo>, dim3(grid), dim3(RB), 0, ak, lowbits, histB.p, ntB, g);
         scan_impl<u32>(ctx, histB.p, offB.p, ntB * RADIX);

@@ -167,7 +167,11 @@ template <class G>
 __global__ void __launch_bounds__(RB) k_radix_hist(const u64 *__restrict__ keys, int shift, u32 *__restrict__ hist,
                                                    size_t ntiles, G g) {
     __shared__ u32 h[RADIX];
-    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // the scatter's XCD-aware order: neighbouring tiles' counts (adjacent words) meet in one L2
+    const size_t per = (ntiles + 7) / 8;
+    for (size_t b = blockIdx.x; b < 8 * per; b += gridDim.x) {
+        const size_t tile = (b & 7) * per + (b >> 3);
+        if (tile >= ntiles) continue;
         u64 base, c0, cs;
         int tcount = 0;
         const bool live = g.tile(tile, base, tcount, c0, cs);

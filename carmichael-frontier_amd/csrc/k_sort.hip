@@ -305,8 +305,8 @@ __global__ void __launch_bounds__(SB) k_seg_table(const u64 *__restrict__ offA, 
 // After the MSD passes, bucket q holds the keys whose top m bits equal q, in input order. One
 // block sorts a bucket of at most BCAP keys by its remaining `lowbits` bits. Each key is packed
 // with its position in the bucket as (low bits << BIB) | position: the packed words are distinct
-// and their order is the stable order, so any exact sort of them is the stable sort, and the
-// position then gathers the id.
+// and their order is the stable order, so any exact sort of them is the stable sort; the position
+// carries each id to its key's place.
 //   k_lds_sort: the top GB bits of the low bits pick a group; LDS atomics give each word a
 //   slot, a scan of the group counts places the groups, and each word's rank inside its group is
 //   the number of smaller words there (groups hold a few words when the keys spread). A bucket with
@@ -391,13 +391,14 @@ __global__ void __launch_bounds__(BT) k_lds_sort(const u64 *kin, const u64 *vin,
     for (int x = tid; x <= NG; x += BT) s_c[x] = 0;
     if (tid == 0) s_big = 0;
     __syncthreads();
-    u64 v[BI];
+    u64 v[BI], idr[BI];
     u32 slot[BI];
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
         const u32 e = wbase + (u32)(it * 64 + lane);
         if (it < R && e < ln) {
             v[it] = ((kb[e] & lowmask) << BIB) | e;
+            idr[it] = vb[e];
             slot[it] = atomicAdd(&s_c[(u32)(v[it] >> gsh) & gmask], 1u);
         }
     }
@@ -448,6 +449,8 @@ __global__ void __launch_bounds__(BT) k_lds_sort(const u64 *kin, const u64 *vin,
         }
         return;
     }
+    // keys, then ids, each staged at its final place and written in order (reads of the bucket are
+    // all done: kin and kout, vin and vout may be one array)
 #pragma unroll
     for (int it = 0; it < BI; ++it)
         if (it < R && wbase + (u32)(it * 64 + lane) < ln) s_v[slot[it]] = v[it];
@@ -455,26 +458,17 @@ __global__ void __launch_bounds__(BT) k_lds_sort(const u64 *kin, const u64 *vin,
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
         const u32 e = wbase + (u32)(it * 64 + lane);
-        if (it < R && e < ln) {
-            const u64 x = s_v[e];
-            slot[it] = (u32)x & (BCAP - 1);
-            v[it] = ((u64)q << lowbits) | (x >> BIB);
-        }
+        if (it < R && e < ln) kob[e] = ((u64)q << lowbits) | (s_v[e] >> BIB);
     }
-    u64 idv[BI];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < BI; ++it)
+        if (it < R && wbase + (u32)(it * 64 + lane) < ln) s_v[slot[it]] = idr[it];
+    __syncthreads();
 #pragma unroll
     for (int it = 0; it < BI; ++it) {
         const u32 e = wbase + (u32)(it * 64 + lane);
-        if (it < R && e < ln) idv[it] = vb[slot[it]];
-    }
-    __syncthreads();   // every id is read before any is written (vin and vout may be one array)
-#pragma unroll
-    for (int it = 0; it < BI; ++it) {
-        const u32 e = wbase + (u32)(it * 64 + lane);
-        if (it < R && e < ln) {
-            kob[e] = v[it];
-            vob[e] = idv[it];
-        }
+        if (it < R && e < ln) vob[e] = s_v[e];
     }
 }
 

@@ -39,8 +39,8 @@ __device__ __forceinline__ u64 upper_bound_in(const u64 *__restrict__ a, u64 lo,
 // rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order gets
 // exact bounds.
 // `from` (wave-uniform): a D row known to be at or below the first key's bound (the previous chunk's
-// last bound when its last key is at most this chunk's first); the 64 rows from there are tried
-// before the 64-ary search.
+// last bound when its last key is at most this chunk's first, else 0); the 64 rows from there are
+// the window unless they all lie below the chunk.
 __device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, bool valid, u64 from, u64 &lb,
                                              u64 &cnt) {
     const int lane = threadIdx.x & 63;
@@ -52,29 +52,31 @@ __device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD,
         cnt = valid ? upper_bound_in(Dk, lb, nD, k) - lb : 0;
         return;
     }
-    // invariant: D rows below lo hold keys < k0, rows at or above hi keys >= k0
-    u64 lo = from, hi = nD;
-    if (hi - lo > 64) {
-        const int c = __popcll(__ballot(Dk[lo + lane] < k0));
-        if (c < 64) hi = lo + c;
-        lo += c;
-    }
-    while (hi - lo > 64) {
-        const u64 step = (hi - lo + 63) >> 6;
-        const u64 p = lo + (u64)lane * step;
-        const int c = __popcll(__ballot(p < hi && Dk[p] < k0));
-        if (c == 0) {
-            hi = lo;
-            break;
+    // the window: the 64 D rows from `from`; when all of them lie below the chunk's first key, a
+    // 64-ary search over the rest finds that key's bound and the window moves there.
+    // invariant of the search: D rows below lo hold keys < k0, rows at or above hi keys >= k0
+    u64 L0 = from;
+    u64 navail = nD - L0 < 64 ? nD - L0 : 64;
+    u64 W = (u64)lane < navail ? Dk[L0 + lane] : 0;
+    if (navail == 64 && __popcll(__ballot(W < k0)) == 64) {
+        u64 lo = L0 + 64, hi = nD;
+        while (hi - lo > 64) {
+            const u64 step = (hi - lo + 63) >> 6;
+            const u64 p = lo + (u64)lane * step;
+            const int c = __popcll(__ballot(p < hi && Dk[p] < k0));
+            if (c == 0) {
+                hi = lo;
+                break;
+            }
+            const u64 last = lo + (u64)(c - 1) * step;   // the last probe below k0
+            lo = last + 1;
+            if (last + step < hi) hi = last + step;      // the first probe at or above k0
         }
-        const u64 last = lo + (u64)(c - 1) * step;   // the last probe below k0
-        lo = last + 1;
-        if (last + step < hi) hi = last + step;      // the first probe at or above k0
+        if (hi > lo) lo += __popcll(__ballot(lo + lane < hi && Dk[lo + lane] < k0));
+        L0 = lo;
+        navail = nD - L0 < 64 ? nD - L0 : 64;
+        W = (u64)lane < navail ? Dk[L0 + lane] : 0;
     }
-    if (hi > lo) lo += __popcll(__ballot(lo + lane < hi && Dk[lo + lane] < k0));
-    const u64 L0 = lo;
-    const u64 navail = nD - L0 < 64 ? nD - L0 : 64;
-    const u64 W = (u64)lane < navail ? Dk[L0 + lane] : 0;
     int a = 0, b = 0;                                 // window keys < k, and <= k
 #pragma unroll
     for (int s = 32; s >= 1; s >>= 1) {
@@ -98,11 +100,18 @@ __global__ void __launch_bounds__(BLK) k_join_count(const u64 *__restrict__ Dk, 
     const u64 w0 = ((u64)blockIdx.x * BLK + threadIdx.x) >> 6, nw = ((u64)gridDim.x * BLK) >> 6;
     for (u64 sp = w0; sp < nspans; sp += nw) {
         u64 tot = 0, from = 0, last = 0;
+        u64 kr[JW];   // the span's keys, loaded ahead of the searches
+#pragma unroll
+        for (int j = 0; j < JW; ++j) {
+            const u64 i = sp * SPAN + (u64)j * 64 + lane;
+            kr[j] = i < nI ? Ik[i] : 0;
+        }
+#pragma unroll
         for (int j = 0; j < JW; ++j) {
             const u64 i0 = sp * SPAN + (u64)j * 64;
             if (i0 >= nI) break;
             const u64 i = i0 + lane;
-            const u64 k = i < nI ? Ik[i] : 0;
+            const u64 k = kr[j];
             u64 lb, c;
             chunk_bounds(Dk, nD, k, i < nI, __shfl(k, 0, 64) >= last ? from : 0, lb, c);
             tot += c;

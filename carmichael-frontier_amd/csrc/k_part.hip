@@ -458,19 +458,40 @@ __device__ __forceinline__ void fast_records(const LeafSrc &L, const Tile &T, co
     }
 }
 
-__global__ void __launch_bounds__(LPB) k_leaf_records(LeafSrc L, u64 *__restrict__ keys, u64 *__restrict__ ids) {
+// Block k takes tiles [k * tpb, (k + 1) * tpb) with the scatter's pipeline: the next tile's parents
+// (and their ids) are fetched into registers while this tile's records are computed and written.
+__global__ void __launch_bounds__(LPB) k_leaf_records(LeafSrc L, u64 tpb, u64 *__restrict__ keys,
+                                                      u64 *__restrict__ ids) {
     __shared__ TileLds S;
     __shared__ u64 s_id[NCAP];
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
-    for (u64 t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const Tile T = tile_setup(L, t, S);
+    const u64 t0 = (u64)blockIdx.x * tpb;
+    const u64 t1 = t0 + tpb < ntiles ? t0 + tpb : ntiles;
+    TilePF pf{};
+    u64 pid = 0;   // partial id of parent pf.p0 + threadIdx.x
+    u64 tpv = 0;
+    auto fetch = [&](u64 v) {
+        const u64 p0 = lane_u64(v, 0), p1 = lane_u64(v, 1);
+        tile_fetch(L, p0, p1, pf);
+        if (p0 + threadIdx.x <= p1) pid = L.nodes[p0 + threadIdx.x].id;
+    };
+    if (t0 < t1) fetch(tile_owner_load(L, t0));
+    if (t0 + 1 < t1) tpv = tile_owner_load(L, t0 + 1);
+    for (u64 t = t0; t < t1; ++t) {
+        if (threadIdx.x <= pf.p1 - pf.p0 && threadIdx.x < (u32)NCAP) s_id[threadIdx.x] = pid;
+        const Tile T = tile_setup_pf(L, t, S, pf);
+        if (t + 1 < t1) {
+            fetch(tpv);
+            if (t + 2 < t1) tpv = tile_owner_load(L, t + 2);
+        }
         if (T.fast) {
-            // parents' ids, and their residues times 2^32: the leaf's 32-bit step then gives the
-            // exact residue (the join compares residues all scaled by 2^-32; records carry keys)
+            // parents past the block's threads: ids from memory; every parent's residue times 2^32,
+            // so that the leaf's 32-bit step gives the exact residue (the join compares residues all
+            // scaled by 2^-32; records carry keys)
             const SideDesc &sd = L.descs[S.inst];
             for (u32 q = threadIdx.x; q < T.np; q += LPB) {
-                s_id[q] = L.nodes[T.p0 + q].id;
+                if (q >= (u32)LPB) s_id[q] = L.nodes[T.p0 + q].id;
                 S.xm[q] = cf_mulmod_m(S.xm[q], sd.s32, sd.mc);
             }
             lds_sync();
@@ -488,7 +509,10 @@ __global__ void __launch_bounds__(LPB) k_leaf_records(LeafSrc L, u64 *__restrict
                 }
             }
         }
-        lds_sync();   // the tile's LDS is read before the next set-up rewrites it
+        lds_sync();   // the tile's LDS is read; its owner map goes back to zero for the next set-up
+#pragma unroll
+        for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;
+        lds_sync();
     }
 }
 
@@ -1393,7 +1417,8 @@ void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t 
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     const u64 ntiles = (n + LT - 1) / LT;
-    CF_LAUNCH(ctx, "k_leaf_records", k_leaf_records, dim3((unsigned)(ntiles < 8192 ? ntiles : 8192)), dim3(LPB), 0, L,
+    const u64 nblk = ntiles < 1024 ? ntiles : 1024, tpb = (ntiles + nblk - 1) / nblk;
+    CF_LAUNCH(ctx, "k_leaf_records", k_leaf_records, dim3((unsigned)((ntiles + tpb - 1) / tpb)), dim3(LPB), 0, L, tpb,
               keys, ids);
 }
 

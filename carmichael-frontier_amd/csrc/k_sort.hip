@@ -686,16 +686,19 @@ u64 max_bucket(cf_ctx *ctx, const u64 *off, const u32 *sfirst, const u32 *sn, in
 
 // The 2^m buckets sorted from (kin, vin) into (kout, vout); *flag gets bit 0 for a bucket above
 // BCAP (copied through). Buckets the counting pass marks are finished by the LSD kernel here.
-void bucket_sort(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, const u64 *off, const u32 *sfirst,
+// Returns the flag word (one read-back).
+u64 bucket_sort(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64 *vout, const u64 *off, const u32 *sfirst,
                  const u32 *sn, size_t n, int m, int m2, int lowbits, u64 *flag) {
     const unsigned nb = 1u << m;
     DBuf<u32> mark(ctx, nb);
     CF_HIP(hipMemsetAsync(mark.p, 0, nb * sizeof(u32), ctx->stream));
     CF_LAUNCH(ctx, "k_lds_sort", k_lds_sort, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn, (u64)n,
               m2, lowbits, (unsigned long long *)flag, mark.p);
-    if (d2h_u64(ctx, flag) & 2)
+    const u64 f = d2h_u64(ctx, flag);
+    if (f & 2)
         CF_LAUNCH(ctx, "k_lds_sort_lsd", k_lds_sort_lsd, dim3(nb), dim3(BT), 0, kin, vin, kout, vout, off, sfirst, sn,
                   (u64)n, m2, lowbits, (const u32 *)mark.p);
+    return f;
 }
 constexpr int MSD_MAX_BITS = 16;    // two passes of at most 8 bits
 }  // namespace
@@ -732,11 +735,12 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
     DBuf<u64> flag(ctx, 1);
     CF_HIP(hipMemsetAsync(flag.p, 0, sizeof(u64), ctx->stream));
     if (m == 0) {   // one bucket of at most MSD_MEAN keys
-        bucket_sort(ctx, kin, vin, kout, vout, nullptr, nullptr, nullptr, n, 0, 0, bits, flag.p);
-        if (d2h_u64(ctx, flag.p) & 1) lsd_sort_pairs(ctx, kout, vout, n, bits);
+        if (bucket_sort(ctx, kin, vin, kout, vout, nullptr, nullptr, nullptr, n, 0, 0, bits, flag.p) & 1)
+            lsd_sort_pairs(ctx, kout, vout, n, bits);
         return;
     }
     const int mcap = std::min(MSD_MAX_BITS, bits);
+    bool overflow = false;
     for (int attempt = 0;; ++attempt) {
         const int m2 = std::min(m, 8), m1 = m - m2, S = 1 << m1;
         const int lowbits = bits - m;
@@ -792,10 +796,10 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
         }
         CF_LAUNCH(ctx, "k_radix_scatter", k_radix_scatter<SegGeo>, dim3(grid), dim3(RB), 0, ak, av, lowbits,
                   (const u64 *)offB.p, ntB, g, ok, ov);
-        bucket_sort(ctx, ok, ov, kout, vout, offB.p, sfirst.p, sn.p, n, m, m2, lowbits, flag.p);
+        overflow = bucket_sort(ctx, ok, ov, kout, vout, offB.p, sfirst.p, sn.p, n, m, m2, lowbits, flag.p) & 1;
         break;
     }
-    if (d2h_u64(ctx, flag.p) & 1) lsd_sort_pairs(ctx, kout, vout, n, bits);
+    if (overflow) lsd_sort_pairs(ctx, kout, vout, n, bits);
 }
 
 void radix_sort_pairs(cf_ctx *ctx, u64 *keys, u64 *vals, size_t n, int bits) {

@@ -38,58 +38,77 @@ __device__ __forceinline__ u64 upper_bound_in(const u64 *__restrict__ a, u64 lo,
 // window by a 6-step search over lane shuffles; a lane whose key lies beyond the window searches the
 // rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order gets
 // exact bounds.
+// The D window of a wave: the 64 D rows from L0 (fewer at the end of D) in lane order; ok once it
+// is loaded. Wave-uniform except W.
+struct Win {
+    u64 L0, W, navail;
+    bool ok;
+};
+
+__device__ __forceinline__ u64 rl64(u64 v, int l) {
+    return ((u64)(u32)__builtin_amdgcn_readlane((int)(v >> 32), l) << 32) |
+           (u64)(u32)__builtin_amdgcn_readlane((int)(u32)v, l);
+}
+
+// `lv`: the chunk's last row inside the I side (lanes past it repeat its key and count nothing).
 // `from` (wave-uniform): a D row known to be at or below the first key's bound (the previous chunk's
-// last bound when its last key is at most this chunk's first, else 0); the 64 rows from there are
-// the window unless they all lie below the chunk.
-__device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, bool valid, u64 from, u64 &lb,
-                                             u64 &cnt) {
+// last bound when its last key is at most this chunk's first, else 0); `keep`: the window is known to
+// start at or below that bound as well. The window is kept while it reaches the chunk's first key
+// (a span of sorted I rows usually falls within one window), else reloaded from `from`, and when all
+// of those rows lie below the chunk too, a 64-ary search over the rest finds the bound. A lane's
+// bounds are then the window rows below (and at or below) its key: the rows below the chunk's first
+// key count for every lane, and the few rows between its first and last key are read out of the
+// window one by one (no lane shuffles).
+__device__ __forceinline__ void chunk_bounds(const u64 *__restrict__ Dk, u64 nD, u64 k, int lv, u64 from, bool keep,
+                                             Win &w, u64 &lb, u64 &cnt) {
     const int lane = threadIdx.x & 63;
-    const u64 k0 = __shfl(k, 0, 64);
-    if (!valid) k = k0;
-    const u64 prev = __shfl(k, lane ? lane - 1 : 0, 64);
-    if (!__all(!valid || prev <= k)) {
+    const bool valid = lane <= lv;
+    const u64 k0 = rl64(k, 0), klast = rl64(k, lv);
+    if (!valid) k = klast;
+    const u64 prev = ((u64)wave_shift_up1((u32)(k >> 32)) << 32) | wave_shift_up1((u32)k);
+    if (!__all(lane == 0 || prev <= k)) {
         lb = lower_bound_in(Dk, 0, nD, k);
         cnt = valid ? upper_bound_in(Dk, lb, nD, k) - lb : 0;
+        w.ok = false;
         return;
     }
-    // the window: the 64 D rows from `from`; when all of them lie below the chunk's first key, a
-    // 64-ary search over the rest finds that key's bound and the window moves there.
-    // invariant of the search: D rows below lo hold keys < k0, rows at or above hi keys >= k0
-    u64 L0 = from;
-    u64 navail = nD - L0 < 64 ? nD - L0 : 64;
-    u64 W = (u64)lane < navail ? Dk[L0 + lane] : 0;
-    if (navail == 64 && __popcll(__ballot(W < k0)) == 64) {
-        u64 lo = L0 + 64, hi = nD;
-        while (hi - lo > 64) {
-            const u64 step = (hi - lo + 63) >> 6;
-            const u64 p = lo + (u64)lane * step;
-            const int c = __popcll(__ballot(p < hi && Dk[p] < k0));
-            if (c == 0) {
-                hi = lo;
-                break;
+    if (!(keep && w.ok && (w.navail < 64 || __popcll(__ballot(w.W < k0)) < 64))) {
+        // invariant of the search: D rows below lo hold keys < k0, rows at or above hi keys >= k0
+        w.L0 = from;
+        w.navail = nD - w.L0 < 64 ? nD - w.L0 : 64;
+        w.W = (u64)lane < w.navail ? Dk[w.L0 + lane] : 0;
+        if (w.navail == 64 && __popcll(__ballot(w.W < k0)) == 64) {
+            u64 lo = w.L0 + 64, hi = nD;
+            while (hi - lo > 64) {
+                const u64 step = (hi - lo + 63) >> 6;
+                const u64 p = lo + (u64)lane * step;
+                const int c = __popcll(__ballot(p < hi && Dk[p] < k0));
+                if (c == 0) {
+                    hi = lo;
+                    break;
+                }
+                const u64 last = lo + (u64)(c - 1) * step;   // the last probe below k0
+                lo = last + 1;
+                if (last + step < hi) hi = last + step;      // the first probe at or above k0
             }
-            const u64 last = lo + (u64)(c - 1) * step;   // the last probe below k0
-            lo = last + 1;
-            if (last + step < hi) hi = last + step;      // the first probe at or above k0
+            if (hi > lo) lo += __popcll(__ballot(lo + lane < hi && Dk[lo + lane] < k0));
+            w.L0 = lo;
+            w.navail = nD - w.L0 < 64 ? nD - w.L0 : 64;
+            w.W = (u64)lane < w.navail ? Dk[w.L0 + lane] : 0;
         }
-        if (hi > lo) lo += __popcll(__ballot(lo + lane < hi && Dk[lo + lane] < k0));
-        L0 = lo;
-        navail = nD - L0 < 64 ? nD - L0 : 64;
-        W = (u64)lane < navail ? Dk[L0 + lane] : 0;
+        w.ok = true;
     }
-    int a = 0, b = 0;                                 // window keys < k, and <= k
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) {
-        const u64 wa = __shfl(W, a + s - 1, 64);
-        const u64 wb = __shfl(W, b + s - 1, 64);
-        if ((u64)(a + s - 1) < navail && wa < k) a += s;
-        if ((u64)(b + s - 1) < navail && wb <= k) b += s;
+    const bool in = (u64)lane < w.navail;
+    const u32 c0 = (u32)__popcll(__ballot(in && w.W < k0));
+    const u32 e = (u32)__popcll(__ballot(in && w.W <= klast));
+    u32 a = c0, b = c0;   // window rows below k, and at or below k
+    for (u32 j = c0; j < e; ++j) {
+        const u64 wj = rl64(w.W, (int)j);
+        a += wj < k;
+        b += wj <= k;
     }
-    const u64 w63 = __shfl(W, 63, 64);
-    if (navail == 64 && a == 63 && w63 < k) a = 64;
-    if (navail == 64 && b == 63 && w63 <= k) b = 64;
-    lb = a < 64 ? L0 + a : lower_bound_in(Dk, L0 + 64, nD, k);
-    const u64 ub = b < 64 ? L0 + b : upper_bound_in(Dk, lb > L0 + 64 ? lb : L0 + 64, nD, k);
+    lb = a < 64 ? w.L0 + a : lower_bound_in(Dk, w.L0 + 64, nD, k);
+    const u64 ub = b < 64 ? w.L0 + b : upper_bound_in(Dk, lb > w.L0 + 64 ? lb : w.L0 + 64, nD, k);
     cnt = valid ? ub - lb : 0;
 }
 
@@ -100,6 +119,7 @@ __global__ void __launch_bounds__(BLK) k_join_count(const u64 *__restrict__ Dk, 
     const u64 w0 = ((u64)blockIdx.x * BLK + threadIdx.x) >> 6, nw = ((u64)gridDim.x * BLK) >> 6;
     for (u64 sp = w0; sp < nspans; sp += nw) {
         u64 tot = 0, from = 0, last = 0;
+        Win win{0, 0, 0, false};
         u64 kr[JW];   // the span's keys, loaded ahead of the searches
 #pragma unroll
         for (int j = 0; j < JW; ++j) {
@@ -113,11 +133,12 @@ __global__ void __launch_bounds__(BLK) k_join_count(const u64 *__restrict__ Dk, 
             const u64 i = i0 + lane;
             const u64 k = kr[j];
             u64 lb, c;
-            chunk_bounds(Dk, nD, k, i < nI, __shfl(k, 0, 64) >= last ? from : 0, lb, c);
-            tot += c;
             const int lv = nI - i0 < 64 ? (int)(nI - i0) - 1 : 63;   // the chunk's last row
-            from = __shfl(lb, lv, 64);
-            last = __shfl(k, lv, 64);
+            const bool after = rl64(k, 0) >= last;
+            chunk_bounds(Dk, nD, k, lv, after ? from : 0, after, win, lb, c);
+            tot += c;
+            from = rl64(lb, lv);
+            last = rl64(k, lv);
         }
         tot = wave_incl_sum64(tot);
         if (lane == 63) scnt[sp] = tot;
@@ -137,6 +158,7 @@ __global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, 
         u64 base = soff[sp];
         if (soff[sp + 1] == base) continue;
         u64 from = 0, last = 0;
+        Win win{0, 0, 0, false};
         for (int j = 0; j < JW; ++j) {
             const u64 i0 = sp * SPAN + (u64)j * 64;
             if (i0 >= nI) break;
@@ -144,10 +166,11 @@ __global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, 
             const bool valid = i < nI;
             const u64 k = valid ? Ik[i] : 0;
             u64 lb, c;
-            chunk_bounds(Dk, nD, k, valid, __shfl(k, 0, 64) >= last ? from : 0, lb, c);
             const int lv = nI - i0 < 64 ? (int)(nI - i0) - 1 : 63;
-            from = __shfl(lb, lv, 64);
-            last = __shfl(k, lv, 64);
+            const bool after = rl64(k, 0) >= last;
+            chunk_bounds(Dk, nD, k, lv, after ? from : 0, after, win, lb, c);
+            from = rl64(lb, lv);
+            last = rl64(k, lv);
             const u64 incl = wave_incl_sum64(c);
             const u64 o = base + incl - c;
             if (c) {
@@ -158,7 +181,7 @@ __global__ void __launch_bounds__(BLK) k_join_write(const u64 *__restrict__ Dk, 
                     if (okey) okey[o + q] = k;
                 }
             }
-            base += __shfl(incl, 63, 64);
+            base += rl64(incl, 63);
         }
     }
 }

@@ -720,7 +720,10 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
     }
     if (n <= 1 || bits <= 0) return;
     int m = 0;
-    while ((n >> m) > MSD_MEAN) ++m;
+    // below 2^24 keys a bucket costs little, so the first try aims lower and leaves room for keys
+    // bunched up to 8x their mean (the D sides of the divisor families bunch about 4x)
+    const size_t mean = n < ((size_t)1 << 24) ? MSD_MEAN / 4 : MSD_MEAN;
+    while ((n >> m) > mean) ++m;
     if (m > MSD_MAX_BITS && (n >> MSD_MAX_BITS) <= MSD_MEAN + MSD_MEAN / 2) m = MSD_MAX_BITS;
     m = std::max(m, bits + BIB - 64);   // the low bits and a position fit one word
     m = std::min(m, bits);
@@ -779,7 +782,7 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
         const SegGeo g{sstart.p, send.p, sfirst.p, sn.p, S};
         CF_LAUNCH(ctx, "k_radix_hist", k_radix_hist<SegGeo>, dim3(grid), dim3(RB), 0, ak, lowbits, histB.p, ntB, g);
         scan_impl<u32>(ctx, histB.p, offB.p, ntB * RADIX);
-        if (m < mcap && attempt < 2) {
+        if (m < mcap && attempt == 0) {
             // keys bunched at this depth (a bucket past the LDS): take more top bits and start again
             const u64 big = max_bucket(ctx, offB.p, sfirst.p, sn.p, m, m2);
             if (big > (u64)BCAP) {

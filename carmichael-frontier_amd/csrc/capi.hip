@@ -192,25 +192,31 @@ void plan_expand(cf_plan *pl) {
     const int nxt = pl->spec.side == SIDE_I ? 0 : sd.n - 1;
     root.pos = (u32)(unsigned short)(short)nxt;
     root.meta = 0;
-    DBuf<Node> cur(ctx, 1);
-    h2d(ctx, cur.p, &root, 1);
-    size_t ncur = 1;
+    DBuf<Node> rootd(ctx, 1);
+    h2d(ctx, rootd.p, &root, 1);
     if (sd.n == 0 || sd.n < sd.r) { pl->empty = true; return; }
+    // level by level, each pass writing a level and counting its nodes' children
+    DBuf<Node> cur;
+    const Node *curp = rootd.p;
+    size_t ncur = 1;
+    DBuf<u32> c(ctx, 1);
+    launch_expand_count(ctx, pl->d_desc.p, curp, 1, 0, c.p);
     for (int depth = 0; depth + 1 < sd.r; ++depth) {
-        DBuf<u32> c(ctx, ncur);
-        launch_expand_count(ctx, pl->d_desc.p, cur.p, ncur, depth, c.p);
         DBuf<u64> o(ctx, ncur + 1);
         const u64 tot = scan_u32_to_u64(ctx, c.p, o.p, ncur);
         if (tot == 0) { pl->empty = true; return; }
         DBuf<Node> nx(ctx, tot);
-        launch_expand_write(ctx, pl->d_desc.p, cur.p, ncur, o.p, tot, depth, nx.p);
+        DBuf<u32> cn(ctx, tot);
+        launch_expand_write_count(ctx, pl->d_desc.p, curp, ncur, o.p, tot, depth, nx.p, cn.p);
         cur = std::move(nx);
+        c = std::move(cn);
+        curp = cur.p;
         ncur = tot;
     }
-    pl->cnt = DBuf<u32>(ctx, ncur);
-    launch_expand_count(ctx, pl->d_desc.p, cur.p, ncur, sd.r - 1, pl->cnt.p);
+    if (!cur.p) cur = std::move(rootd);   // r == 1: the root is the last level
     pl->off = DBuf<u64>(ctx, ncur + 1);
-    pl->total = scan_u32_to_u64(ctx, pl->cnt.p, pl->off.p, ncur);
+    pl->total = scan_u32_to_u64(ctx, c.p, pl->off.p, ncur);
+    pl->cnt = std::move(c);
     pl->leafp = std::move(cur);
     pl->nleafp = ncur;
     if (pl->total == 0) pl->empty = true;

@@ -237,9 +237,6 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
 
 // tree expansion
 void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, int depth, u32 *cnt);
-// `off` is the exclusive scan of the per-node counts with nn + 1 entries (off[nn] = total)
-void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
-                         size_t nout, int depth, Node *out);
 // one level down (children of nodes[0, nn) with exclusive child scan off) together with each child's
 // own child count (cnt, nout entries)
 void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,

@@ -117,36 +117,6 @@ __global__ void __launch_bounds__(BLK) k_expand_count(const SideDesc *__restrict
     }
 }
 
-// Block-level narrowing: the block covers outputs [j0, j1); its parents lie in [p0, p1].
-__device__ __forceinline__ void block_parent_range(const u64 *__restrict__ off, size_t nn, u64 j0, u64 j1,
-                                                   size_t *p0, size_t *p1) {
-    __shared__ size_t s_p[2];
-    if (threadIdx.x == 0) s_p[0] = find_parent(off, 0, nn - 1, j0);
-    if (threadIdx.x == 1) s_p[1] = find_parent(off, 0, nn - 1, j1 - 1);
-    __syncthreads();
-    *p0 = s_p[0];
-    *p1 = s_p[1];
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(BLK) k_expand_write(const SideDesc *__restrict__ descs, const Node *__restrict__ nodes,
-                                                      size_t nn, const u64 *__restrict__ off, size_t nout, int depth,
-                                                      Node *__restrict__ out) {
-    const size_t ntiles = (nout + BLK - 1) / BLK;
-    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const u64 j0 = (u64)tile * BLK;
-        const u64 j1 = j0 + BLK < nout ? j0 + BLK : nout;
-        size_t p0, p1;
-        block_parent_range(off, nn, j0, j1, &p0, &p1);
-        const u64 j = j0 + threadIdx.x;
-        if (j < j1) {
-            const size_t p = find_parent(off, p0, p1, j);
-            const Node nd = nodes[p];
-            const SideDesc &s = descs[nd.meta >> 8];
-            out[j] = make_child(s, GTab{s}, nd, depth, (int)(j - off[p]));
-        }
-    }
-}
 
 // A level is written in tiles of EC = BLK * EPT outputs, one block of BLK threads each (EPT outputs
 // per thread): EPT = 4 on large levels, so a block's staged tables serve 1024 outputs; EPT = 1 on
@@ -395,12 +365,6 @@ void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
     CF_LAUNCH(ctx, "k_expand_count", k_expand_count, dim3(grid_for(nn, BLK)), dim3(BLK), 0, d_desc, nodes, nn, depth, cnt);
 }
 
-void launch_expand_write(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
-                         size_t nout, int depth, Node *out) {
-    if (!nout) return;
-    CF_LAUNCH(ctx, "k_expand_write", k_expand_write, dim3(grid_for(nout, BLK)), dim3(BLK), 0, d_desc, nodes, nn, off, nout,
-                       depth, out);
-}
 
 void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                                size_t nout, int depth, Node *out, u32 *cnt) {

@@ -104,7 +104,7 @@ int cf_join_sorted(cf_ctx *ctx, const uint64_t *Dk_dev, const uint64_t *Did_dev,
  * No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): cf_join_sorted with the
  * outputs supplied by the caller and every intermediate in a static per-device scratch of the
- * library (about 24 MiB, resident while the library is loaded); calls no allocator.
+ * library (32 KiB, resident while the library is loaded); calls no allocator.
  * Requires nI <= CF_JOIN_NOALLOC_MAX_I (join a longer sorted I side in slices: the pairs of a slice
  * are that slice's part of the I-major output). Writes the pairs to out_d_dev / out_i_dev in
  * cf_join_sorted's order and sets *npairs; if *npairs exceeds cap, nothing is written and

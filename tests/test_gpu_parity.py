@@ -171,10 +171,14 @@ def _bunched(kind, rng):
         return np.full(50000, 123456789, np.uint64)
     if kind == "two_values":
         return rng.integers(0, 2, 200000, dtype=np.uint64) * np.uint64(1 << 45)
+    if kind == "top_of_range":           # keys up to 2^64 - 1, a fifth of them exactly that
+        k = rng.integers(0, 1 << 63, 150000, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+        k[::5] = np.uint64(0xFFFFFFFFFFFFFFFF)
+        return k
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["half_in_one_range", "values_repeated", "all_equal", "two_values"])
+@pytest.mark.parametrize("kind", ["half_in_one_range", "values_repeated", "all_equal", "two_values", "top_of_range"])
 def test_sort_side_bunched_keys(kind):
     """Keys bunched on a few ranges or values: buckets past the LDS take more top bits, groups of
     many equal keys go to the LSD passes in LDS, and a bucket of equal keys past the LDS to the LSD
@@ -206,12 +210,17 @@ def test_sort_pairs_in_place(n, M):
 def test_join_sorted_matches_oracle():
     import mitm_gpu2 as G
     rng = np.random.default_rng(5)
-    # the last case leaves the I keys unsorted: each thread's bound then also moves back
+    # the unsorted case goes through a per-lane search; the last puts keys at 2^64 - 1 on both sides
     for nD, nI, M, sort_i in [(5000, 20000, 3000, True), (100, 100, 50, True), (200000, 300000, 1 << 30, True),
-                              (1, 1, 2, True), (20, 7, 9, True), (4000, 9001, 2500, False)]:
+                              (1, 1, 2, True), (20, 7, 9, True), (4000, 9001, 2500, False),
+                              (3000, 70000, 1 << 64, True)]:
         say("join", nD, nI, M, sort_i)
-        Dk = np.sort(rng.integers(0, M, nD, dtype=np.uint64))
+        Dk = rng.integers(0, M, nD, dtype=np.uint64)
         Ik = rng.integers(0, M, nI, dtype=np.uint64)
+        if M == 1 << 64:
+            Dk[:3] = np.uint64(0xFFFFFFFFFFFFFFFF)
+            Ik[::3] = np.uint64(0xFFFFFFFFFFFFFFFF)
+        Dk = np.sort(Dk)
         if sort_i:
             Ik = np.sort(Ik)
         Did = rng.integers(0, 1 << 60, nD, dtype=np.uint64)

@@ -737,7 +737,7 @@ void sort_pairs_into(cf_ctx *ctx, const u64 *kin, const u64 *vin, u64 *kout, u64
     }
     DBuf<u64> flag(ctx, 1);
     CF_HIP(hipMemsetAsync(flag.p, 0, sizeof(u64), ctx->stream));
-    if (m == 0) {   // one bucket of at most MSD_MEAN keys
+    if (m == 0) {   // one bucket: no more keys than the target mean
         if (bucket_sort(ctx, kin, vin, kout, vout, nullptr, nullptr, nullptr, n, 0, 0, bits, flag.p) & 1)
             lsd_sort_pairs(ctx, kout, vout, n, bits);
         return;

@@ -31,13 +31,11 @@ __device__ __forceinline__ u64 upper_bound_in(const u64 *__restrict__ a, u64 lo,
     return lo;
 }
 
-// The D run of each key of one 64-row chunk (lane = row; `valid` marks rows inside the I side, lane
-// 0 always one): lb = first D row with key >= k, cnt = rows with key == k. Called by whole waves.
-// Ascending chunks (the sorted I side) find the first key's bound by a 64-ary search (one probe per
-// lane per step, 6 bits per step), load the 64 D keys from there, and place every lane's key in that
-// window by a 6-step search over lane shuffles; a lane whose key lies beyond the window searches the
-// rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order gets
-// exact bounds.
+// chunk_bounds: the D run of each key of one 64-row chunk (lane = row): lb = first D row with key >= k,
+// cnt = rows with key == k. Called by whole waves. Ascending chunks (the sorted I side) place their
+// keys in a window of 64 D keys held one per lane; a lane whose key lies beyond the window searches
+// the rest of D itself. A chunk out of order falls back to a binary search per lane, so any I order
+// gets exact bounds.
 // The D window of a wave: the 64 D rows from L0 (fewer at the end of D) in lane order; ok once it
 // is loaded. Wave-uniform except W.
 struct Win {

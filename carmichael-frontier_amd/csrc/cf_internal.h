@@ -282,6 +282,7 @@ struct SegLayout {
     u64 nseg;
     u32 nblk;
     int capl;
+    const u32 *hi;   // I side: the records' leaf indices, apart from their words (null: {word, leaf} pairs)
 };
 struct IChunkLayout {   // the buffers behind a SegLayout
     DBuf<u64> O;
@@ -289,7 +290,8 @@ struct IChunkLayout {   // the buffers behind a SegLayout
     u64 nseg = 0;
     u32 nblk = 0;
     int capl = 0;
-    SegLayout view() const { return SegLayout{O.p, fill.p, nseg, nblk, capl}; }
+    u32 *hi = nullptr;
+    SegLayout view() const { return SegLayout{O.p, fill.p, nseg, nblk, capl, hi}; }
 };
 // slots a chunk of n leaves needs in the capacity layout
 // launch shapes and geometry targets that change speed, never results (cf_tuning, 0 = default)
@@ -305,14 +307,17 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // part_cap_slots returns n for it)
 // tp receives the chunk's tile owner map (2 entries per 2048 leaves), which the exact layout and
 // pass 2 take as given
+// Records are {bucket word, leaf} pairs (the D side with the exact keys in keys), or, with hi_at
+// (I side only), two arrays of 32-bit words: bucket words from rec on, leaf indices from word hi_at on
+// (rec holding at least 2 * hi_at words); the probe reads only the first.
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp, u64 *keys = nullptr);
+                      DBuf<u64> &tp, u64 *keys = nullptr, u64 hi_at = 0);
 // count, scan and scatter: n records exactly, in partition order
 // keys (D side): each record's exact key at the record's slot
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
-                        u64 *keys = nullptr);
+                        u64 *keys = nullptr, u64 hi_at = 0);
 // D side bucketed from its partitioned records (after the scatter with keys): windows of 2^wb
 // buckets within each partition (dwin_geometry: false if a partition would need more than 256);
 // part_d_windows moves the records into fixed-capacity window regions (wreg: P * R * dwin_capacity()

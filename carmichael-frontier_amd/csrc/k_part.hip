@@ -621,11 +621,11 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // rank, with its partition and place in the tile in the staged high word; the store side reads the
 // staged records in order (so the lanes of a wave read the words of only a few partitions) and
 // takes each record's slot from its partition's word.
-template <bool KEYS>
+template <bool KEYS, bool SPLIT = false>
 __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                   const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
                                                   u32 *__restrict__ ovf, uint2 *__restrict__ rec,
-                                                  u64 *__restrict__ keys) {
+                                                  u64 *__restrict__ keys, u32 *__restrict__ rhi) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u64 lc[256];
@@ -755,7 +755,12 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                     over |= !fits;
                     if (exact || fits) {
                         const u32 d = (u32)(w[k] >> 32) + rk;
-                        rec[d] = make_uint2(r[k].x, jb + (r[k].y & 0xFFFFu));
+                        if (SPLIT) {
+                            ((u32 *)rec)[d] = r[k].x;
+                            rhi[d] = jb + (r[k].y & 0xFFFFu);
+                        } else {
+                            rec[d] = make_uint2(r[k].x, jb + (r[k].y & 0xFFFFu));
+                        }
                         if constexpr (KEYS) keys[d] = s_k[q];
                     }
                 }
@@ -794,13 +799,20 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
 __global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec) {
-    part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
+    part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr, nullptr);
+}
+// The I side with its records split into two arrays of words (bucket words, then leaf indices), for
+// chunks probed against several tables: the probe then reads 4 bytes per record instead of 8.
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_split(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec, u32 *__restrict__ rhi) {
+    part_scatter_body<false, true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr, rhi);
 }
 __global__ void __launch_bounds__(LPB) k_part_scatter_keys(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                           const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
                                                           u32 *__restrict__ ovf, uint2 *__restrict__ rec,
                                                           u64 *__restrict__ keys) {
-    part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, keys);
+    part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, keys, nullptr);
 }
 
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).
@@ -908,7 +920,7 @@ __device__ __forceinline__ u64 uniform_u64(u64 v) {
            (u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)v);
 }
 
-__global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, const uint2 *__restrict__ recs,
+__global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, const u32 *__restrict__ recs,
                                                     const unsigned short *__restrict__ T,
                                                     const u32 *__restrict__ F, int FB, u32 *__restrict__ xq,
                                                     unsigned long long *__restrict__ ncand, u64 ccap,
@@ -924,6 +936,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     const u64 t_lo = ntiles * xcd / NXCD, t_hi = ntiles * (xcd + 1) / NXCD;
     const u32 fmask = (1u << kh.fpb) - 1u;
     const u64 K = kh_k(kh);
+    const u32 rsz = S.hi ? 4u : 8u;   // bytes per record in the bucket-word stream
     const u32 fbm = FB > 0 ? (u32)FB - 1u : 0u;
     const int lgfb = FB > 0 ? __builtin_ctz((u32)FB) : 0;   // FB is 4, 8 or 16
     u64 cur_part = ~0ull;
@@ -993,7 +1006,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
             for (int sj = 0; sj < SPW; ++sj) {
                 fills[sj] = (u32)__builtin_amdgcn_readlane((int)fv, grp * SPW + sj);
                 starts[sj] = lane_u64(sv, grp * SPW + sj);
-                rr[sj] = buf_rsrc(recs + starts[sj], (u64)fills[sj] * sizeof(uint2));
+                rr[sj] = buf_rsrc((const u8_t *)recs + starts[sj] * rsz, (u64)fills[sj] * rsz);
                 rounds_end = fills[sj] > rounds_end ? fills[sj] : rounds_end;
             }
             // the wave's SPW segments are probed together, 64 * PROBE_BATCH slots of each per round
@@ -1011,7 +1024,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     // past the fill the descriptor returns 0: bucket 0 of the partition, masked below;
                     // non-temporal (cache policy 2): the records stream through L2 once and should not
                     // evict the partition's table slice
-                    lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * sizeof(uint2)), 0, 2);
+                    lo[e] = (u32)__builtin_amdgcn_raw_buffer_load_b32(rr[sj], (int)(v * rsz), 0, 2);
                     valid |= (u32)(v < fills[sj]) << e;
                 }
                 u32 look = valid;
@@ -1309,7 +1322,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
 // Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
 // with the D records of its bucket; every equal key emits a pair.
 __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, KeyHash kh,
-                                                         const uint2 *__restrict__ recs,
+                                                         const u32 *__restrict__ recs,
                                                          SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
@@ -1336,7 +1349,7 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
         } else {
             g = e >> S.capl;
         }
-        const uint2 r = recs[e];
+        const uint2 r = S.hi ? make_uint2(recs[e], S.hi[e]) : ((const uint2 *)recs)[e];
         const u64 b = (g / S.nblk) * kh_k(kh) + (r.x >> kh.fpb);
         u64 key, iid;
         leaf_exact(L, r.y, key, iid);
@@ -1497,13 +1510,15 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_fits
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp, u64 *keys) {
+                      DBuf<u64> &tp, u64 *keys, u64 hi_at) {
+    CF_REQUIRE(!(keys && hi_at), "D records are {word, leaf} pairs");
+    lay.hi = hi_at ? (u32 *)rec + hi_at : nullptr;
     if (!cap_fits(n, kh, tn)) {
         // too many segments for 32-bit slots (a chunk near 2^32 leaves at a low fill): the exact
         // layout directly, never overflowing
         fill_u32(ctx, ovf, 1, 0);
         (void)leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-        part_scatter_exact(ctx, descs, nodes, nn, off, lo, n, depth, kh, lay, rec, tp.p, keys);
+        part_scatter_exact(ctx, descs, nodes, nn, off, lo, n, depth, kh, lay, rec, tp.p, keys, hi_at);
         return;
     }
     u64 tpb;
@@ -1519,6 +1534,9 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     if (keys)
         CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, keys);
+    else if (lay.hi)
+        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
+                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, lay.hi);
     else
         CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
@@ -1526,7 +1544,10 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
-                        u64 *keys) {
+                        u64 *keys, u64 hi_at) {
+    CF_REQUIRE(!(keys && hi_at), "D records are {word, leaf} pairs");
+    CF_REQUIRE(!hi_at || hi_at >= n, "the split records need room for every record");
+    lay.hi = hi_at ? (u32 *)rec + hi_at : nullptr;
     const int P = 1 << kh.lgP;
     LeafSrc L{descs, nodes, nn, off, lo, n, depth, tp};
     const u64 ntiles = (n + LT - 1) / LT;
@@ -1543,6 +1564,9 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     if (keys)
         CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, keys);
+    else if (lay.hi)
+        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
+                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, lay.hi);
     else
         CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
@@ -1584,7 +1608,7 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
     const u64 gmax = tn.probe_blocks > 0 ? (u64)tn.probe_blocks : 256;
     unsigned g = (unsigned)(ntiles < gmax ? ntiles : gmax);
     g = g < NXCD ? NXCD : g - g % NXCD;
-    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PBP), 0, kh, S, (const uint2 *)rec, T, F, FB, xq,
+    CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PBP), 0, kh, S, (const u32 *)rec, T, F, FB, xq,
               ncand, ccap, cand);
 }
 
@@ -1595,7 +1619,7 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
                       u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
     LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, dl.lo, dl.n, depth, dl.tp};
-    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const uint2 *)rec, S,
+    CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const u32 *)rec, S,
               ncand, ccap,
               cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst, kerr);
 }

@@ -552,8 +552,12 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     // one pass into segments sized for the expected share; the probe that follows
                     // is wasted only if a segment overflowed (heavily repeated keys), and then the
                     // chunk is laid out exactly and probed again
+                    // records split into bucket words and leaf indices when the chunk is probed against
+                    // several tables (every probe then reads 4 bytes a record; the scatter writes its
+                    // two streams less efficiently, which one probe does not repay)
+                    const u64 split_at = tabs.size() >= 2 ? cap_m : 0;
                     part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
-                                     ovf.p, tp);
+                                     ovf.p, tp, nullptr, split_at);
                     for (size_t k = 0; k < tabs.size(); ++k) {
                         const DTable &tb = tabs[k];
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
@@ -564,7 +568,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                             nc = host_value(ctx, tape, flags.p);
                             if (ov) {
                                 part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                                   rec.p, tp.p);
+                                                   rec.p, tp.p, nullptr, split_at);
                                 part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
                                                  (unsigned long long *)flags.p, ccap, cand.p, b->tune);
                                 nc = host_value(ctx, tape, flags.p);

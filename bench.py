@@ -676,7 +676,7 @@ def main():
                                        "traffic_vs_algorithmic_with_parents":
                                            round(traffic / (per_rec * nI_total / max(k_launches, 1) + pb), 3)}
         roofline["alu"] = measure_alu(["k_part_scatter", "k_part_probe", "k_expand_write_count", "k_dwin_sort",
-                                       "k_part_scatter_keys", "k_dwin_scatter"], breakdown, args.family, setting,
+                                       "k_part_scatter_d", "k_dwin_scatter"], breakdown, args.family, setting,
                                       valu_peaks(ctx))
 
     cpu = None

@@ -258,8 +258,8 @@ void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
 void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount);
-// DR: n records of 16 bytes (exact key, D leaf index) in bucket order; T: CF_TABLE_SLOTS u16 slots per
-// bucket, zeroed
+// DR: the n D records in bucket order (leaf index relative to lo | fingerprint << 40); T:
+// CF_TABLE_SLOTS u16 slots per bucket, zeroed
 void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
 // validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
@@ -307,23 +307,24 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // part_cap_slots returns n for it)
 // tp receives the chunk's tile owner map (2 entries per 2048 leaves), which the exact layout and
 // pass 2 take as given
-// Records are {bucket word, leaf} pairs (the D side with the exact keys in keys), or, with hi_at
-// (I side only), two arrays of 32-bit words: bucket words from rec on, leaf indices from word hi_at on
-// (rec holding at least 2 * hi_at words); the probe reads only the first.
+// Records are {bucket word, leaf} pairs, or, with hi_at (I side only), two arrays of 32-bit words:
+// bucket words from rec on, leaf indices from word hi_at on (rec holding at least 2 * hi_at words);
+// the probe reads only the first. dside: the D side's records (the same records; a kernel of its own,
+// so that profiles tell the sides apart).
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp, u64 *keys = nullptr, u64 hi_at = 0);
+                      DBuf<u64> &tp, bool dside = false, u64 hi_at = 0);
 // count, scan and scatter: n records exactly, in partition order
-// keys (D side): each record's exact key at the record's slot
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
-                        u64 *keys = nullptr, u64 hi_at = 0);
-// D side bucketed from its partitioned records (after the scatter with keys): windows of 2^wb
-// buckets within each partition (dwin_geometry: false if a partition would need more than 256);
-// part_d_windows moves the records into fixed-capacity window regions (wreg: P * R * dwin_capacity()
-// records of 16 bytes; *ovf set if a window overflowed); after the caller's exclusive scan of wcount
-// into wbase (P * R + 1 entries), part_d_sort writes bstart (NB + 1), DR in bucket order
-// ({key, D leaf}), the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket);
+                        bool dside = false, u64 hi_at = 0);
+// D side bucketed from its partitioned records: windows of 2^wb buckets within each partition
+// (dwin_geometry: false if a partition would need more than 256); part_d_windows moves the records
+// into fixed-capacity window regions (wreg: P * R * dwin_capacity() records of 8 bytes; *ovf set if a
+// window overflowed); after the caller's exclusive scan of wcount into wbase (P * R + 1 entries),
+// part_d_sort writes bstart (NB + 1), DR in bucket order (D leaf index | fingerprint << 40: a
+// record's exact key is recomputed from its leaf when a candidate with its fingerprint meets it),
+// the table slots T (every bucket) and the filter F (zeroed, FB bits per bucket);
 // kerr gets KERR_DWINDOW if a window holds a record outside its buckets (inconsistent input; the
 // window's buckets are then written empty)
 constexpr u32 KERR_DWINDOW = 1;   // a D record outside its window's buckets
@@ -331,7 +332,7 @@ constexpr u32 KERR_CAND = 2;      // more candidates reserved than the candidate
 bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R);
 u64 dwin_capacity();
 void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
-                    const u64 *keys, u32 *wcount, void *wreg, u32 *ovf);
+                    u32 *wcount, void *wreg, u32 *ovf);
 void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
                  const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB, u32 *kerr);
 // the D leaves of a radius group, for recomputing a pair's D id in pass 2

@@ -18,8 +18,9 @@
 // few bits per bucket, one per class of fingerprint low bits present; the probe keeps its
 // partition's slice of F in LDS and reads T only for records whose class is present.
 //
-//  D side: records counted per bucket and scattered into bucket order (DRec = exact key, id); the
-//          scatter writes the table slots as it goes; F is built from T.
+//  D side: records counted per bucket and scattered into bucket order (DR: the D leaf index; a
+//          record's exact key is recomputed from it when a candidate meets it); the scatter writes
+//          the table slots as it goes; F is built from T.
 //  I side: leaves scattered by partition as 8-byte records (low word: bucket offset << fpb | f,
 //          high word: leaf index in the chunk): each block ranks a tile of 4096 leaves by partition
 //          in LDS and writes one run per partition into slots sized from the expected share.
@@ -40,6 +41,17 @@ constexpr int LPT = 8;             // leaves per thread in the leaf kernels
 constexpr int LT = LPB * LPT;      // 4096 leaves per tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
+// A DR entry: the D record's leaf index (relative to its table's first leaf), above it the record's
+// fingerprint (the fpb bits its table slot holds) and 5 more bits of its hash (hash bits 12-16, used
+// by neither the bucket nor the fingerprint), so that a candidate recomputes only the keys of records
+// that agree with it on 19 hash bits.
+constexpr int DR_FP_SHIFT = 40;
+constexpr int DR_EX_SHIFT = 54;
+constexpr u64 DR_LEAF_MASK = ((u64)1 << DR_FP_SHIFT) - 1;
+__device__ __forceinline__ u32 hash_ex(u64 h) { return (u32)(h >> 12) & 31u; }
+// A window-built D table holds fewer than 2^27 records, so its records' leaf indices leave their top
+// 5 bits to carry hash_ex from the scatter to the window sort.
+constexpr int DREC_EX_SHIFT = 27;
 
 // Block barrier for LDS traffic only: orders the block's LDS accesses without waiting for its
 // outstanding global loads and stores (__syncthreads is also a workgroup memory fence, which on this
@@ -59,10 +71,6 @@ struct LeafSrc {
     u64 lo, n;        // leaves [lo, lo + n)
     int depth;        // r - 1
     const u64 *tp;    // per tile of LT leaves: owners of its first and last leaf (2 entries per tile)
-};
-
-struct DRec {
-    u64 key, ref;     // exact key and D leaf index (the record id is recomputed for an output pair)
 };
 
 // residue state of the leaf choosing index x below node nd
@@ -369,11 +377,10 @@ __device__ __forceinline__ u64 leaf_generic(const LeafSrc &L, const Tile &T, con
     return leaf_hash(xm, x2, r.s->mc.mask2, r.s->koff);
 }
 
-// Hashes (and with KEYS the exact keys) of the LPT leaves of this lane on a fast tile; FULL: the
-// tile holds LT leaves, else lanes past its end repeat its last leaf.
-template <bool NARROW, bool FULL, bool KEYS>
-__device__ __forceinline__ void fast_hashes(const Tile &T, const TileLds &S, const FastCtx &c, u64 (&h)[LPT],
-                                            u64 *key) {
+// Hashes of the LPT leaves of this lane on a fast tile; FULL: the tile holds LT leaves, else lanes
+// past its end repeat its last leaf.
+template <bool NARROW, bool FULL>
+__device__ __forceinline__ void fast_hashes(const Tile &T, const TileLds &S, const FastCtx &c, u64 (&h)[LPT]) {
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
         const u32 i = threadIdx.x + LPB * k;
@@ -381,7 +388,6 @@ __device__ __forceinline__ void fast_hashes(const Tile &T, const TileLds &S, con
         int x;
         u32 q;
         h[k] = leaf_fast<NARROW>(S, c, FULL || i < T.cnt ? i : T.cnt - 1, xm, x2, x, q);
-        if constexpr (KEYS) key[k] = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
     }
 }
 
@@ -390,33 +396,14 @@ template <bool FULL = false>
 __device__ __forceinline__ void tile_hashes(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT]) {
     if (T.fast) {
         const FastCtx c(S);
-        if (c.narrow) fast_hashes<true, FULL, false>(T, S, c, h, nullptr);
-        else fast_hashes<false, FULL, false>(T, S, c, h, nullptr);
+        if (c.narrow) fast_hashes<true, FULL>(T, S, c, h);
+        else fast_hashes<false, FULL>(T, S, c, h);
     } else {
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + LPB * k;
             u64 xm, x2;
             LeafRef r;
             h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
-        }
-    }
-}
-
-// Hashes and exact keys of the LPT leaves of this lane (the D side of the partitioned build).
-template <bool FULL = false>
-__device__ __forceinline__ void tile_hash_keys(const LeafSrc &L, const Tile &T, const TileLds &S, u64 (&h)[LPT],
-                                               u64 (&key)[LPT]) {
-    if (T.fast) {
-        const FastCtx c(S);
-        if (c.narrow) fast_hashes<true, FULL, true>(T, S, c, h, key);
-        else fast_hashes<false, FULL, true>(T, S, c, h, key);
-    } else {
-        for (int k = 0; k < LPT; ++k) {
-            const u32 i = threadIdx.x + LPB * k;
-            u64 xm, x2;
-            LeafRef r;
-            h[k] = leaf_generic(L, T, S, i < T.cnt ? i : T.cnt - 1, xm, x2, r);
-            key[k] = cf_key(xm, x2, r.s->mc) + r.s->koff;
         }
     }
 }
@@ -533,9 +520,8 @@ __global__ void __launch_bounds__(LPB) k_bucket_count(LeafSrc L, KeyHash kh, u32
 }
 
 // Scatter into bucket order; the rank a record takes in its bucket is also its table slot.
-__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key);
 __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, const u64 *__restrict__ bstart,
-                                                       u32 *__restrict__ bcur, DRec *__restrict__ DR,
+                                                       u32 *__restrict__ bcur, u64 *__restrict__ DR,
                                                        unsigned short *__restrict__ T) {
     __shared__ TileLds S;
     const u32 fmask = (1u << kh.fpb) - 1u;
@@ -546,34 +532,27 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
         for (int k = 0; k < LPT; ++k) {
             const u32 i = threadIdx.x + LPB * k;
             if (i >= tl.cnt) continue;
-            u64 h, key;
-            d_leaf(L, tl, S, i, h, key);
+            u64 xm, x2, h;
+            if (tl.fast) {
+                const FastCtx c(S);
+                int x;
+                u32 q;
+                h = leaf_fast(S, c, i, xm, x2, x, q);
+            } else {
+                LeafRef r;
+                h = leaf_generic(L, tl, S, i, xm, x2, r);
+            }
             const u64 b = hbucket(h, kh);
             const u32 slot = atomicAdd(&bcur[b], 1u);
             const u64 b0 = bstart[b];
-            DR[b0 + slot] = DRec{key, tl.j0 - L.lo + i};   // the leaf relative to the source's first
+            // the leaf relative to the source's first, and its fingerprint
+            DR[b0 + slot] = (tl.j0 - L.lo + i) | (u64)(hfp(h) & fmask) << DR_FP_SHIFT | (u64)hash_ex(h) << DR_EX_SHIFT;
             if (slot < (u32)TSLOTS) {
                 const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
                 T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | (hfp(h) & fmask));
             }
         }
         lds_sync();
-    }
-}
-
-// D leaf of a tile: hash and exact key
-__device__ __forceinline__ void d_leaf(const LeafSrc &L, const Tile &T, const TileLds &S, u32 i, u64 &h, u64 &key) {
-    u64 xm, x2;
-    if (T.fast) {
-        const FastCtx c(S);
-        int x;
-        u32 q;
-        h = leaf_fast(S, c, i, xm, x2, x, q);
-        key = xm + c.m * (((x2 - xm) * S.minv) & c.mask2) + c.koff;
-    } else {
-        LeafRef r;
-        h = leaf_generic(L, T, S, i, xm, x2, r);
-        key = cf_key(xm, x2, r.s->mc) + r.s->koff;
     }
 }
 
@@ -610,7 +589,6 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // *ovf (its excess records are not written), and the caller redoes the chunk with the exact layout.
 // The capacity layout needs no first pass: its segments are sized for the expected share
 // (hashing spreads records evenly) with a margin of about 8 standard deviations.
-// With KEYS (the D side) every record's exact key goes to keys[] at the record's slot.
 // Slots are 32-bit: a chunk holds at most 2^32 records, and the caller takes the capacity layout
 // only when it has fewer than 2^32 slots (~0 marks a record past its segment's end).
 //
@@ -621,16 +599,16 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // rank, with its partition and place in the tile in the staged high word; the store side reads the
 // staged records in order (so the lanes of a wave read the words of only a few partitions) and
 // takes each record's slot from its partition's word.
-template <bool KEYS, bool SPLIT = false>
+// DSIDE: the D side's records, whose leaf indices (below 2^27) carry hash_ex in their top 5 bits.
+template <bool SPLIT = false, bool DSIDE = false>
 __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                   const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
                                                   u32 *__restrict__ ovf, uint2 *__restrict__ rec,
-                                                  u64 *__restrict__ keys, u32 *__restrict__ rhi) {
+                                                  u32 *__restrict__ rhi) {
     __shared__ TileLds S;
     __shared__ u32 hist[256];
     __shared__ u64 lc[256];
     __shared__ u32 s_ovf;
-    __shared__ u64 s_k[KEYS ? LT : 1];
     // once a tile's leaves are hashed its parents' staged state is dead: the records are staged over
     // xm and x2 (32 KB)
     uint2 *s_r = (uint2 *)S.xm;
@@ -675,16 +653,15 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
         {
             // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
             u32 lo[LPT], pt[LPT];
-            u64 kk[KEYS ? LPT : 1];
             {
                 u64 h[LPT];
-                if constexpr (KEYS) tile_hash_keys<FULL>(L, T, S, h, kk);
-                else tile_hashes<FULL>(L, T, S, h);
+                tile_hashes<FULL>(L, T, S, h);
 #pragma unroll
                 for (int k = 0; k < LPT; ++k) {
                     const u32 hi = (u32)(h[k] >> 32);
                     pt[k] = hi >> (32 - kh.lgP);
                     lo[k] = (hoff32(hi, kh) << kh.fpb) | (hfp(h[k]) & fmask);
+                    if constexpr (DSIDE) pt[k] |= hash_ex(h[k]) << 8;   // staged above the partition
                 }
             }
             if (t + 1 < t1) {
@@ -694,7 +671,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             u32 rank[LPT];
 #pragma unroll
             for (int k = 0; k < LPT; ++k)
-                if (FULL || threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k]], 1u);
+                if (FULL || threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k] & 0xFFu], 1u);
             lds_sync();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
             for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
@@ -722,9 +699,8 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             for (int k = 0; k < LPT; ++k) {
                 const u32 i = threadIdx.x + LPB * k;
                 if (FULL || i < T.cnt) {
-                    const u32 at = (lc32[2 * pt[k]] & 0xFFFFu) + rank[k];
+                    const u32 at = (lc32[2 * (pt[k] & 0xFFu)] & 0xFFFFu) + rank[k];
                     s_r[at] = make_uint2(lo[k], (pt[k] << 16) | i);
-                    if constexpr (KEYS) s_k[at] = kk[k];
                 }
             }
             lds_sync();
@@ -759,9 +735,9 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                             ((u32 *)rec)[d] = r[k].x;
                             rhi[d] = jb + (r[k].y & 0xFFFFu);
                         } else {
-                            rec[d] = make_uint2(r[k].x, jb + (r[k].y & 0xFFFFu));
+                            rec[d] = make_uint2(r[k].x, (jb + (r[k].y & 0xFFFFu)) |
+                                                            (DSIDE ? (r[k].y >> 24) << DREC_EX_SHIFT : 0u));
                         }
-                        if constexpr (KEYS) keys[d] = s_k[q];
                     }
                 }
             }
@@ -793,28 +769,25 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     }
 }
 
-// I side (records only) and D side (records and exact keys): separate kernels so that profiles tell
-// them apart. The I side runs two blocks per CU (four waves per SIMD: at most 128 VGPRs); the D side's
-// key staging (32 KB more LDS) leaves it one.
+// I side and D side: the same records from separate kernels, so that profiles tell them apart. Two
+// blocks per CU (four waves per SIMD: at most 128 VGPRs).
 __global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec) {
-    part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr, nullptr);
+    part_scatter_body<false>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
+}
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_d(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec) {
+    part_scatter_body<false, true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
 }
 // The I side with its records split into two arrays of words (bucket words, then leaf indices), for
 // chunks probed against several tables: the probe then reads 4 bytes per record instead of 8.
 __global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_split(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec, u32 *__restrict__ rhi) {
-    part_scatter_body<false, true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr, rhi);
+    part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, rhi);
 }
-__global__ void __launch_bounds__(LPB) k_part_scatter_keys(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
-                                                          const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
-                                                          u32 *__restrict__ ovf, uint2 *__restrict__ rec,
-                                                          u64 *__restrict__ keys) {
-    part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, keys, nullptr);
-}
-
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).
 __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ninst, u64 key) {
     int a = 0, b = ninst - 1;
@@ -825,8 +798,8 @@ __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ni
     return (u32)a;
 }
 
-// Exact key and id of leaf lo + leaf (rare: only on a fingerprint hit).
-__device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key, u64 &id) {
+// Exact key and id of leaf lo + leaf (rare: only on a fingerprint hit), and its hash.
+__device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key, u64 &id, u64 *hash = nullptr) {
     const u64 j = L.lo + leaf;
     // the tile owner map (when given) brackets the parent to the tile's few nodes
     const size_t p = L.tp ? find_parent(L.off, L.tp[2 * (leaf / LT)], L.tp[2 * (leaf / LT) + 1], j)
@@ -838,6 +811,7 @@ __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key,
     leaf_residue(s, nd, x, xm, x2);
     key = cf_key(xm, x2, s.mc) + s.koff;
     id = id_update(s, nd.id, x, L.depth);
+    if (hash) *hash = leaf_hash(xm, x2, s.mc.mask2, s.koff);
 }
 
 // Is the slot pattern fs2 (0x8000 | f in both halves of a word) among the 8 slots of a bucket? Slots
@@ -1088,8 +1062,8 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
         if (cbase + i < ccap) cand[cbase + i] = CAND_NONE;
 }
 
-// ---- D side, partitioned: the D records were scattered by partition like the I side (with their
-// exact keys alongside). A partition's K buckets are cut into windows of WIN = 2^wb buckets
+// ---- D side, partitioned: the D records were scattered by partition like the I side ({bucket word,
+// leaf}, 8 bytes). A partition's K buckets are cut into windows of WIN = 2^wb buckets
 // (about 2048 D records each); two more passes bucket them with no per-record global atomic:
 //   window scatter: each block ranks a tile of its partition's records by window in LDS and
 //          reserves room per window with one atomic per (tile, window) in a fixed-capacity region
@@ -1110,13 +1084,13 @@ struct DWin {
 };
 
 __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLayout S, const uint2 *__restrict__ recs,
-                                                     const u64 *__restrict__ keys, u32 *__restrict__ wcount,
-                                                     uint4 *__restrict__ wreg, u32 *__restrict__ ovf) {
+                                                     u32 *__restrict__ wcount, uint2 *__restrict__ wreg,
+                                                     u32 *__restrict__ ovf) {
     __shared__ u32 hist[256];
     __shared__ u32 loc[256];
     __shared__ u32 woff[256];
     __shared__ u32 s_w[PB / 64];
-    __shared__ uint4 s_r[WPT * 512];
+    __shared__ uint2 s_r[WPT * 512];
     __shared__ u8_t s_win[WPT * 512];
     __shared__ u32 s_ovf;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1134,14 +1108,12 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
     for (u32 c0 = 0;; c0 += 512) {
         const u32 cn = c0 < fill ? (fill - c0 < 512u ? fill - c0 : 512u) : 0u;
         uint2 r[8];
-        u64 kk[8];
         u32 rank[8], win[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const u32 v = (u32)k * 64 + lane;
             if (v < cn) {
                 r[k] = recs[st + c0 + v];
-                kk[k] = keys[st + c0 + v];
                 win[k] = (r[k].x >> kh.fpb) >> dw.wb;
                 rank[k] = atomicAdd(&hist[win[k]], 1u);
             }
@@ -1166,7 +1138,7 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
             const u32 vv = (u32)k * 64 + lane;
             if (vv < cn) {
                 const u32 at = loc[win[k]] + rank[k] + 0u;
-                s_r[at] = make_uint4(r[k].x, r[k].y, (u32)kk[k], (u32)(kk[k] >> 32));
+                s_r[at] = r[k];
                 s_win[at] = (u8_t)win[k];
             }
         }
@@ -1193,12 +1165,12 @@ __global__ void __launch_bounds__(PB) k_dwin_scatter(KeyHash kh, DWin dw, SegLay
 
 constexpr int DSB = 512;               // threads of a window-sort block
 __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u32 *__restrict__ wcount,
-                                                   const u64 *__restrict__ wbase, const uint4 *__restrict__ wreg,
-                                                   u64 *__restrict__ bstart, DRec *__restrict__ DR,
+                                                   const u64 *__restrict__ wbase, const uint2 *__restrict__ wreg,
+                                                   u64 *__restrict__ bstart, u64 *__restrict__ DR,
                                                    unsigned short *__restrict__ T, u32 *__restrict__ F, int FB,
                                                    u32 *__restrict__ kerr) {
     constexpr int WMAXB = 4096;        // buckets a window may have
-    __shared__ uint4 s_r[CAPW];
+    __shared__ uint2 s_r[CAPW];
     __shared__ unsigned short s_idx[CAPW];
     __shared__ u32 pos[WMAXB + 1];     // per-bucket counts, then (scanned in place) bucket starts
     __shared__ u32 s_w[DSB / 64];
@@ -1275,10 +1247,12 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         if (i < n) s_idx[pos[(s_r[i].x >> kh.fpb) - (u32)b_lo] + rk[k]] = (unsigned short)i;
     }
     lds_sync();
-    // records in bucket order
+    // records (D leaves, fingerprints and hash_ex) in bucket order
+    const u32 fpm = (1u << kh.fpb) - 1u;
     for (u32 k = threadIdx.x; k < n; k += DSB) {
-        const uint4 r = s_r[s_idx[k]];
-        DR[base + k] = DRec{(u64)r.z | ((u64)r.w << 32), (u64)r.y};
+        const uint2 r = s_r[s_idx[k]];
+        DR[base + k] = (u64)(r.y & ((1u << DREC_EX_SHIFT) - 1u)) | (u64)(r.x & fpm) << DR_FP_SHIFT |
+                       (u64)(r.y >> DREC_EX_SHIFT) << DR_EX_SHIFT;
     }
     // table slots: one bucket per thread, 16 bytes each
     const u32 fmask = (1u << kh.fpb) - 1u;
@@ -1319,15 +1293,16 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     }
 }
 
-// Probe, pass 2: each candidate's exact key and id are recomputed from the node tree and compared
-// with the D records of its bucket; every equal key emits a pair.
+// Probe, pass 2: the D records of each candidate's bucket that agree with the candidate on the
+// fingerprint and hash_ex have their exact keys recomputed from their leaves; the candidate's own
+// key is recomputed once, at the first record with its fingerprint. Every equal key emits a pair.
 __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, KeyHash kh,
                                                          const u32 *__restrict__ recs,
                                                          SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
                                                          int ninst, const u64 *__restrict__ bstart,
-                                                         const DRec *__restrict__ DR,
+                                                         const u64 *__restrict__ DR,
                                                          unsigned long long *__restrict__ npairs, u64 cap,
                                                          u64 *__restrict__ out_d, u64 *__restrict__ out_i,
                                                          u64 *__restrict__ out_inst, u32 *__restrict__ kerr) {
@@ -1351,14 +1326,24 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
         }
         const uint2 r = S.hi ? make_uint2(recs[e], S.hi[e]) : ((const uint2 *)recs)[e];
         const u64 b = (g / S.nblk) * kh_k(kh) + (r.x >> kh.fpb);
-        u64 key, iid;
-        leaf_exact(L, r.y, key, iid);
+        const u64 fp = r.x & ((1u << kh.fpb) - 1u);
+        u64 key = 0, iid = 0;
+        u32 ex = 0;
+        bool have = false;
         const u64 lo = bstart[b], hi = bstart[b + 1];
         for (u64 q = lo; q < hi; ++q) {
-            const DRec d = DR[q];
-            if (d.key != key) continue;
+            const u64 dr = DR[q];
+            if (((dr >> DR_FP_SHIFT) & 0x3FFFu) != fp) continue;
+            if (!have) {
+                u64 h;
+                leaf_exact(L, r.y, key, iid, &h);
+                ex = hash_ex(h);
+                have = true;
+            }
+            if ((u32)(dr >> DR_EX_SHIFT) != ex) continue;
             u64 dkey, did;
-            leaf_exact(LD, d.ref, dkey, did);
+            leaf_exact(LD, dr & DR_LEAF_MASK, dkey, did);
+            if (dkey != key) continue;
             const unsigned long long at = atomicAdd(npairs, 1ULL);
             if (at < cap) {
                 out_d[at] = did;
@@ -1449,10 +1434,10 @@ void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, 
     DBuf<u64> tp;
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
     CF_LAUNCH(ctx, "k_bucket_scatter", k_bucket_scatter, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bstart, bcur,
-              (DRec *)DR, T);
+              (u64 *)DR, T);
 }
 
-// D side, partitioned (after part_scatter_cap / part_scatter_exact with keys): window geometry,
+// D side, partitioned (after part_scatter_cap / part_scatter_exact of the D side): window geometry,
 // window scatter and window sort.
 bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R) {
     // about 2048 D records per window at the group's records per bucket
@@ -1467,21 +1452,21 @@ bool dwin_geometry(const KeyHash &kh, u64 nD, int &wb, u32 &R) {
 }
 
 void part_d_windows(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const SegLayout &S, const u64 *rec,
-                    const u64 *keys, u32 *wcount, void *wreg, u32 *ovf) {
+                    u32 *wcount, void *wreg, u32 *ovf) {
     const int P = 1 << kh.lgP;
     fill_u32(ctx, wcount, (size_t)P * R, 0);
     fill_u32(ctx, ovf, 1, 0);
     if (!S.nseg) return;
     const u64 tpp = (S.nblk + WPT - 1) / WPT;
     CF_LAUNCH(ctx, "k_dwin_scatter", k_dwin_scatter, dim3((unsigned)(tpp * P)), dim3(PB), 0, kh, DWin{wb, R}, S,
-              (const uint2 *)rec, keys, wcount, (uint4 *)wreg, ovf);
+              (const uint2 *)rec, wcount, (uint2 *)wreg, ovf);
 }
 
 void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcount, const u64 *wbase,
                  const void *wreg, u64 *bstart, void *DR, unsigned short *T, u32 *F, int FB, u32 *kerr) {
     const int P = 1 << kh.lgP;
     CF_LAUNCH(ctx, "k_dwin_sort", k_dwin_sort, dim3((unsigned)((u64)P * R)), dim3(DSB), 0, kh, DWin{wb, R}, wcount,
-              wbase, (const uint4 *)wreg, bstart, (DRec *)DR, T, F, FB, kerr);
+              wbase, (const uint2 *)wreg, bstart, (u64 *)DR, T, F, FB, kerr);
 }
 
 u64 dwin_capacity() { return CAPW; }
@@ -1510,15 +1495,15 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_fits
 
 void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
-                      DBuf<u64> &tp, u64 *keys, u64 hi_at) {
-    CF_REQUIRE(!(keys && hi_at), "D records are {word, leaf} pairs");
+                      DBuf<u64> &tp, bool dside, u64 hi_at) {
+    CF_REQUIRE(!(dside && hi_at), "D records are {word, leaf} pairs");
     lay.hi = hi_at ? (u32 *)rec + hi_at : nullptr;
     if (!cap_fits(n, kh, tn)) {
         // too many segments for 32-bit slots (a chunk near 2^32 leaves at a low fill): the exact
         // layout directly, never overflowing
         fill_u32(ctx, ovf, 1, 0);
         (void)leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-        part_scatter_exact(ctx, descs, nodes, nn, off, lo, n, depth, kh, lay, rec, tp.p, keys, hi_at);
+        part_scatter_exact(ctx, descs, nodes, nn, off, lo, n, depth, kh, lay, rec, tp.p, dside, hi_at);
         return;
     }
     u64 tpb;
@@ -1531,9 +1516,9 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     lay.fill = DBuf<u32>(ctx, lay.nseg);
     fill_u32(ctx, ovf, 1, 0);
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, keys);
+    if (dside)
+        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
+                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
     else if (lay.hi)
         CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, lay.hi);
@@ -1544,8 +1529,8 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
-                        u64 *keys, u64 hi_at) {
-    CF_REQUIRE(!(keys && hi_at), "D records are {word, leaf} pairs");
+                        bool dside, u64 hi_at) {
+    CF_REQUIRE(!(dside && hi_at), "D records are {word, leaf} pairs");
     CF_REQUIRE(!hi_at || hi_at >= n, "the split records need room for every record");
     lay.hi = hi_at ? (u32 *)rec + hi_at : nullptr;
     const int P = 1 << kh.lgP;
@@ -1561,9 +1546,9 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     lay.O = DBuf<u64>(ctx, lay.nseg + 1);
     CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(LPB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
-    if (keys)
-        CF_LAUNCH(ctx, "k_part_scatter_keys", k_part_scatter_keys, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, keys);
+    if (dside)
+        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
+                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
     else if (lay.hi)
         CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
                   (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, lay.hi);
@@ -1621,7 +1606,7 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, dl.lo, dl.n, depth, dl.tp};
     CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const u32 *)rec, S,
               ncand, ccap,
-              cand, koffs, ninst, bstart, (const DRec *)DR, npairs, cap, od, oi, oinst, kerr);
+              cand, koffs, ninst, bstart, (const u64 *)DR, npairs, cap, od, oi, oinst, kerr);
 }
 
 void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {

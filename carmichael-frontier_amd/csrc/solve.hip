@@ -350,7 +350,8 @@ struct DTable {
 };
 
 // The D leaves of a group in nr ranges of at most m records: one range while the window build takes
-// the whole side (fewer than 2^32 records and at most 256 windows per partition), else the fewest
+// the whole side (fewer than 2^27 records, whose leaf indices leave 5 bits of a 32-bit word to the
+// hash bits the records carry, and at most 256 windows per partition), else the fewest
 // equal ranges it takes one by one. The global-atomic build (d_part false) takes one range of any size.
 void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
     nr = 1;
@@ -361,7 +362,7 @@ void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
         const Partitioning q = assign_buckets(m, tn);
         int wb = 0;
         u32 R = 0;
-        if (m < ((u64)1 << 32) && dwin_geometry(q.kh, m, wb, R)) return;
+        if (m < ((u64)1 << 27) && dwin_geometry(q.kh, m, wb, R)) return;
         if (nr >= 65536) break;
         nr = nr < 8 ? nr + 1 : nr + nr / 8;
     }
@@ -376,27 +377,28 @@ void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
 void build_dtable(cf_ctx *ctx, const SideDesc *d_desc, const Frontier &fd, int depth, const Partitioning &pt, int FB,
                   bool d_part, u64 lo, u64 m, const Tuning &tn, DTable &t, u32 *ovf, u32 *kerr, Tape *tape) {
     const KeyHash &kh = pt.kh;
+    CF_REQUIRE(m < ((u64)1 << 40), "a D table holds fewer than 2^40 records (DR keeps 40-bit leaf indices)");
     t.lo = lo;
     t.n = m;
     t.bstart = DBuf<u64>(ctx, pt.NB + 1);
-    t.DR = DBuf<u64>(ctx, 2 * (m ? m : 1));
+    t.DR = DBuf<u64>(ctx, m ? m : 1);
     t.T = DBuf<unsigned short>(ctx, pt.NB * CF_TABLE_SLOTS);
     t.F = DBuf<u32>(ctx, std::max<u64>(filter_words(kh, FB), 1));
     if (!m) return;
     fill_u32(ctx, t.F.p, filter_words(kh, FB), 0);
     int wb = 0;
     u32 R = 0;
-    if (d_part && m < ((u64)1 << 32) && dwin_geometry(kh, m, wb, R)) {
+    if (d_part && m < ((u64)1 << 27) && dwin_geometry(kh, m, wb, R)) {
         IChunkLayout dl;
         const u64 slots = std::max<u64>(part_cap_slots(m, kh, tn), m);
-        DBuf<u64> drec(ctx, slots), dkeys(ctx, slots);
-        part_scatter_cap(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, tn, dl, drec.p, ovf, t.tp, dkeys.p);
+        DBuf<u64> drec(ctx, slots);
+        part_scatter_cap(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, tn, dl, drec.p, ovf, t.tp, true);
         if (host_value_u32(ctx, tape, ovf))
-            part_scatter_exact(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, dl, drec.p, t.tp.p, dkeys.p);
+            part_scatter_exact(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, dl, drec.p, t.tp.p, true);
         const u64 nw = (u64)pt.P() * R;
         DBuf<u32> wcount(ctx, nw);
-        DBuf<u64> wreg(ctx, 2 * nw * dwin_capacity()), wbase(ctx, nw + 1);
-        part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, dkeys.p, wcount.p, wreg.p, ovf);
+        DBuf<u64> wreg(ctx, nw * dwin_capacity()), wbase(ctx, nw + 1);
+        part_d_windows(ctx, kh, wb, R, dl.view(), drec.p, wcount.p, wreg.p, ovf);
         if (!host_value_u32(ctx, tape, ovf)) {
             scan_u32_to_u64_async(ctx, wcount.p, wbase.p, nw);
             part_d_sort(ctx, kh, wb, R, wcount.p, wbase.p, wreg.p, t.bstart.p, t.DR.p, t.T.p, t.F.p, FB, kerr);
@@ -557,7 +559,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     // two streams less efficiently, which one probe does not repay)
                     const u64 split_at = tabs.size() >= 2 ? cap_m : 0;
                     part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
-                                     ovf.p, tp, nullptr, split_at);
+                                     ovf.p, tp, false, split_at);
                     for (size_t k = 0; k < tabs.size(); ++k) {
                         const DTable &tb = tabs[k];
                         part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
@@ -568,7 +570,7 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                             nc = host_value(ctx, tape, flags.p);
                             if (ov) {
                                 part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                                   rec.p, tp.p, nullptr, split_at);
+                                                   rec.p, tp.p, false, split_at);
                                 part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
                                                  (unsigned long long *)flags.p, ccap, cand.p, b->tune);
                                 nc = host_value(ctx, tape, flags.p);
@@ -715,9 +717,10 @@ int cf_batch_footprint(cf_batch *b, uint64_t *bytes, uint64_t *nD_total, uint64_
         u64 nr = 1, m = nD;
         d_ranges(nD, b->tune, true, nr, m);
         const Partitioning pt = assign_buckets(m, b->tune);
-        const u64 tables = 16 * nD + 30 * pt.NB * nr;       // DR, T, bstart, counters, filter of every range
-        // + one range's partitioned D records and keys + D parents (kept through the I phase for pass 2)
-        const u64 d_build = tables + 16 * std::max<u64>(part_cap_slots(m, pt.kh, b->tune), m) + 32 * m + 60 * nnD;
+        const u64 tables = 8 * nD + 30 * pt.NB * nr;        // DR, T, bstart, counters, filter of every range
+        // + one range's partitioned D records and window regions + D parents (kept through the I phase
+        // for pass 2)
+        const u64 d_build = tables + 8 * std::max<u64>(part_cap_slots(m, pt.kh, b->tune), m) + 16 * m + 60 * nnD;
         const u64 i_phase = tables + 8 * std::max<u64>(part_cap_slots(nI, pt.kh, b->tune), nI) +
                             60 * (nnI + nnD) + nI / 4 + nD / 128;
         peak = std::max(peak, std::max(d_build, i_phase));

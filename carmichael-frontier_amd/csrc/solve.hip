@@ -477,7 +477,8 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
         ms_d = ms_i = 0;
         parents_d = parents_i = 0;
         // each radius group: its D side is bucketed into its own probe table, then its I side is
-        // generated, partitioned and probed against it. Eager: one group after another. Captured:
+        // generated, partitioned and probed against it. Eager: one group after another (largest
+        // radius first). Captured:
         // each group on its own branch (side stream), all forked from and joined back to the
         // capturing stream, so a replay runs the groups side by side.
         const bool fork = capturing && groups.size() > 1;
@@ -501,7 +502,11 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
             CF_HIP(hipEventRecord(ev_fork, ctx->stream));
         }
         size_t gi = 0;
-        for (auto &g : groups) {
+        // largest radius first: a replay's kernels are submitted branch after branch, so the group
+        // with the most work (the largest radius in the family batches) starts at once instead of
+        // after the submission of every other group's launches
+        for (auto git = groups.rbegin(); git != groups.rend(); ++git) {
+            auto &g = *git;
             const size_t si = gi % std::max<size_t>(b->side.size(), 1);
             GroupBranch branch(ctx, fork ? b->side[si] : nullptr, ev_fork, fork ? ev_done[gi] : nullptr, (int)si + 1);
             ++gi;

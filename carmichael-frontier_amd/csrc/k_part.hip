@@ -563,9 +563,9 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
 // every record to its final slot, one contiguous run per partition and block.
 __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, u32 nblk, u32 *__restrict__ H) {
     __shared__ TileLds S;
-    __shared__ u32 hist[256];
+    __shared__ u32 hist[512];
     const int P = 1 << lgP;
-    if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+    if (threadIdx.x < 512) hist[threadIdx.x] = 0;
     tile_lds_init(S);
     const u64 ntiles = (L.n + LT - 1) / LT;
     const u64 t0 = (u64)blockIdx.x * tpb;
@@ -593,21 +593,26 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // only when it has fewer than 2^32 slots (~0 marks a record past its segment's end).
 //
 // Per partition the block keeps its next slot gb and the room left in its segment rm (no limit in
-// the exact layout), in registers of wave 0 (lane l: partitions 4l .. 4l + 3); per tile, wave 0
-// scans the partition counts and writes one 64-bit word lc[p] = gb << 32 | min(room, 0xFFFF) << 16 |
-// the partition's offset in the tile. A ranked record is staged at its partition's offset plus its
+// the exact layout), in registers of the scan waves (wave w, lane l: partitions 256w + 4l .. +3; one
+// wave for up to 256 partitions, two for 512); per tile, the scan waves scan the partition counts
+// (wave 1 adds wave 0's total, which it sums itself) and write one 64-bit word
+// lc[p] = gb << 32 | min(room, 0xFFFF) << 16 | the partition's offset in the tile. A ranked record is staged at its partition's offset plus its
 // rank, with its partition and place in the tile in the staged high word; the store side reads the
 // staged records in order (so the lanes of a wave read the words of only a few partitions) and
 // takes each record's slot from its partition's word.
 // DSIDE: the D side's records, whose leaf indices (below 2^27) carry hash_ex in their top 5 bits.
-template <bool SPLIT = false, bool DSIDE = false>
+// LGMAXP: partitions at most 2^LGMAXP (8 or 9); the staged high word holds the partition from bit 16
+// and, on the D side, hash_ex above it.
+template <bool SPLIT = false, bool DSIDE = false, int LGMAXP = 8>
 __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk,
                                                   const u64 *__restrict__ O, int capl, u32 *__restrict__ fill,
                                                   u32 *__restrict__ ovf, uint2 *__restrict__ rec,
                                                   u32 *__restrict__ rhi) {
+    constexpr u32 MAXP = 1u << LGMAXP, PM = MAXP - 1;
+    constexpr u32 NSW = MAXP / 256;                // scan waves
     __shared__ TileLds S;
-    __shared__ u32 hist[256];
-    __shared__ u64 lc[256];
+    __shared__ u32 hist[MAXP];
+    __shared__ u64 lc[MAXP];
     __shared__ u32 s_ovf;
     // once a tile's leaves are hashed its parents' staged state is dead: the records are staged over
     // xm and x2 (32 KB)
@@ -617,13 +622,15 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     const u32 fmask = (1u << kh.fpb) - 1u;
     const bool exact = O != nullptr;
     const u32 cap = 1u << capl;
-    constexpr int PPL = 256 / 64;   // partitions per lane of wave 0
+    constexpr int PPL = 256 / 64;   // partitions per lane of a scan wave
+    const bool scanw = threadIdx.x < 64 * NSW;
+    const u32 pb = (threadIdx.x >> 6) * 256 + (threadIdx.x & 63) * PPL;   // this lane's first partition
     u32 gb[PPL], rm[PPL], hv[PPL];
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
-        const u32 p = threadIdx.x * PPL + j;
+        const u32 p = pb + j;
         const u64 g = (u64)p * nblk + blockIdx.x;
-        gb[j] = threadIdx.x < 64 && p < (u32)P ? (exact ? (u32)O[g] : (u32)(g << capl)) : 0u;
+        gb[j] = scanw && p < (u32)P ? (exact ? (u32)O[g] : (u32)(g << capl)) : 0u;
         rm[j] = exact ? 0xFFFFu : cap;
         hv[j] = 0;
     }
@@ -648,7 +655,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     // run in loops of their own.
     auto tile = [&](u64 t, auto full_tag) {
         constexpr bool FULL = decltype(full_tag)::value;
-        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+        if (threadIdx.x < MAXP) hist[threadIdx.x] = 0;
         const Tile T = tile_setup_pf(L, t, S, pf);
         {
             // each leaf's record word (bucket offset in the partition << fpb | fingerprint) and partition
@@ -661,7 +668,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                     const u32 hi = (u32)(h[k] >> 32);
                     pt[k] = hi >> (32 - kh.lgP);
                     lo[k] = (hoff32(hi, kh) << kh.fpb) | (hfp(h[k]) & fmask);
-                    if constexpr (DSIDE) pt[k] |= hash_ex(h[k]) << 8;   // staged above the partition
+                    if constexpr (DSIDE) pt[k] |= hash_ex(h[k]) << LGMAXP;   // staged above the partition
                 }
             }
             if (t + 1 < t1) {
@@ -671,22 +678,30 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             u32 rank[LPT];
 #pragma unroll
             for (int k = 0; k < LPT; ++k)
-                if (FULL || threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k] & 0xFFu], 1u);
+                if (FULL || threadIdx.x + LPB * k < T.cnt) rank[k] = atomicAdd(&hist[pt[k] & PM], 1u);
             lds_sync();   // also: every lane is done with the staged parents and the owner map
 #pragma unroll
             for (int k = 0; k < LPT; ++k) S.own[threadIdx.x * LPT + k] = 0;   // clear for the next tile
-            // tile-local exclusive offsets of the partitions: wave 0 scans the counts, PPL per lane
-            if (threadIdx.x < 64) {
+            // tile-local exclusive offsets of the partitions: the scan waves scan the counts, PPL per lane
+            if (scanw) {
                 u32 sum = 0;
 #pragma unroll
                 for (int j = 0; j < PPL; ++j) {
-                    hv[j] = hist[threadIdx.x * PPL + j];
+                    hv[j] = hist[pb + j];
                     sum += hv[j];
                 }
                 u32 off = wave_incl_sum(sum) - sum;
+                if (NSW > 1 && threadIdx.x >= 64) {
+                    // partitions 0 .. 255 come first: wave 1 sums wave 0's counts itself
+                    const u32 l = threadIdx.x & 63;
+                    u32 s0 = 0;
+#pragma unroll
+                    for (int j = 0; j < PPL; ++j) s0 += hist[l * PPL + j];
+                    off += (u32)__builtin_amdgcn_readlane((int)wave_incl_sum(s0), 63);
+                }
 #pragma unroll
                 for (int j = 0; j < PPL; ++j) {
-                    const u32 p = threadIdx.x * PPL + j;
+                    const u32 p = pb + j;
                     if (p < (u32)P)
                         lc[p] = ((u64)gb[j] << 32) | ((rm[j] < 0xFFFFu ? rm[j] : 0xFFFFu) << 16) | off;
                     off += hv[j];
@@ -699,7 +714,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
             for (int k = 0; k < LPT; ++k) {
                 const u32 i = threadIdx.x + LPB * k;
                 if (FULL || i < T.cnt) {
-                    const u32 at = (lc32[2 * (pt[k] & 0xFFu)] & 0xFFFFu) + rank[k];
+                    const u32 at = (lc32[2 * (pt[k] & PM)] & 0xFFFFu) + rank[k];
                     s_r[at] = make_uint2(lo[k], (pt[k] << 16) | i);
                 }
             }
@@ -720,7 +735,7 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
 #pragma unroll
             for (int k = 0; k < LPT; ++k) r[k] = s_r[threadIdx.x + LPB * k];
 #pragma unroll
-            for (int k = 0; k < LPT; ++k) w[k] = lc[(r[k].y >> 16) & 0xFFu];
+            for (int k = 0; k < LPT; ++k) w[k] = lc[(r[k].y >> 16) & PM];
             u32 over = 0;
 #pragma unroll
             for (int k = 0; k < LPT; ++k) {
@@ -736,14 +751,14 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
                             rhi[d] = jb + (r[k].y & 0xFFFFu);
                         } else {
                             rec[d] = make_uint2(r[k].x, (jb + (r[k].y & 0xFFFFu)) |
-                                                            (DSIDE ? (r[k].y >> 24) << DREC_EX_SHIFT : 0u));
+                                                            (DSIDE ? (r[k].y >> (16 + LGMAXP)) << DREC_EX_SHIFT : 0u));
                         }
                     }
                 }
             }
             if (over) s_ovf = 1;
         }
-        if (threadIdx.x < 64) {
+        if (scanw) {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
                 gb[j] += hv[j];
@@ -758,10 +773,10 @@ __device__ __forceinline__ void part_scatter_body(LeafSrc L, KeyHash kh, u64 tpb
     for (; t < t1; ++t) tile(t, std::false_type{});
     lds_sync();
     if (!exact) {
-        if (threadIdx.x < 64) {
+        if (scanw) {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
-                const u32 p = threadIdx.x * PPL + j;
+                const u32 p = pb + j;
                 if (p < (u32)P) fill[(u64)p * nblk + blockIdx.x] = cap - rm[j];
             }
         }
@@ -787,6 +802,22 @@ __global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k
                                                      int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
                                                      uint2 *__restrict__ rec, u32 *__restrict__ rhi) {
     part_scatter_body<true>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, rhi);
+}
+// The same three for geometries of 512 partitions (large D sides).
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_p512(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec) {
+    part_scatter_body<false, false, 9>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
+}
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_d_p512(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec) {
+    part_scatter_body<false, true, 9>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, nullptr);
+}
+__global__ void __launch_bounds__(LPB) __attribute__((amdgpu_waves_per_eu(4))) k_part_scatter_split_p512(LeafSrc L, KeyHash kh, u64 tpb, u32 nblk, const u64 *__restrict__ O,
+                                                     int capl, u32 *__restrict__ fill, u32 *__restrict__ ovf,
+                                                     uint2 *__restrict__ rec, u32 *__restrict__ rhi) {
+    part_scatter_body<true, false, 9>(L, kh, tpb, nblk, O, capl, fill, ovf, rec, rhi);
 }
 // Instance of a key: the last koff <= key (koffs ascending, koffs[0] = 0).
 __device__ __forceinline__ u32 inst_of_key(const u64 *__restrict__ koffs, int ninst, u64 key) {
@@ -1471,6 +1502,31 @@ void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcoun
 
 u64 dwin_capacity() { return CAPW; }
 
+// The partition scatter for the layout's side and record form, and for the geometry's partition count
+// (the 512-partition kernels only where the geometry has more than 256).
+static void launch_part_scatter(cf_ctx *ctx, const LeafSrc &L, const KeyHash &kh, u64 tpb, u32 nblk, const u64 *O,
+                                int capl, u32 *fill, u32 *ovf, u64 *rec, u32 *hi, bool dside) {
+    const bool wide = kh.lgP > 8;
+    if (dside && wide)
+        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d_p512, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O, capl,
+                  fill, ovf, (uint2 *)rec);
+    else if (dside)
+        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O, capl, fill,
+                  ovf, (uint2 *)rec);
+    else if (hi && wide)
+        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split_p512, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O,
+                  capl, fill, ovf, (uint2 *)rec, hi);
+    else if (hi)
+        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O, capl,
+                  fill, ovf, (uint2 *)rec, hi);
+    else if (wide)
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter_p512, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O, capl, fill,
+                  ovf, (uint2 *)rec);
+    else
+        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk, O, capl, fill, ovf,
+                  (uint2 *)rec);
+}
+
 // Capacity layout of a chunk of n I leaves: blocks of tpb tiles whose expected share per partition
 // is a set fraction of a 2^9-slot segment (default 70%: mean ~350, segment ~ mean + 8 sd).
 constexpr int CAPL = 9;
@@ -1516,15 +1572,7 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     lay.fill = DBuf<u32>(ctx, lay.nseg);
     fill_u32(ctx, ovf, 1, 0);
     LeafSrc L = leaf_src(ctx, descs, nodes, nn, off, lo, n, depth, tp);
-    if (dside)
-        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
-    else if (lay.hi)
-        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec, lay.hi);
-    else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)nullptr, CAPL, lay.fill.p, ovf, (uint2 *)rec);
+    launch_part_scatter(ctx, L, kh, tpb, nblk, nullptr, CAPL, lay.fill.p, ovf, rec, lay.hi, dside);
 }
 
 void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
@@ -1546,15 +1594,7 @@ void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     lay.O = DBuf<u64>(ctx, lay.nseg + 1);
     CF_LAUNCH(ctx, "k_part_hist", k_part_hist, dim3(nblk), dim3(LPB), 0, L, kh.lgP, tpb, nblk, lay.fill.p);
     scan_u32_to_u64_async(ctx, lay.fill.p, lay.O.p, lay.nseg);
-    if (dside)
-        CF_LAUNCH(ctx, "k_part_scatter_d", k_part_scatter_d, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
-    else if (lay.hi)
-        CF_LAUNCH(ctx, "k_part_scatter_split", k_part_scatter_split, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec, lay.hi);
-    else
-        CF_LAUNCH(ctx, "k_part_scatter", k_part_scatter, dim3(nblk), dim3(LPB), 0, L, kh, tpb, nblk,
-                  (const u64 *)lay.O.p, 0, (u32 *)nullptr, (u32 *)nullptr, (uint2 *)rec);
+    launch_part_scatter(ctx, L, kh, tpb, nblk, lay.O.p, 0, nullptr, nullptr, rec, lay.hi, dside);
 }
 
 u64 cand_margin(const Tuning &tn) {

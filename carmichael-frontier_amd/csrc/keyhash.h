@@ -13,7 +13,7 @@
 #include "modarith.h"
 
 struct KeyHash {
-    int lgP;    // 4 <= lgP <= 8
+    int lgP;    // 4 <= lgP <= CF_MAX_LGP
     int lgR;    // lgP <= lgR <= 24
     int fpb;    // fingerprint bits kept in an I record (<= 14)
     int pad;
@@ -23,6 +23,8 @@ struct KeyHash {
 constexpr u64 CF_RANGE_BUCKETS_LDS = 8192;   // most buckets per hash range
 constexpr int CF_RANGE_BITS_LDS = 12;         // most hash-range bits (2^12 ranges)
 constexpr int CF_TABLE_SLOTS = 8;             // 16-bit fingerprint slots per bucket (16 bytes)
+constexpr int CF_MAX_LGP = 9;                 // most partition bits (512 partitions; the scatter's
+                                              // 512-partition kernels take geometries above 256)
 
 CF_HD u64 kh_nb(const KeyHash &kh) { return kh.KR << kh.lgR; }
 CF_HD u64 kh_k(const KeyHash &kh) { return kh.KR << (kh.lgR - kh.lgP); }
@@ -66,6 +68,10 @@ inline bool kh_choose_at(u64 nD, KeyHash &kh, u64 dpb, u64 slice_bytes) {
     const u64 want = nD / dpb > 16 ? nD / dpb : 16;
     int lgP = 4;
     while (lgP < 8 && want * (2 * CF_TABLE_SLOTS) / ((u64)1 << lgP) > slice_bytes) ++lgP;
+    // 512 partitions shorten the scatters' runs (8 records per partition and tile): taken only when
+    // 256 would leave slices above 1.5 times the target (large D sides; a batch's radius group just
+    // above the target keeps 256)
+    if (lgP == 8 && CF_MAX_LGP > 8 && 2 * (want * (2 * CF_TABLE_SLOTS) / 256) > 3 * slice_bytes) lgP = 9;
     int lgR = lgP;
     while (lgR < CF_RANGE_BITS_LDS && ((want + ((u64)1 << lgR) - 1) >> lgR) > CF_RANGE_BUCKETS_LDS) ++lgR;
     const u64 KR = (want + ((u64)1 << lgR) - 1) >> lgR;

@@ -35,7 +35,7 @@ PEAK_HBM_GBS = 8000.0
 #   XCD probes one partition at a time and the partition's table slice fits its L2), so they are
 #   not HBM bytes; the measured traffic (FETCH_SIZE) shows what they cost when they do miss.
 KERNEL_BYTES_PER_I = {"k_part_scatter": 8, "k_part_probe": 8}
-NODE_BYTES = 48 + 8   # a last-level tree node and its entry of the leaf-offset scan
+NODE_BYTES = 32 + 8   # a last-level tree node (LeafNode) and its entry of the leaf-offset scan
 
 
 def run_batched(batch):
@@ -668,7 +668,7 @@ def main():
         if traffic:
             roofline["traffic_vs_algorithmic"] = round(traffic / (per_rec * nI_total / max(k_launches, 1)), 3)
             if dominant == "k_part_scatter" and detail:
-                # the scatter also reads each last-level I node (48 bytes) and its leaf offset (8 bytes):
+                # the scatter also reads each last-level I node (32 bytes) and its leaf offset (8 bytes):
                 # the reads its record writes do not account for
                 pb = NODE_BYTES * st_prof["nI_parents"] / max(k_launches, 1)
                 roofline["parents"] = {"nodes_per_step": st_prof["nI_parents"], "bytes_per_launch": round(pb),

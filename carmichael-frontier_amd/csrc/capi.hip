@@ -195,29 +195,39 @@ void plan_expand(cf_plan *pl) {
     DBuf<Node> rootd(ctx, 1);
     h2d(ctx, rootd.p, &root, 1);
     if (sd.n == 0 || sd.n < sd.r) { pl->empty = true; return; }
-    // level by level, each pass writing a level and counting its nodes' children
+    // level by level, each pass writing a level and counting its nodes' children; the last interior
+    // level as LeafNodes
     DBuf<Node> cur;
     const Node *curp = rootd.p;
     size_t ncur = 1;
     DBuf<u32> c(ctx, 1);
     launch_expand_count(ctx, pl->d_desc.p, curp, 1, 0, c.p);
+    DBuf<LeafNode> last;
     for (int depth = 0; depth + 1 < sd.r; ++depth) {
         DBuf<u64> o(ctx, ncur + 1);
         const u64 tot = scan_u32_to_u64(ctx, c.p, o.p, ncur);
         if (tot == 0) { pl->empty = true; return; }
-        DBuf<Node> nx(ctx, tot);
         DBuf<u32> cn(ctx, tot);
-        launch_expand_write_count(ctx, pl->d_desc.p, curp, ncur, o.p, tot, depth, nx.p, cn.p);
-        cur = std::move(nx);
+        if (depth + 2 < sd.r) {
+            DBuf<Node> nx(ctx, tot);
+            launch_expand_write_count(ctx, pl->d_desc.p, curp, ncur, o.p, tot, depth, nx.p, cn.p);
+            cur = std::move(nx);
+            curp = cur.p;
+        } else {
+            last = DBuf<LeafNode>(ctx, tot);
+            launch_expand_write_count(ctx, pl->d_desc.p, curp, ncur, o.p, tot, depth, last.p, cn.p);
+        }
         c = std::move(cn);
-        curp = cur.p;
         ncur = tot;
     }
-    if (!cur.p) cur = std::move(rootd);   // r == 1: the root is the last level
+    if (!last.p) {   // r == 1: the root is the last level
+        last = DBuf<LeafNode>(ctx, 1);
+        launch_leaf_nodes(ctx, rootd.p, 1, last.p);
+    }
     pl->off = DBuf<u64>(ctx, ncur + 1);
     pl->total = scan_u32_to_u64(ctx, c.p, pl->off.p, ncur);
     pl->cnt = std::move(c);
-    pl->leafp = std::move(cur);
+    pl->leafp = std::move(last);
     pl->nleafp = ncur;
     if (pl->total == 0) pl->empty = true;
 }

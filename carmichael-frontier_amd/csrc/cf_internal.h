@@ -161,6 +161,15 @@ struct __align__(16) Node {
 };
 static_assert(sizeof(Node) == 48, "node layout");
 
+// A node of a tree's last interior level, as the leaf kernels read it: the node without its exact
+// product, which decides only the node's own child count (taken when the level is written).
+struct __align__(16) LeafNode {
+    u64 xm, x2;      // as Node
+    u64 id;
+    u32 pos, meta;
+};
+static_assert(sizeof(LeafNode) == 32, "leaf-node layout");
+
 // Device-side description of one instance side (I or D). Arrays live in device memory.
 struct SideDesc {
     ModCtx mc;
@@ -205,7 +214,7 @@ struct cf_plan {
     SideSpec spec;
     SideTables tab;
     DBuf<SideDesc> d_desc;
-    DBuf<Node> leafp;      // depth r-1 nodes
+    DBuf<LeafNode> leafp;  // depth r-1 nodes
     size_t nleafp = 0;
     DBuf<u32> cnt;         // leaves per depth r-1 node
     DBuf<u64> off;         // exclusive scan of cnt, nleafp + 1 entries
@@ -241,26 +250,31 @@ void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
 // own child count (cnt, nout entries)
 void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
                                size_t nout, int depth, Node *out, u32 *cnt);
+// the same for the last interior level, written as LeafNodes
+void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, LeafNode *out, u32 *cnt);
+// nodes (roots of radius 1) as LeafNodes
+void launch_leaf_nodes(cf_ctx *ctx, const Node *nodes, size_t nn, LeafNode *out);
 // records (key, id) of leaves [lo, lo + n) of a last level, record j written at j - lo (leaf tiles
 // with their parents staged in LDS)
-void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void leaf_records(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                   int depth, u64 *keys, u64 *ids);
 void launch_gen_rank(cf_ctx *ctx, const SideDesc *d_desc, u64 lo, size_t count, u64 *keys, u64 *ids);
 // the same records with runs sharing their first r - 1 indices computed once (one wave per 64
 // consecutive prefixes); rank range checked against C(n, r)
 void launch_gen_prefix(cf_ctx *ctx, const SideDesc *d_desc, int n, int r, u64 lo, size_t count, u64 *keys, u64 *ids);
 // per first index x: lo[x], hi[x] = leaf offsets where its nodes start and end (zeroed by the caller)
-void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi);
+void launch_first_bounds(cf_ctx *ctx, const LeafNode *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi);
 u64 max_u64(cf_ctx *ctx, const u64 *a, size_t n);   // synchronous
 
 // radix-partitioned join (k_part.hip)
 void fill_u64(cf_ctx *ctx, u64 *a, size_t n, u64 v);
 void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v);
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount);
 // DR: the n D records in bucket order (leaf index relative to lo | fingerprint << 40); T:
 // CF_TABLE_SLOTS u16 slots per bucket, zeroed
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                          int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T);
 // validated side specs of an instance (the preconditions of cf_plan_create_d / _i); throw on a violation
 // the modulus an instance's keys live in: key_M = M for M < 2^64, else a divisor of M below 2^64
@@ -311,11 +325,11 @@ u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn);
 // bucket words from rec on, leaf indices from word hi_at on (rec holding at least 2 * hi_at words);
 // the probe reads only the first. dside: the D side's records (the same records; a kernel of its own,
 // so that profiles tell the sides apart).
-void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
                       DBuf<u64> &tp, bool dside = false, u64 hi_at = 0);
 // count, scan and scatter: n records exactly, in partition order
-void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
                         bool dside = false, u64 hi_at = 0);
 // D side bucketed from its partitioned records: windows of 2^wb buckets within each partition
@@ -338,7 +352,7 @@ void part_d_sort(cf_ctx *ctx, const KeyHash &kh, int wb, u32 R, const u32 *wcoun
 // the D leaves of a radius group, for recomputing a pair's D id in pass 2
 struct DLeaves {
     const SideDesc *descs;
-    const Node *nodes;
+    const LeafNode *nodes;
     size_t nn;
     const u64 *off;
     u64 lo, n;        // the table's D leaves [lo, lo + n); its records refer to them relative to lo
@@ -356,7 +370,7 @@ void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const 
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
                       const u32 *F, int FB, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand,
                       const Tuning &tn);
-void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
@@ -375,7 +389,7 @@ void wide_check_pairs(cf_ctx *ctx, const u64 *pd, const u64 *pi, const u64 *pj, 
                       const SideDesc *id, const WideMod *wm, u32 *keep, u64 *pos, u64 *od, u64 *oi, u64 *oj,
                       unsigned long long *count);
 // per-instance leaf totals of a frontier added into out[0, ninst) (nodes of an instance are contiguous)
-void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
+void inst_bounds(cf_ctx *ctx, const LeafNode *nodes, const u64 *off, size_t nn, int ninst, u64 *out);
 
 // joins
 // equal-key join of key-sorted sides: pairs per span of I rows (scnt[join_spans(nI)]), their scan

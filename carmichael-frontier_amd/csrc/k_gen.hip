@@ -144,12 +144,23 @@ __global__ void __launch_bounds__(BLK) k_mark_owners(const u64 *__restrict__ off
 // When the tile's parents belong to one instance of at most NSTAGE primes (nearly every tile), its
 // primes, their Montgomery forms and the one bound-table row the children's counts read are staged
 // in LDS, so the counts' binary searches read LDS rather than global memory.
-template <int EPT>
+// OutT: Node, or LeafNode for the last interior level (the children's products are not kept).
+__device__ __forceinline__ void put_node(Node *out, const Node &c) { *out = c; }
+__device__ __forceinline__ void put_node(LeafNode *out, const Node &c) {
+    LeafNode l;
+    l.xm = c.xm;
+    l.x2 = c.x2;
+    l.id = c.id;
+    l.pos = c.pos;
+    l.meta = c.meta;
+    *out = l;
+}
+template <int EPT, class OutT>
 __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__restrict__ descs,
                                                            const Node *__restrict__ nodes, size_t nn,
                                                            const u64 *__restrict__ off,
                                                            const u64 *__restrict__ tp, size_t nout, int depth,
-                                                           Node *__restrict__ out, u32 *__restrict__ cnt) {
+                                                           OutT *__restrict__ out, u32 *__restrict__ cnt) {
     __shared__ U128 s_b[NSTAGE];
     __shared__ u64 s_p[NSTAGE], s_pr[NSTAGE];
     constexpr int EC = BLK * EPT;
@@ -231,10 +242,16 @@ __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__re
             cc = (u32)child_count(s, GTab{s}, c, depth + 1);
         }
         if (i < cnt_t) {
-            out[j0 + i] = c;
+            put_node(out + j0 + i, c);
             cnt[j0 + i] = cc;
         }
     }
+}
+
+// Nodes as LeafNodes (the roots of a radius-1 side are its last interior level).
+__global__ void __launch_bounds__(BLK) k_leaf_nodes(const Node *__restrict__ nodes, size_t nn,
+                                                    LeafNode *__restrict__ out) {
+    for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) put_node(out + j, nodes[j]);
 }
 
 // Per first index: the leaf offsets where its nodes start and end. The nodes of one first index
@@ -242,7 +259,7 @@ __global__ void __launch_bounds__(BLK) k_expand_write_count(const SideDesc *__re
 // and the node closing it writes hi[first] = off[j + 1]; its leaves are hi - lo (0 - 0 for a first
 // index with no node). One streaming pass, no atomics (a per-node atomic on the run's counter
 // serializes every node of a run).
-__global__ void k_first_bounds(const Node *__restrict__ nodes, const u64 *__restrict__ off, size_t nn,
+__global__ void k_first_bounds(const LeafNode *__restrict__ nodes, const u64 *__restrict__ off, size_t nn,
                                u64 *__restrict__ lo, u64 *__restrict__ hi) {
     for (size_t j = (size_t)blockIdx.x * BLK + threadIdx.x; j < nn; j += (size_t)gridDim.x * BLK) {
         const u32 f = nodes[j].pos >> 16;
@@ -366,8 +383,9 @@ void launch_expand_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes,
 }
 
 
-void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
-                               size_t nout, int depth, Node *out, u32 *cnt) {
+template <class OutT>
+static void expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, OutT *out, u32 *cnt) {
     if (!nout) return;
     const bool wide = nout >= EPT4_MIN;
     const u64 ec = wide ? 4 * BLK : BLK;
@@ -375,14 +393,26 @@ void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *
     DBuf<u64> tp(ctx, ntiles);
     CF_LAUNCH(ctx, "k_mark_owners", k_mark_owners, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, off, nn, ec, tp.p);
     if (wide)
-        CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count<4>, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc,
-                  nodes, nn, off, tp.p, nout, depth, out, cnt);
+        CF_LAUNCH(ctx, "k_expand_write_count", (k_expand_write_count<4, OutT>), dim3((unsigned)ntiles), dim3(BLK), 0,
+                  d_desc, nodes, nn, off, tp.p, nout, depth, out, cnt);
     else
-        CF_LAUNCH(ctx, "k_expand_write_count", k_expand_write_count<1>, dim3((unsigned)ntiles), dim3(BLK), 0, d_desc,
-                  nodes, nn, off, tp.p, nout, depth, out, cnt);
+        CF_LAUNCH(ctx, "k_expand_write_count", (k_expand_write_count<1, OutT>), dim3((unsigned)ntiles), dim3(BLK), 0,
+                  d_desc, nodes, nn, off, tp.p, nout, depth, out, cnt);
+}
+void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, Node *out, u32 *cnt) {
+    expand_write_count(ctx, d_desc, nodes, nn, off, nout, depth, out, cnt);
+}
+void launch_expand_write_count(cf_ctx *ctx, const SideDesc *d_desc, const Node *nodes, size_t nn, const u64 *off,
+                               size_t nout, int depth, LeafNode *out, u32 *cnt) {
+    expand_write_count(ctx, d_desc, nodes, nn, off, nout, depth, out, cnt);
+}
+void launch_leaf_nodes(cf_ctx *ctx, const Node *nodes, size_t nn, LeafNode *out) {
+    if (!nn) return;
+    CF_LAUNCH(ctx, "k_leaf_nodes", k_leaf_nodes, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, nodes, nn, out);
 }
 
-void launch_first_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi) {
+void launch_first_bounds(cf_ctx *ctx, const LeafNode *nodes, const u64 *off, size_t nn, u64 *lo, u64 *hi) {
     if (!nn) return;
     CF_LAUNCH(ctx, "k_first_bounds", k_first_bounds, dim3(grid_for(nn, BLK, 65536)), dim3(BLK), 0, nodes, off, nn, lo,
               hi);

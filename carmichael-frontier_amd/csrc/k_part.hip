@@ -65,7 +65,7 @@ __device__ __forceinline__ void lds_sync() {
 
 struct LeafSrc {
     const SideDesc *descs;
-    const Node *nodes;
+    const LeafNode *nodes;
     size_t nn;
     const u64 *off;   // exclusive scan of leaves per node, nn + 1 entries
     u64 lo, n;        // leaves [lo, lo + n)
@@ -74,7 +74,7 @@ struct LeafSrc {
 };
 
 // residue state of the leaf choosing index x below node nd
-__device__ __forceinline__ void leaf_residue(const SideDesc &s, const Node &nd, int x, u64 &xm, u64 &x2) {
+__device__ __forceinline__ void leaf_residue(const SideDesc &s, const LeafNode &nd, int x, u64 &xm, u64 &x2) {
     if (s.leaf32) {
         const u32 p = (u32)s.p[x];
         xm = cf_redc32_mul(nd.xm, p, s.mc.m, (u32)s.mc.mneg);
@@ -146,7 +146,7 @@ __device__ __forceinline__ Tile tile_setup(const LeafSrc &L, u64 t, TileLds &S) 
         for (u32 i = threadIdx.x; i < (u32)LT; i += LPB) S.own[i] = 0;
     if (T.fast) {
         for (u32 q = threadIdx.x; q < T.np; q += LPB) {
-            const Node &nd = L.nodes[T.p0 + q];
+            const LeafNode &nd = L.nodes[T.p0 + q];
             const u64 o = L.off[T.p0 + q];
             S.xm[q] = nd.xm;
             S.x2[q] = nd.x2;
@@ -216,7 +216,7 @@ __device__ __forceinline__ void tile_fetch(const LeafSrc &L, u64 p0, u64 p1, Til
     f.p1 = p1;
     const u64 q = p0 + threadIdx.x;
     if (q <= p1) {
-        const Node *nd = L.nodes + q;
+        const LeafNode *nd = L.nodes + q;
         f.xm = nd->xm;
         f.x2 = nd->x2;
         f.pos = nd->pos;
@@ -257,7 +257,7 @@ __device__ __forceinline__ Tile tile_setup_pf(const LeafSrc &L, u64 t, TileLds &
         }
         // parents beyond the block's threads (a tile of many short subtrees): read here
         for (u32 qq = LPB + threadIdx.x; qq < T.np; qq += LPB) {
-            const Node &nd = L.nodes[T.p0 + qq];
+            const LeafNode &nd = L.nodes[T.p0 + qq];
             const u64 o = L.off[T.p0 + qq];
             if (qq < (u32)NCAP) {
                 S.xm[qq] = nd.xm;
@@ -358,7 +358,7 @@ __device__ __forceinline__ u64 leaf_fast(const TileLds &S, const FastCtx &c, u32
 
 // Generic tile: the same from global memory.
 struct LeafRef {
-    Node nd;
+    LeafNode nd;
     const SideDesc *s;
     int x;
 };
@@ -835,7 +835,7 @@ __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key,
     // the tile owner map (when given) brackets the parent to the tile's few nodes
     const size_t p = L.tp ? find_parent(L.off, L.tp[2 * (leaf / LT)], L.tp[2 * (leaf / LT) + 1], j)
                           : find_parent(L.off, 0, L.nn - 1, j);
-    const Node nd = L.nodes[p];
+    const LeafNode nd = L.nodes[p];
     const SideDesc &s = L.descs[nd.meta >> 8];
     const int x = child_index(s, nd, (int)(j - L.off[p]));
     u64 xm, x2;
@@ -1388,7 +1388,7 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
 // Per-instance record totals from node boundaries: nodes of one instance are contiguous and in
 // instance order, so instance i's nodes are [lo, hi) with lo and hi found by binary search on the
 // instance field, and its leaves are off[hi] - off[lo]. One thread per instance.
-__global__ void __launch_bounds__(PB) k_inst_bounds(const Node *__restrict__ nodes, const u64 *__restrict__ off,
+__global__ void __launch_bounds__(PB) k_inst_bounds(const LeafNode *__restrict__ nodes, const u64 *__restrict__ off,
                                                     size_t nn, int ninst, u64 *__restrict__ out) {
     const int i = blockIdx.x * PB + threadIdx.x;
     if (i >= ninst) return;
@@ -1431,7 +1431,7 @@ void fill_u32(cf_ctx *ctx, u32 *a, size_t n, u32 v) {
 }
 
 // Leaf source of [lo, lo + n) with its tile owners computed into tp (2 * ceil(n / LT) entries).
-static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                         int depth, DBuf<u64> &tp) {
     const u64 ntiles = (n + LT - 1) / LT;
     tp = DBuf<u64>(ctx, 2 * ntiles);
@@ -1440,7 +1440,7 @@ static LeafSrc leaf_src(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, s
     return LeafSrc{descs, nodes, nn, off, lo, n, depth, tp.p};
 }
 
-void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void leaf_records(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                   int depth, u64 *keys, u64 *ids) {
     if (!n) return;
     DBuf<u64> tp;
@@ -1451,7 +1451,7 @@ void leaf_records(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t 
               keys, ids);
 }
 
-void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                        int depth, const KeyHash &kh, u32 *bcount) {
     if (!n) return;
     DBuf<u64> tp;
@@ -1459,7 +1459,7 @@ void part_bucket_count(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, si
     CF_LAUNCH(ctx, "k_bucket_count", k_bucket_count, dim3(grid_for(n, LT)), dim3(LPB), 0, L, kh, bcount);
 }
 
-void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+void part_bucket_scatter(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                          u64 n, int depth, const KeyHash &kh, const u64 *bstart, u32 *bcur, void *DR, unsigned short *T) {
     if (!n) return;
     DBuf<u64> tp;
@@ -1549,7 +1549,7 @@ static bool cap_fits(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_sl
 
 u64 part_cap_slots(u64 n, const KeyHash &kh, const Tuning &tn) { return cap_fits(n, kh, tn) ? cap_slots(n, kh, tn) : n; }
 
-void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
+void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo, u64 n,
                       int depth, const KeyHash &kh, const Tuning &tn, IChunkLayout &lay, u64 *rec, u32 *ovf,
                       DBuf<u64> &tp, bool dside, u64 hi_at) {
     CF_REQUIRE(!(dside && hi_at), "D records are {word, leaf} pairs");
@@ -1575,7 +1575,7 @@ void part_scatter_cap(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
     launch_part_scatter(ctx, L, kh, tpb, nblk, nullptr, CAPL, lay.fill.p, ovf, rec, lay.hi, dside);
 }
 
-void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+void part_scatter_exact(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                         u64 n, int depth, const KeyHash &kh, IChunkLayout &lay, u64 *rec, const u64 *tp,
                         bool dside, u64 hi_at) {
     CF_REQUIRE(!(dside && hi_at), "D records are {word, leaf} pairs");
@@ -1637,7 +1637,7 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
               ncand, ccap, cand);
 }
 
-void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, size_t nn, const u64 *off, u64 lo,
+void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
                       const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
                       const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
@@ -1649,7 +1649,7 @@ void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const Node *nodes, siz
               cand, koffs, ninst, bstart, (const u64 *)DR, npairs, cap, od, oi, oinst, kerr);
 }
 
-void inst_bounds(cf_ctx *ctx, const Node *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {
+void inst_bounds(cf_ctx *ctx, const LeafNode *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {
     if (!nn || ninst <= 0) return;
     CF_LAUNCH(ctx, "k_inst_bounds", k_inst_bounds, dim3((unsigned)((ninst + PB - 1) / PB)), dim3(PB), 0, nodes, off, nn,
               ninst, out);

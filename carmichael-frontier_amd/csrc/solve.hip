@@ -23,7 +23,7 @@ typedef unsigned __int128 u128;
 namespace {
 
 struct Frontier {
-    DBuf<Node> nodes;
+    DBuf<LeafNode> nodes;   // the last interior level
     size_t nn = 0;
     DBuf<u32> cnt;
     DBuf<u64> off;
@@ -65,31 +65,38 @@ Frontier expand_to_last(cf_ctx *ctx, const SideDesc *d_desc, const Node *roots, 
     DBuf<Node> cur;
     const Node *curp = roots;
     size_t ncur = nroots;
-    // the roots' child counts; every level below gets its counts from the pass that writes it
+    // the roots' child counts; every level below gets its counts from the pass that writes it; the
+    // last interior level is written as LeafNodes
     DBuf<u32> c(ctx, ncur);
     launch_expand_count(ctx, d_desc, curp, ncur, 0, c.p);
+    DBuf<LeafNode> last;
     for (int depth = 0; depth + 1 < r; ++depth) {
         DBuf<u64> o(ctx, ncur + 1);
         scan_u32_to_u64_async(ctx, c.p, o.p, ncur);
         const u64 tot = host_value(ctx, tape, o.p + ncur);
         if (tot == 0) return f;
-        DBuf<Node> nx(ctx, tot);
         DBuf<u32> cn(ctx, tot);
-        launch_expand_write_count(ctx, d_desc, curp, ncur, o.p, tot, depth, nx.p, cn.p);
-        cur = std::move(nx);
+        if (depth + 2 < r) {
+            DBuf<Node> nx(ctx, tot);
+            launch_expand_write_count(ctx, d_desc, curp, ncur, o.p, tot, depth, nx.p, cn.p);
+            cur = std::move(nx);
+            curp = cur.p;
+        } else {
+            last = DBuf<LeafNode>(ctx, tot);
+            launch_expand_write_count(ctx, d_desc, curp, ncur, o.p, tot, depth, last.p, cn.p);
+        }
         c = std::move(cn);
-        curp = cur.p;
         ncur = tot;
     }
-    if (!cur.p) {   // r == 1: the roots are the last level
-        cur = DBuf<Node>(ctx, ncur);
-        CF_HIP(hipMemcpyAsync(cur.p, roots, ncur * sizeof(Node), hipMemcpyDeviceToDevice, ctx->stream));
+    if (!last.p) {   // r == 1: the roots are the last level
+        last = DBuf<LeafNode>(ctx, ncur);
+        launch_leaf_nodes(ctx, roots, ncur, last.p);
     }
     f.cnt = std::move(c);
     f.off = DBuf<u64>(ctx, ncur + 1);
     scan_u32_to_u64_async(ctx, f.cnt.p, f.off.p, ncur);
     f.total = host_value(ctx, tape, f.off.p + ncur);
-    f.nodes = std::move(cur);
+    f.nodes = std::move(last);
     f.nn = ncur;
     return f;
 }

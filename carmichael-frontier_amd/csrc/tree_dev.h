@@ -2,7 +2,8 @@
 #pragma once
 #include "cf_internal.h"
 
-__device__ __forceinline__ int node_nxt(const Node &nd) { return (int)(short)(nd.pos & 0xFFFFu); }
+template <class N>
+__device__ __forceinline__ int node_nxt(const N &nd) { return (int)(short)(nd.pos & 0xFFFFu); }
 
 // Record id after choosing index x at position `depth` (0-based) of an r-subset.
 __device__ __forceinline__ u64 id_update(const SideDesc &s, u64 id, int x, int depth) {
@@ -25,7 +26,8 @@ __device__ __forceinline__ size_t find_parent(const u64 *__restrict__ off, size_
 }
 
 // Index chosen by child number k of a node.
-__device__ __forceinline__ int child_index(const SideDesc &s, const Node &nd, int k) {
+template <class N>
+__device__ __forceinline__ int child_index(const SideDesc &s, const N &nd, int k) {
     return (s.side == SIDE_I) ? node_nxt(nd) + k : node_nxt(nd) - k;
 }
 

@@ -355,9 +355,18 @@ struct DLeaves {
     const LeafNode *nodes;
     size_t nn;
     const u64 *off;
-    u64 lo, n;        // the table's D leaves [lo, lo + n); its records refer to them relative to lo
-    const u64 *tp;    // tile owner map of [lo, lo + n), or null
 };
+// One D table as pass 2 reads it: bucket starts, records, and the D leaves [lo, lo + n) its records
+// refer to (relative to lo), with their tile owner map (or null).
+struct DTabDev {
+    const u64 *bstart;
+    const u64 *DR;
+    u64 lo, n;
+    const u64 *tp;
+};
+// n table descriptors into device memory (by kernels whose arguments carry them, so a captured solve
+// replays them)
+void set_dtabs(cf_ctx *ctx, const DTabDev *h, size_t n, DTabDev *d);
 // xq: 8 u32 queue counters; ncand: 1 counter; cand: ccap slots
 // rec: 8-byte I records (low word bucket offset << fpb | fingerprint, high word leaf index)
 // bucket filter of a D table: bits per bucket for the partition geometry (0: none fits in LDS), its
@@ -367,14 +376,16 @@ int filter_bits(const KeyHash &kh);
 u64 cand_margin(const Tuning &tn);
 u64 filter_words(const KeyHash &kh, int FB);
 void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const KeyHash &kh, int FB, u32 *F);
+// candidates are the records' slots tagged with the table (slot | tag << 32); reset: *ncand cleared
+// first (the first table of a chunk), else the candidates follow those already counted
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
                       const u32 *F, int FB, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand,
-                      const Tuning &tn);
+                      const Tuning &tn, u32 tag, bool reset);
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
-                      const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
-                      const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
-                      u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr);
+                      const DLeaves &dl, const DTabDev *tabs, const unsigned long long *ncand, u64 ccap,
+                      const u64 *cand, const u64 *koffs, int ninst, unsigned long long *npairs, u64 cap, u64 *od,
+                      u64 *oi, u64 *oinst, u32 *kerr);
 // moduli of 2^64 and above: the instance is joined modulo wide_divisor(M) (the largest divisor of M
 // in [2^40, 2^62]) and its pairs are checked exactly modulo M afterwards (k_wide.hip)
 struct WideMod {

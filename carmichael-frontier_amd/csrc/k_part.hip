@@ -929,7 +929,7 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                                                     const unsigned short *__restrict__ T,
                                                     const u32 *__restrict__ F, int FB, u32 *__restrict__ xq,
                                                     unsigned long long *__restrict__ ncand, u64 ccap,
-                                                    u64 *__restrict__ cand) {
+                                                    u64 *__restrict__ cand, u32 tag) {
     __shared__ u32 sF[FWMAX];
     __shared__ u64 s_nx[2];
     const int lane = threadIdx.x & 63;
@@ -1080,7 +1080,8 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     while (hmask) {
                         const int e = __ffs(hmask) - 1;
                         hmask &= hmask - 1;
-                        if (at < ccap) cand[at] = starts[e / PROBE_BATCH] + v0 + (u64)(e % PROBE_BATCH) * 64 + lane;
+                        if (at < ccap)
+                            cand[at] = (starts[e / PROBE_BATCH] + v0 + (u64)(e % PROBE_BATCH) * 64 + lane) | (u64)tag << 32;
                         ++at;
                     }
                 }
@@ -1324,16 +1325,16 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     }
 }
 
-// Probe, pass 2: the D records of each candidate's bucket that agree with the candidate on the
+// Probe, pass 2, over the candidates of every table a chunk was probed against (a candidate's tag
+// names its table): the D records of each candidate's bucket that agree with the candidate on the
 // fingerprint and hash_ex have their exact keys recomputed from their leaves; the candidate's own
 // key is recomputed once, at the first record with its fingerprint. Every equal key emits a pair.
-__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, KeyHash kh,
+__global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD0, KeyHash kh,
                                                          const u32 *__restrict__ recs,
                                                          SegLayout S,
                                                          const unsigned long long *__restrict__ ncand, u64 ccap,
                                                          const u64 *__restrict__ cand, const u64 *__restrict__ koffs,
-                                                         int ninst, const u64 *__restrict__ bstart,
-                                                         const u64 *__restrict__ DR,
+                                                         int ninst, const DTabDev *__restrict__ tabs,
                                                          unsigned long long *__restrict__ npairs, u64 cap,
                                                          u64 *__restrict__ out_d, u64 *__restrict__ out_i,
                                                          u64 *__restrict__ out_inst, u32 *__restrict__ kerr) {
@@ -1343,7 +1344,14 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD, 
     for (u64 c = (u64)blockIdx.x * PB + threadIdx.x; c < nc; c += (u64)gridDim.x * PB) {
         const u64 raw = cand[c];
         if (raw == CAND_NONE) continue;
-        const u64 e = raw;
+        const u64 e = raw & 0xFFFFFFFFull;
+        const DTabDev tb = tabs[raw >> 32];
+        const u64 *__restrict__ bstart = tb.bstart;
+        const u64 *__restrict__ DR = tb.DR;
+        LeafSrc LD = LD0;
+        LD.lo = tb.lo;
+        LD.n = tb.n;
+        LD.tp = tb.tp;
         u64 g;
         if (S.O) {   // the last segment starting at or before e (it is not empty)
             u64 a = 0, bb = S.nseg - 1;
@@ -1622,9 +1630,9 @@ void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const 
 
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,
                       const u32 *F, int FB, u32 *xq, unsigned long long *ncand, u64 ccap, u64 *cand,
-                      const Tuning &tn) {
+                      const Tuning &tn, u32 tag, bool reset) {
     fill_u32(ctx, xq, NXCD, 0);
-    fill_u64(ctx, (u64 *)ncand, 1, 0);
+    if (reset) fill_u64(ctx, (u64 *)ncand, 1, 0);
     if (!S.nseg) return;
     // tiles are dealt per XCD, so every XCD needs at least one block: the grid is a non-zero
     // multiple of the XCD count; by default one 1024-thread block per CU (the filter slice takes
@@ -1634,19 +1642,35 @@ void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const 
     unsigned g = (unsigned)(ntiles < gmax ? ntiles : gmax);
     g = g < NXCD ? NXCD : g - g % NXCD;
     CF_LAUNCH(ctx, "k_part_probe", k_part_probe, dim3(g), dim3(PBP), 0, kh, S, (const u32 *)rec, T, F, FB, xq,
-              ncand, ccap, cand);
+              ncand, ccap, cand, tag);
 }
 
 void part_probe_pass2(cf_ctx *ctx, const SideDesc *descs, const LeafNode *nodes, size_t nn, const u64 *off, u64 lo,
                       int depth, const KeyHash &kh, const SegLayout &S, const u64 *rec, const u64 *tp,
-                      const DLeaves &dl, const unsigned long long *ncand, u64 ccap, const u64 *cand,
-                      const u64 *koffs, int ninst, const u64 *bstart, const void *DR, unsigned long long *npairs,
-                      u64 cap, u64 *od, u64 *oi, u64 *oinst, u32 *kerr) {
+                      const DLeaves &dl, const DTabDev *tabs, const unsigned long long *ncand, u64 ccap,
+                      const u64 *cand, const u64 *koffs, int ninst, unsigned long long *npairs, u64 cap, u64 *od,
+                      u64 *oi, u64 *oinst, u32 *kerr) {
     LeafSrc L{descs, nodes, nn, off, lo, 0, depth, tp};
-    LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, dl.lo, dl.n, depth, dl.tp};
+    LeafSrc LD{dl.descs, dl.nodes, dl.nn, dl.off, 0, 0, depth, nullptr};
     CF_LAUNCH(ctx, "k_probe_candidates", k_probe_candidates, dim3(1024), dim3(PB), 0, L, LD, kh, (const u32 *)rec, S,
-              ncand, ccap,
-              cand, koffs, ninst, bstart, (const u64 *)DR, npairs, cap, od, oi, oinst, kerr);
+              ncand, ccap, cand, koffs, ninst, tabs, npairs, cap, od, oi, oinst, kerr);
+}
+
+// Table descriptors into device memory, up to 32 per launch (as kernel arguments).
+struct DTabBatch {
+    DTabDev t[32];
+    u32 n;
+};
+__global__ void __launch_bounds__(64) k_set_dtabs(DTabBatch b, DTabDev *__restrict__ out) {
+    if (threadIdx.x < b.n) out[threadIdx.x] = b.t[threadIdx.x];
+}
+void set_dtabs(cf_ctx *ctx, const DTabDev *h, size_t n, DTabDev *d) {
+    for (size_t i = 0; i < n; i += 32) {
+        DTabBatch b{};
+        b.n = (u32)(n - i < 32 ? n - i : 32);
+        for (u32 k = 0; k < b.n; ++k) b.t[k] = h[i + k];
+        CF_LAUNCH(ctx, "k_set_dtabs", k_set_dtabs, dim3(1), dim3(64), 0, b, d + i);
+    }
 }
 
 void inst_bounds(cf_ctx *ctx, const LeafNode *nodes, const u64 *off, size_t nn, int ninst, u64 *out) {

@@ -538,6 +538,14 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                 build_dtable(ctx, d_desc.p, fd, g.first - 1, pt, FB, d_part, lo, std::min<u64>(m_rng, nD - lo), b->tune,
                              tabs[k], ovf.p, kerr.p, tape);
             }
+            // the tables as pass 2 reads them
+            DBuf<DTabDev> dtabs(ctx, std::max<size_t>(tabs.size(), 1));
+            {
+                std::vector<DTabDev> h(tabs.size());
+                for (size_t k = 0; k < tabs.size(); ++k)
+                    h[k] = DTabDev{tabs[k].bstart.p, tabs[k].DR.p, tabs[k].lo, tabs[k].n, tabs[k].tp.p};
+                set_dtabs(ctx, h.data(), h.size(), dtabs.p);
+            }
             if (!capturing) gc.mark();
             // ---- I side: expand, then per chunk of leaves: partition (one scatter for every table),
             // probe each table, check the candidates exactly
@@ -559,8 +567,9 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                         rec = DBuf<u64>(ctx, cap_m);
                     }
                     // candidates: fingerprint hits and records of overflowing buckets (~1e-4 of the records)
-                    if (ccap < std::max<u64>(m >> 5, 1 << 16) + cand_margin(b->tune)) {
-                        ccap = std::max<u64>(m >> 5, 1 << 16) + cand_margin(b->tune);
+                    const u64 cwant = std::max<u64>(m >> 5, 1 << 16) + tabs.size() * cand_margin(b->tune);
+                    if (ccap < cwant) {
+                        ccap = cwant;
                         cand = DBuf<u64>(ctx, ccap);
                     }
                     // one pass into segments sized for the expected share; the probe that follows
@@ -572,43 +581,37 @@ void solve_part(cf_batch *b, BatchOut &out, cf_stats *st, u64 max_chunk, bool d_
                     const u64 split_at = tabs.size() >= 2 ? cap_m : 0;
                     part_scatter_cap(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, b->tune, lay, rec.p,
                                      ovf.p, tp, false, split_at);
-                    for (size_t k = 0; k < tabs.size(); ++k) {
-                        const DTable &tb = tabs[k];
-                        part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
-                                         (unsigned long long *)flags.p, ccap, cand.p, b->tune);
-                        u64 nc;
-                        if (k == 0) {
-                            const u64 ov = host_value_u32(ctx, tape, ovf.p);
-                            nc = host_value(ctx, tape, flags.p);
-                            if (ov) {
-                                part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay,
-                                                   rec.p, tp.p, false, split_at);
-                                part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
-                                                 (unsigned long long *)flags.p, ccap, cand.p, b->tune);
-                                nc = host_value(ctx, tape, flags.p);
-                            }
-                        } else {
-                            nc = host_value(ctx, tape, flags.p);
-                        }
-                        while (nc > ccap) {
-                            // more candidates than room (heavily repeated keys): probe again with room for
-                            // all. The count includes the unused tails of partial reservations, and which
-                            // wave takes which tile varies from run to run. A wave leaves unused fewer slots
-                            // than the candidates of the round that makes it reserve again, plus at most one
-                            // partial reservation at its end, so a run needs at most 2 * (candidates) +
-                            // cand_margin slots, and nc >= candidates: one rerun suffices; the loop guards it
-                            ccap = 2 * nc + cand_margin(b->tune);
-                            cand = DBuf<u64>(ctx, ccap);
-                            part_probe_pass1(ctx, kh, lay.view(), rec.p, tb.T.p, tb.F.p, FB, xq.p,
-                                             (unsigned long long *)flags.p, ccap, cand.p, b->tune);
-                            nc = host_value(ctx, tape, flags.p);
-                        }
-                        const DLeaves dleaves{d_desc.p, fd.nodes.p, fd.nn, fd.off.p, tb.lo, tb.n, tb.tp.p};
-                        part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
-                                         tp.p, dleaves, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p, n,
-                                         tb.bstart.p, tb.DR.p, (unsigned long long *)cnt_pairs.p, cap, out.pd.p,
-                                         out.pi.p, out.pinst.p, kerr.p);
+                    // every table's candidates into one list (tagged with the table), then one pass 2
+                    auto probe = [&](size_t k) {
+                        part_probe_pass1(ctx, kh, lay.view(), rec.p, tabs[k].T.p, tabs[k].F.p, FB, xq.p,
+                                         (unsigned long long *)flags.p, ccap, cand.p, b->tune, (u32)k, k == 0);
+                    };
+                    probe(0);
+                    if (host_value_u32(ctx, tape, ovf.p)) {
+                        // a segment overflowed (heavily repeated keys): the chunk laid out exactly
+                        part_scatter_exact(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, m, g.first - 1, kh, lay, rec.p,
+                                           tp.p, false, split_at);
+                        probe(0);
                     }
+                    for (size_t k = 1; k < tabs.size(); ++k) probe(k);
+                    u64 nc = host_value(ctx, tape, flags.p);
+                    while (nc > ccap) {
+                        // more candidates than room (heavily repeated keys): probe again with room for
+                        // all. The count includes the unused tails of partial reservations, and which
+                        // wave takes which tile varies from run to run. A wave leaves unused fewer slots
+                        // than the candidates of the round that makes it reserve again, plus at most one
+                        // partial reservation at the end of each probe, so a run over the tables needs at
+                        // most 2 * (candidates) + one cand_margin per table, and nc >= candidates: one
+                        // rerun suffices; the loop guards it
+                        ccap = 2 * nc + tabs.size() * cand_margin(b->tune);
+                        cand = DBuf<u64>(ctx, ccap);
+                        for (size_t k = 0; k < tabs.size(); ++k) probe(k);
+                        nc = host_value(ctx, tape, flags.p);
+                    }
+                    const DLeaves dleaves{d_desc.p, fd.nodes.p, fd.nn, fd.off.p};
+                    part_probe_pass2(ctx, i_desc.p, f.nodes.p, f.nn, f.off.p, L, g.first - 1, kh, lay.view(), rec.p,
+                                     tp.p, dleaves, dtabs.p, (const unsigned long long *)flags.p, ccap, cand.p, koffs.p,
+                                     n, (unsigned long long *)cnt_pairs.p, cap, out.pd.p, out.pi.p, out.pinst.p, kerr.p);
                 }
             }
             if (!capturing) {

@@ -136,11 +136,11 @@ class Batch:
                 ctx.free(dev.value)
         order = np.argsort(pj, kind="stable")
         pd, pi, pj = pd[order], pi[order], pj[order]
-        bounds = np.searchsorted(pj, np.arange(n + 1, dtype=np.uint64))
-        out = []
-        for j in range(n):
-            a, b = int(bounds[j]), int(bounds[j + 1])
-            out.append(SolveResult(pd[a:b].copy(), pi[a:b].copy(), int(counts[2 * j]), int(counts[2 * j + 1])))
+        # each instance's pairs are a view of its own range of these freshly sorted arrays
+        bounds = np.searchsorted(pj, np.arange(n + 1, dtype=np.uint64)).tolist()
+        cl = counts.tolist()
+        out = [SolveResult(pd[bounds[j]:bounds[j + 1]], pi[bounds[j]:bounds[j + 1]], cl[2 * j], cl[2 * j + 1])
+               for j in range(n)]
         return out, st.as_dict()
 
     def close(self) -> None:

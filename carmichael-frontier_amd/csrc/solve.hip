@@ -679,6 +679,9 @@ struct BatchGraph {
     BatchOut out;
     u64 npairs = 0;
     u64 chunk = 0;
+    // pinned host block a replay's counts are read back into with one synchronization:
+    // [pair count, error flags, nD of every instance, nI of every instance]
+    u64 *hdr = nullptr;
 };
 
 static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
@@ -694,6 +697,7 @@ static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
     G->out.cnt_pairs.release();
     G->out.kerr.release();
     arena_release(ctx, &G->arena);
+    if (G->hdr) (void)hipHostFree(G->hdr);
     delete G;
 }
 
@@ -806,6 +810,12 @@ int cf_batch_capture(cf_batch *b, uint64_t max_chunk) {
         graph_destroy(ctx, G.release());
         CF_HIP(ei);
     }
+    const hipError_t eh = hipHostMalloc((void **)&G->hdr, (2 + 2 * (size_t)b->n) * sizeof(u64), hipHostMallocDefault);
+    if (eh != hipSuccess) {
+        G->hdr = nullptr;
+        graph_destroy(ctx, G.release());
+        CF_HIP(eh);
+    }
     b->graph = G.release();
     API_END
 }
@@ -837,9 +847,15 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
         CF_HIP(hipEventRecord(e0, ctx->stream));
         CF_HIP(hipGraphLaunch(G->exec, ctx->stream));
         CF_HIP(hipEventRecord(e1, ctx->stream));
-        const u64 np = d2h_u64(ctx, G->out.cnt_pairs.p);
-        u32 ke = 0;
-        d2h(ctx, &ke, G->out.kerr.p, 1);
+        u64 *h = G->hdr;
+        h[1] = 0;
+        CF_HIP(hipMemcpyAsync(h, G->out.cnt_pairs.p, sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipMemcpyAsync(h + 1, G->out.kerr.p, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipMemcpyAsync(h + 2, G->out.dcount.p, b->n * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipMemcpyAsync(h + 2 + b->n, G->out.icount.p, b->n * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+        CF_HIP(hipStreamSynchronize(ctx->stream));
+        const u64 np = h[0];
+        const u32 ke = (u32)h[1];
         float ms = 0;
         CF_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
@@ -847,9 +863,7 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
         if (ke & KERR_DWINDOW) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
         if (ke & KERR_CAND) throw CfError{CF_EINTERNAL, "the probe's candidate list was truncated"};
         if (np != G->npairs) throw CfError{CF_EINTERNAL, "a replay of the captured solve found a different pair count"};
-        std::vector<u64> nD(b->n), nI(b->n);
-        d2h(ctx, nD.data(), G->out.dcount.p, b->n);
-        d2h(ctx, nI.data(), G->out.icount.p, b->n);
+        const u64 *nD = h + 2, *nI = h + 2 + b->n;
         if (np) {
             DBuf<u64> pd(ctx, np), pi(ctx, np), pj(ctx, np);
             CF_HIP(hipMemcpyAsync(pd.p, G->out.pd.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));

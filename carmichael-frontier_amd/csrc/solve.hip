@@ -682,6 +682,7 @@ struct BatchGraph {
     // pinned host block a replay's counts are read back into with one synchronization:
     // [pair count, error flags, nD of every instance, nI of every instance]
     u64 *hdr = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;   // around each replay's graph launch
 };
 
 static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
@@ -698,6 +699,8 @@ static void graph_destroy(cf_ctx *ctx, BatchGraph *G) {
     G->out.kerr.release();
     arena_release(ctx, &G->arena);
     if (G->hdr) (void)hipHostFree(G->hdr);
+    if (G->t0) (void)hipEventDestroy(G->t0);
+    if (G->t1) (void)hipEventDestroy(G->t1);
     delete G;
 }
 
@@ -810,9 +813,11 @@ int cf_batch_capture(cf_batch *b, uint64_t max_chunk) {
         graph_destroy(ctx, G.release());
         CF_HIP(ei);
     }
-    const hipError_t eh = hipHostMalloc((void **)&G->hdr, (2 + 2 * (size_t)b->n) * sizeof(u64), hipHostMallocDefault);
+    hipError_t eh = hipHostMalloc((void **)&G->hdr, (2 + 2 * (size_t)b->n) * sizeof(u64), hipHostMallocDefault);
+    if (eh != hipSuccess) G->hdr = nullptr;
+    if (eh == hipSuccess) eh = hipEventCreate(&G->t0);
+    if (eh == hipSuccess) eh = hipEventCreate(&G->t1);
     if (eh != hipSuccess) {
-        G->hdr = nullptr;
         graph_destroy(ctx, G.release());
         CF_HIP(eh);
     }
@@ -839,37 +844,41 @@ int cf_batch_solve(cf_batch *b, uint64_t max_chunk, uint64_t **out_d, uint64_t *
     b->ctx->peak_live = b->ctx->live_bytes;
     BatchGraph *G = b->graph;
     if (G && (!max_chunk || max_chunk == G->chunk)) {
-        // replay the captured solve: one graph launch, then the few results are read back
+        // replay the captured solve: one graph launch, then the few results are read back. The pair
+        // count of every replay equals the captured one (checked below), so the pair copies are
+        // queued with the counts and one synchronization serves all of them.
         cf_ctx *ctx = b->ctx;
-        hipEvent_t e0, e1;
-        CF_HIP(hipEventCreate(&e0));
-        CF_HIP(hipEventCreate(&e1));
-        CF_HIP(hipEventRecord(e0, ctx->stream));
+        const u64 npc = G->npairs;
+        DBuf<u64> pd, pi, pj;
+        if (npc) {
+            pd = DBuf<u64>(ctx, npc);
+            pi = DBuf<u64>(ctx, npc);
+            pj = DBuf<u64>(ctx, npc);
+        }
+        CF_HIP(hipEventRecord(G->t0, ctx->stream));
         CF_HIP(hipGraphLaunch(G->exec, ctx->stream));
-        CF_HIP(hipEventRecord(e1, ctx->stream));
+        CF_HIP(hipEventRecord(G->t1, ctx->stream));
         u64 *h = G->hdr;
         h[1] = 0;
         CF_HIP(hipMemcpyAsync(h, G->out.cnt_pairs.p, sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
         CF_HIP(hipMemcpyAsync(h + 1, G->out.kerr.p, sizeof(u32), hipMemcpyDeviceToHost, ctx->stream));
         CF_HIP(hipMemcpyAsync(h + 2, G->out.dcount.p, b->n * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
         CF_HIP(hipMemcpyAsync(h + 2 + b->n, G->out.icount.p, b->n * sizeof(u64), hipMemcpyDeviceToHost, ctx->stream));
+        if (npc) {
+            CF_HIP(hipMemcpyAsync(pd.p, G->out.pd.p, npc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(pi.p, G->out.pi.p, npc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            CF_HIP(hipMemcpyAsync(pj.p, G->out.pinst.p, npc * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        }
         CF_HIP(hipStreamSynchronize(ctx->stream));
         const u64 np = h[0];
         const u32 ke = (u32)h[1];
         float ms = 0;
-        CF_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        CF_HIP(hipEventElapsedTime(&ms, G->t0, G->t1));
         if (ke & KERR_DWINDOW) throw CfError{CF_EINTERNAL, "a D record fell outside its hash range's buckets"};
         if (ke & KERR_CAND) throw CfError{CF_EINTERNAL, "the probe's candidate list was truncated"};
-        if (np != G->npairs) throw CfError{CF_EINTERNAL, "a replay of the captured solve found a different pair count"};
+        if (np != npc) throw CfError{CF_EINTERNAL, "a replay of the captured solve found a different pair count"};
         const u64 *nD = h + 2, *nI = h + 2 + b->n;
         if (np) {
-            DBuf<u64> pd(ctx, np), pi(ctx, np), pj(ctx, np);
-            CF_HIP(hipMemcpyAsync(pd.p, G->out.pd.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            CF_HIP(hipMemcpyAsync(pi.p, G->out.pi.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            CF_HIP(hipMemcpyAsync(pj.p, G->out.pinst.p, np * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            CF_HIP(hipStreamSynchronize(ctx->stream));
             *out_d = pd.release();
             *out_i = pi.release();
             *out_inst = pj.release();

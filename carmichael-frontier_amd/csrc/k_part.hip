@@ -42,16 +42,16 @@ constexpr int LT = LPB * LPT;      // 4096 leaves per tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
 // A DR entry: the D record's leaf index (relative to its table's first leaf), above it the record's
-// fingerprint (the fpb bits its table slot holds) and 5 more bits of its hash (hash bits 12-16, used
+// fingerprint (the fpb bits its table slot holds) and 4 more bits of its hash (hash bits 12-15, used
 // by neither the bucket nor the fingerprint), so that a candidate recomputes only the keys of records
-// that agree with it on 19 hash bits.
+// that agree with it on 18 hash bits.
 constexpr int DR_FP_SHIFT = 40;
 constexpr int DR_EX_SHIFT = 54;
 constexpr u64 DR_LEAF_MASK = ((u64)1 << DR_FP_SHIFT) - 1;
-__device__ __forceinline__ u32 hash_ex(u64 h) { return (u32)(h >> 12) & 31u; }
-// A window-built D table holds fewer than 2^27 records, so its records' leaf indices leave their top
-// 5 bits to carry hash_ex from the scatter to the window sort.
-constexpr int DREC_EX_SHIFT = 27;
+__device__ __forceinline__ u32 hash_ex(u64 h) { return (u32)(h >> 12) & 15u; }
+// A window-built D table holds fewer than 2^28 records, so its records' leaf indices leave their top
+// 4 bits to carry hash_ex from the scatter to the window sort.
+constexpr int DREC_EX_SHIFT = 28;
 
 // Block barrier for LDS traffic only: orders the block's LDS accesses without waiting for its
 // outstanding global loads and stores (__syncthreads is also a workgroup memory fence, which on this
@@ -600,7 +600,7 @@ __global__ void __launch_bounds__(LPB) k_part_hist(LeafSrc L, int lgP, u64 tpb, 
 // rank, with its partition and place in the tile in the staged high word; the store side reads the
 // staged records in order (so the lanes of a wave read the words of only a few partitions) and
 // takes each record's slot from its partition's word.
-// DSIDE: the D side's records, whose leaf indices (below 2^27) carry hash_ex in their top 5 bits.
+// DSIDE: the D side's records, whose leaf indices (below 2^28) carry hash_ex in their top 4 bits.
 // LGMAXP: partitions at most 2^LGMAXP (8 or 9); the staged high word holds the partition from bit 16
 // and, on the D side, hash_ex above it.
 template <bool SPLIT = false, bool DSIDE = false, int LGMAXP = 8>

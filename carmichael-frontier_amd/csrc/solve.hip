@@ -357,7 +357,7 @@ struct DTable {
 };
 
 // The D leaves of a group in nr ranges of at most m records: one range while the window build takes
-// the whole side (fewer than 2^27 records, whose leaf indices leave 5 bits of a 32-bit word to the
+// the whole side (fewer than 2^28 records, whose leaf indices leave 4 bits of a 32-bit word to the
 // hash bits the records carry, and at most 256 windows per partition), else the fewest
 // equal ranges it takes one by one. The global-atomic build (d_part false) takes one range of any size.
 void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
@@ -369,7 +369,7 @@ void d_ranges(u64 nD, const Tuning &tn, bool d_part, u64 &nr, u64 &m) {
         const Partitioning q = assign_buckets(m, tn);
         int wb = 0;
         u32 R = 0;
-        if (m < ((u64)1 << 27) && dwin_geometry(q.kh, m, wb, R)) return;
+        if (m < ((u64)1 << 28) && dwin_geometry(q.kh, m, wb, R)) return;
         if (nr >= 65536) break;
         nr = nr < 8 ? nr + 1 : nr + nr / 8;
     }
@@ -395,7 +395,7 @@ void build_dtable(cf_ctx *ctx, const SideDesc *d_desc, const Frontier &fd, int d
     fill_u32(ctx, t.F.p, filter_words(kh, FB), 0);
     int wb = 0;
     u32 R = 0;
-    if (d_part && m < ((u64)1 << 27) && dwin_geometry(kh, m, wb, R)) {
+    if (d_part && m < ((u64)1 << 28) && dwin_geometry(kh, m, wb, R)) {
         IChunkLayout dl;
         const u64 slots = std::max<u64>(part_cap_slots(m, kh, tn), m);
         DBuf<u64> drec(ctx, slots);

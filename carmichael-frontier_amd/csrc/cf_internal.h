@@ -375,7 +375,7 @@ int filter_bits(const KeyHash &kh);
 // candidate slots the probe may leave unused (added to the candidate buffer)
 u64 cand_margin(const Tuning &tn);
 u64 filter_words(const KeyHash &kh, int FB);
-void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const KeyHash &kh, int FB, u32 *F);
+void part_filter(cf_ctx *ctx, const u64 *DR, const u64 *bstart, const KeyHash &kh, int FB, u32 *F);
 // candidates are the records' slots tagged with the table (slot | tag << 32); reset: *ncand cleared
 // first (the first table of a chunk), else the candidates follow those already counted
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,

@@ -861,15 +861,26 @@ __device__ __forceinline__ bool bucket_hit(const uint4 &a, u32 fs2) {
 
 constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
 
-// Bucket filter: FB bits per bucket (FB = 4, 8 or 16), bucket b at bits [b * FB, (b + 1) * FB) of
-// an array of 32-bit words; bit f of a bucket is set when one of its D records has a fingerprint
-// whose low log2(FB) bits are f, and all FB bits are set for an overflowing bucket (more records
-// than slots).
-// An I record whose bit is clear has no fingerprint match and no full bucket, so the probe rejects
-// it without reading the table. One thread per word: it reads the slots of its 32 / FB buckets.
-__global__ void __launch_bounds__(PB) k_bucket_filter(const unsigned short *__restrict__ T,
-                                                      const u64 *__restrict__ bstart, u64 NB, int FB,
-                                                      u32 *__restrict__ F, u64 nwords) {
+// Bucket filter: FB bits per bucket (FB = 4, 8 or 16) in an array of 32-bit words, bucket b's share
+// at bits [b * FB, (b + 1) * FB), so a word serves 32 / FB consecutive buckets.
+//   FB = 4: bit f of a bucket's field is set when one of its D records has a fingerprint whose low 2
+//           bits are f; all 4 bits are set for an overflowing bucket (more records than slots).
+//   FB >= 8: the word's buckets share its 32 bits: each of their D records sets two, at fingerprint
+//           bits 0-4 and 5-9 (a blocked Bloom filter with two probes: about 16% of the records that
+//           miss the table pass at FB = 8 and 2 records per bucket, against 22% for one bit each
+//           in per-bucket fields).
+// An I record whose bits are not all set has no fingerprint match and no full bucket, so the probe
+// rejects it without reading the table. filter_bit_mask gives the bits a record with fingerprint fp
+// in global bucket b sets in (or needs of) its word b * FB / 32.
+__device__ __forceinline__ u32 filter_bit_mask(u64 b, u32 fp, int FB) {
+    if (FB >= 8) return (1u << (fp & 31u)) | (1u << ((fp >> 5) & 31u));
+    return 1u << ((u32)(b * (u64)FB) & 31u) << (fp & (u32)(FB - 1));
+}
+
+// The filter of a table bucketed by global atomics: one thread per word, over the records of its
+// 32 / FB buckets (their fingerprints from DR).
+__global__ void __launch_bounds__(PB) k_bucket_filter(const u64 *__restrict__ DR, const u64 *__restrict__ bstart,
+                                                      u64 NB, int FB, u32 *__restrict__ F, u64 nwords) {
     const u32 per = 32u / (u32)FB;
     const u32 ones = FB == 32 ? ~0u : (1u << FB) - 1u;
     for (u64 w = (u64)blockIdx.x * PB + threadIdx.x; w < nwords; w += (u64)gridDim.x * PB) {
@@ -877,14 +888,12 @@ __global__ void __launch_bounds__(PB) k_bucket_filter(const unsigned short *__re
         for (u32 k = 0; k < per; ++k) {
             const u64 b = w * per + k;
             if (b >= NB) break;
-            const u64 tot = bstart[b + 1] - bstart[b];
-            u32 bits = 0;
-            if (tot > (u64)TSLOTS) {
-                bits = ones;
-            } else {
-                for (u32 q = 0; q < (u32)tot; ++q) bits |= 1u << (T[b * TSLOTS + q] & (FB - 1));
+            const u64 lo = bstart[b], hi = bstart[b + 1];
+            if (FB < 8 && hi - lo > (u64)TSLOTS) {
+                word |= ones << (k * FB);
+                continue;
             }
-            word |= bits << (k * FB);
+            for (u64 q = lo; q < hi; ++q) word |= filter_bit_mask(b, (u32)(DR[q] >> DR_FP_SHIFT) & 0x3FFFu, FB);
         }
         F[w] = word;
     }
@@ -1033,7 +1042,18 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     valid |= (u32)(v < fills[sj]) << e;
                 }
                 u32 look = valid;
-                if (FB > 0) {
+                if (FB >= 8) {
+                    // the bucket's word, two bits by the fingerprint (filter_bit_mask)
+                    u32 pass = 0;
+#pragma unroll
+                    for (int e = 0; e < NB2; ++e) {
+                        const u32 w = sF[(fbit0 + ((lo[e] >> kh.fpb) << lgfb)) >> 5];
+                        const u32 f = lo[e] & fmask;
+                        const u32 m = (1u << (f & 31u)) | (1u << ((f >> 5) & 31u));
+                        pass |= (u32)((w & m) == m) << e;
+                    }
+                    look &= pass;
+                } else if (FB > 0) {
                     u32 pass = 0;
 #pragma unroll
                     for (int e = 0; e < NB2; ++e) {
@@ -1310,13 +1330,11 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
                 if (gbk < gb || gbk >= gb + nb) continue;
                 const u32 i = (u32)(gbk - gb);
                 const u32 c = pos[i + 1] - pos[i];
-                u32 bits = 0;
-                if (c > (u32)TSLOTS) {
-                    bits = fones;
-                } else {
-                    for (u32 e = 0; e < c; ++e) bits |= 1u << (s_r[s_idx[pos[i] + e]].x & (u32)(FB - 1));
+                if (FB < 8 && c > (u32)TSLOTS) {
+                    word |= fones << (q * FB);
+                    continue;
                 }
-                word |= bits << (q * FB);
+                for (u32 e = 0; e < c; ++e) word |= filter_bit_mask(gbk, s_r[s_idx[pos[i] + e]].x & fmask, FB);
             }
             const bool edge = wd * 32 < bit0 || wd * 32 + 32 > bit1;
             if (edge) atomicOr(&F[wd], word);
@@ -1621,10 +1639,10 @@ int filter_bits(const KeyHash &kh) {
 
 u64 filter_words(const KeyHash &kh, int FB) { return FB ? (kh_nb(kh) * (u64)FB + 31) / 32 : 0; }
 
-void part_filter(cf_ctx *ctx, const unsigned short *T, const u64 *bstart, const KeyHash &kh, int FB, u32 *F) {
+void part_filter(cf_ctx *ctx, const u64 *DR, const u64 *bstart, const KeyHash &kh, int FB, u32 *F) {
     const u64 nw = filter_words(kh, FB);
     if (!nw) return;
-    CF_LAUNCH(ctx, "k_bucket_filter", k_bucket_filter, dim3(grid_for(nw, PB, 65536)), dim3(PB), 0, T, bstart,
+    CF_LAUNCH(ctx, "k_bucket_filter", k_bucket_filter, dim3(grid_for(nw, PB, 65536)), dim3(PB), 0, DR, bstart,
               kh_nb(kh), FB, F, nw);
 }
 

@@ -420,7 +420,7 @@ void build_dtable(cf_ctx *ctx, const SideDesc *d_desc, const Frontier &fd, int d
     scan_u32_to_u64_async(ctx, cnt.p, t.bstart.p, pt.NB);
     fill_u32(ctx, cnt.p, pt.NB, 0);   // reused as the scatter cursors
     part_bucket_scatter(ctx, d_desc, fd.nodes.p, fd.nn, fd.off.p, lo, m, depth, kh, t.bstart.p, cnt.p, t.DR.p, t.T.p);
-    part_filter(ctx, t.T.p, t.bstart.p, kh, FB, t.F.p);
+    part_filter(ctx, t.DR.p, t.bstart.p, kh, FB, t.F.p);
 }
 
 // One radius group's branch of a captured solve: while it lives, launches go to the group's side

@@ -861,40 +861,27 @@ __device__ __forceinline__ bool bucket_hit(const uint4 &a, u32 fs2) {
 
 constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
 
-// Bucket filter: FB bits per bucket (FB = 4, 8 or 16) in an array of 32-bit words, bucket b's share
-// at bits [b * FB, (b + 1) * FB), so a word serves 32 / FB consecutive buckets.
-//   FB = 4: bit f of a bucket's field is set when one of its D records has a fingerprint whose low 2
-//           bits are f; all 4 bits are set for an overflowing bucket (more records than slots).
-//   FB >= 8: the word's buckets share its 32 bits: each of their D records sets two, at fingerprint
-//           bits 0-4 and 5-9 (a blocked Bloom filter with two probes: about 16% of the records that
-//           miss the table pass at FB = 8 and 2 records per bucket, against 22% for one bit each
-//           in per-bucket fields).
-// An I record whose bits are not all set has no fingerprint match and no full bucket, so the probe
-// rejects it without reading the table. filter_bit_mask gives the bits a record with fingerprint fp
-// in global bucket b sets in (or needs of) its word b * FB / 32.
-__device__ __forceinline__ u32 filter_bit_mask(u64 b, u32 fp, int FB) {
-    if (FB >= 8) return (1u << (fp & 31u)) | (1u << ((fp >> 5) & 31u));
-    return 1u << ((u32)(b * (u64)FB) & 31u) << (fp & (u32)(FB - 1));
-}
+// Bucket filter: a blocked Bloom filter of FB bits per bucket (4 <= FB <= 16) in 32-bit words.
+// Bucket b belongs to word b * FB / 32 (rounded down), so a word serves about 32 / FB consecutive
+// buckets, which share its 32 bits: each of their D records sets two, at its fingerprint bits 0-4
+// and 5-9. An I record whose two bits are not both set matches no D record of its bucket, so the
+// probe rejects it without reading the table. At 2 D records per bucket about 16% of the records
+// that miss pass at FB = 8 (22% with one bit each in 8-bit per-bucket fields), 14% at FB = 9 and
+// 31% at FB = 5. A bucket with more records than slots has every record's bits set, and the
+// records it passes meet the overflow flag of its slot 7.
+__device__ __forceinline__ u32 filter_bit_mask(u32 fp) { return (1u << (fp & 31u)) | (1u << ((fp >> 5) & 31u)); }
+// the buckets of word w: [bucket_lo(w), bucket_lo(w + 1))
+__device__ __forceinline__ u64 filter_bucket_lo(u64 w, int FB) { return (w * 32 + (u64)FB - 1) / (u64)FB; }
 
 // The filter of a table bucketed by global atomics: one thread per word, over the records of its
-// 32 / FB buckets (their fingerprints from DR).
+// buckets (their fingerprints from DR).
 __global__ void __launch_bounds__(PB) k_bucket_filter(const u64 *__restrict__ DR, const u64 *__restrict__ bstart,
                                                       u64 NB, int FB, u32 *__restrict__ F, u64 nwords) {
-    const u32 per = 32u / (u32)FB;
-    const u32 ones = FB == 32 ? ~0u : (1u << FB) - 1u;
     for (u64 w = (u64)blockIdx.x * PB + threadIdx.x; w < nwords; w += (u64)gridDim.x * PB) {
         u32 word = 0;
-        for (u32 k = 0; k < per; ++k) {
-            const u64 b = w * per + k;
-            if (b >= NB) break;
-            const u64 lo = bstart[b], hi = bstart[b + 1];
-            if (FB < 8 && hi - lo > (u64)TSLOTS) {
-                word |= ones << (k * FB);
-                continue;
-            }
-            for (u64 q = lo; q < hi; ++q) word |= filter_bit_mask(b, (u32)(DR[q] >> DR_FP_SHIFT) & 0x3FFFu, FB);
-        }
+        const u64 b1 = filter_bucket_lo(w + 1, FB) < NB ? filter_bucket_lo(w + 1, FB) : NB;
+        for (u64 b = filter_bucket_lo(w, FB); b < b1; ++b)
+            for (u64 q = bstart[b]; q < bstart[b + 1]; ++q) word |= filter_bit_mask((u32)(DR[q] >> DR_FP_SHIFT) & 0x3FFFu);
         F[w] = word;
     }
 }
@@ -918,7 +905,7 @@ constexpr u32 CAND_CHUNK = 256;                 // candidate slots a wave reserv
 constexpr int SPW = 2;                          // segments a wave probes together (one round)
 constexpr int GPW = 2;                          // such groups per wave per tile
 constexpr int SPT = GPW * SPW * (PBP / 64);     // segments per tile
-constexpr int FWMAX = 32768;                    // filter words a block holds (128 KB of LDS)
+constexpr int FWMAX = 38912;                    // filter words a block holds (152 KB of LDS)
 
 __device__ __forceinline__ u64 seg_start(const SegLayout &S, u64 g) { return S.O ? S.O[g] : (g << S.capl); }
 
@@ -951,8 +938,6 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     const u32 fmask = (1u << kh.fpb) - 1u;
     const u64 K = kh_k(kh);
     const u32 rsz = S.hi ? 4u : 8u;   // bytes per record in the bucket-word stream
-    const u32 fbm = FB > 0 ? (u32)FB - 1u : 0u;
-    const int lgfb = FB > 0 ? __builtin_ctz((u32)FB) : 0;   // FB is 4, 8 or 16
     u64 cur_part = ~0ull;
     u32 fbit0 = 0;                               // bit of the partition's first bucket within sF
     // the wave's reserved candidate slots [cbase, cbase + cleft): one counter update per CAND_CHUNK
@@ -1042,23 +1027,15 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     valid |= (u32)(v < fills[sj]) << e;
                 }
                 u32 look = valid;
-                if (FB >= 8) {
-                    // the bucket's word, two bits by the fingerprint (filter_bit_mask)
+                if (FB > 0) {
+                    // the bucket's word (its first bit, counted from the slice's first word, over 32),
+                    // two bits by the fingerprint (filter_bit_mask)
                     u32 pass = 0;
 #pragma unroll
                     for (int e = 0; e < NB2; ++e) {
-                        const u32 w = sF[(fbit0 + ((lo[e] >> kh.fpb) << lgfb)) >> 5];
-                        const u32 f = lo[e] & fmask;
-                        const u32 m = (1u << (f & 31u)) | (1u << ((f >> 5) & 31u));
+                        const u32 w = sF[(fbit0 + (lo[e] >> kh.fpb) * (u32)FB) >> 5];
+                        const u32 m = filter_bit_mask(lo[e] & fmask);
                         pass |= (u32)((w & m) == m) << e;
-                    }
-                    look &= pass;
-                } else if (FB > 0) {
-                    u32 pass = 0;
-#pragma unroll
-                    for (int e = 0; e < NB2; ++e) {
-                        const u32 bit = fbit0 + ((lo[e] >> kh.fpb) << lgfb) + (lo[e] & fbm);
-                        pass |= ((sF[bit >> 5] >> (bit & 31)) & 1u) << e;
                     }
                     look &= pass;
                 }
@@ -1319,24 +1296,19 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         *(uint4 *)(T + (gb + i) * TSLOTS) =
             make_uint4(sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16), sl[4] | (sl[5] << 16), sl[6] | (sl[7] << 16));
     }
-    // filter words: FB bits per bucket; a word shared with a neighbouring window is merged atomically
+    // filter words of the window's buckets; a word shared with a neighbouring window is merged
+    // atomically
     if (FB > 0) {
         const u64 bit0 = gb * (u64)FB, bit1 = (gb + nb) * (u64)FB;
-        const u32 fones = FB >= 32 ? ~0u : (1u << FB) - 1u;
         for (u64 wd = (bit0 >> 5) + threadIdx.x; wd <= ((bit1 - 1) >> 5); wd += DSB) {
+            const u64 wb0 = filter_bucket_lo(wd, FB), wb1 = filter_bucket_lo(wd + 1, FB);
+            const u64 a = wb0 > gb ? wb0 : gb, z = wb1 < gb + nb ? wb1 : gb + nb;
             u32 word = 0;
-            for (u32 q = 0; q < 32u / (u32)FB; ++q) {
-                const u64 gbk = (wd * 32 + (u64)q * FB) / FB;   // global bucket of this field
-                if (gbk < gb || gbk >= gb + nb) continue;
+            for (u64 gbk = a; gbk < z; ++gbk) {
                 const u32 i = (u32)(gbk - gb);
-                const u32 c = pos[i + 1] - pos[i];
-                if (FB < 8 && c > (u32)TSLOTS) {
-                    word |= fones << (q * FB);
-                    continue;
-                }
-                for (u32 e = 0; e < c; ++e) word |= filter_bit_mask(gbk, s_r[s_idx[pos[i] + e]].x & fmask, FB);
+                for (u32 e = pos[i]; e < pos[i + 1]; ++e) word |= filter_bit_mask(s_r[s_idx[e]].x & fmask);
             }
-            const bool edge = wd * 32 < bit0 || wd * 32 + 32 > bit1;
+            const bool edge = wb0 < gb || wb1 > gb + nb;
             if (edge) atomicOr(&F[wd], word);
             else F[wd] = word;
         }
@@ -1632,7 +1604,7 @@ u64 cand_margin(const Tuning &tn) {
 int filter_bits(const KeyHash &kh) {
     // the widest filter whose partition slice (with up to 31 bits of misalignment) fits in LDS
     const u64 K = kh_k(kh);
-    for (int fb = 16; fb >= 4; fb >>= 1)
+    for (int fb = 16; fb >= 4; --fb)
         if (((31 + K * (u64)fb + 31) >> 5) <= (u64)FWMAX) return fb;
     return 0;
 }

@@ -861,27 +861,33 @@ __device__ __forceinline__ bool bucket_hit(const uint4 &a, u32 fs2) {
 
 constexpr int PROBE_BATCH = 8;     // records per thread whose loads are issued before any compare
 
-// Bucket filter: a blocked Bloom filter of FB bits per bucket (4 <= FB <= 16) in 32-bit words.
-// Bucket b belongs to word b * FB / 32 (rounded down), so a word serves about 32 / FB consecutive
-// buckets, which share its 32 bits: each of their D records sets two, at its fingerprint bits 0-4
-// and 5-9. An I record whose two bits are not both set matches no D record of its bucket, so the
+// Bucket filter: a blocked Bloom filter of FB bits per bucket (4 <= FB <= 16) in 32-bit words. Each
+// partition's slice starts on a word: partition p owns words [p * PW, (p + 1) * PW), PW =
+// ceil(K * FB / 32), and its bucket k belongs to word p * PW + k * FB / 32 (rounded down), so a word
+// serves about 32 / FB consecutive buckets of one partition, which share its 32 bits: each of their
+// D records sets two, at its fingerprint bits 0-4 and 5-9. An I record whose two bits are not both set matches no D record of its bucket, so the
 // probe rejects it without reading the table. At 2 D records per bucket about 16% of the records
 // that miss pass at FB = 8 (22% with one bit each in 8-bit per-bucket fields), 14% at FB = 9 and
 // 31% at FB = 5. A bucket with more records than slots has every record's bits set, and the
 // records it passes meet the overflow flag of its slot 7.
 __device__ __forceinline__ u32 filter_bit_mask(u32 fp) { return (1u << (fp & 31u)) | (1u << ((fp >> 5) & 31u)); }
-// the buckets of word w: [bucket_lo(w), bucket_lo(w + 1))
-__device__ __forceinline__ u64 filter_bucket_lo(u64 w, int FB) { return (w * 32 + (u64)FB - 1) / (u64)FB; }
+CF_HD u64 filter_slice_words(u64 K, int FB) { return (K * (u64)FB + 31) >> 5; }
+// the first bucket (within its partition) of word wl of a partition's slice
+__device__ __forceinline__ u64 filter_bucket_lo(u64 wl, int FB) { return (wl * 32 + (u64)FB - 1) / (u64)FB; }
 
 // The filter of a table bucketed by global atomics: one thread per word, over the records of its
 // buckets (their fingerprints from DR).
 __global__ void __launch_bounds__(PB) k_bucket_filter(const u64 *__restrict__ DR, const u64 *__restrict__ bstart,
-                                                      u64 NB, int FB, u32 *__restrict__ F, u64 nwords) {
+                                                      u64 K, int FB, u32 *__restrict__ F, u64 nwords) {
+    const u64 PW = filter_slice_words(K, FB);
     for (u64 w = (u64)blockIdx.x * PB + threadIdx.x; w < nwords; w += (u64)gridDim.x * PB) {
+        const u64 part = w / PW, wl = w - part * PW;
+        const u64 k1 = filter_bucket_lo(wl + 1, FB) < K ? filter_bucket_lo(wl + 1, FB) : K;
         u32 word = 0;
-        const u64 b1 = filter_bucket_lo(w + 1, FB) < NB ? filter_bucket_lo(w + 1, FB) : NB;
-        for (u64 b = filter_bucket_lo(w, FB); b < b1; ++b)
+        for (u64 k = filter_bucket_lo(wl, FB); k < k1; ++k) {
+            const u64 b = part * K + k;
             for (u64 q = bstart[b]; q < bstart[b + 1]; ++q) word |= filter_bit_mask((u32)(DR[q] >> DR_FP_SHIFT) & 0x3FFFu);
+        }
         F[w] = word;
     }
 }
@@ -939,7 +945,6 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
     const u64 K = kh_k(kh);
     const u32 rsz = S.hi ? 4u : 8u;   // bytes per record in the bucket-word stream
     u64 cur_part = ~0ull;
-    u32 fbit0 = 0;                               // bit of the partition's first bucket within sF
     // the wave's reserved candidate slots [cbase, cbase + cleft): one counter update per CAND_CHUNK
     // candidates instead of one per round that has any
     u64 cbase = 0;
@@ -957,9 +962,8 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
             // this block's first tile of a partition: its filter slice into LDS (the barrier above
             // ordered every read of the previous slice before this)
             // (every load of the slice is issued before the first LDS store waits on one)
-            const u64 bit0 = part * K * (u64)FB;
-            const u64 w0 = bit0 >> 5;
-            const u32 nw = (u32)(((bit0 + K * (u64)FB + 31) >> 5) - w0);
+            const u32 nw = (u32)filter_slice_words(K, FB);
+            const u64 w0 = part * nw;
             constexpr int FPT = 16;   // words in flight per thread (a batch of FPT * PBP words)
             for (u32 i0 = 0; i0 < nw; i0 += FPT * PBP) {
                 u32 fw[FPT];
@@ -975,7 +979,6 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     if (i < nw) sF[i] = fw[j];
                 }
             }
-            fbit0 = (u32)(bit0 & 31);
             cur_part = part;
             lds_sync();
         }
@@ -1028,12 +1031,11 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                 }
                 u32 look = valid;
                 if (FB > 0) {
-                    // the bucket's word (its first bit, counted from the slice's first word, over 32),
-                    // two bits by the fingerprint (filter_bit_mask)
+                    // the bucket's word of the slice, two bits by the fingerprint (filter_bit_mask)
                     u32 pass = 0;
 #pragma unroll
                     for (int e = 0; e < NB2; ++e) {
-                        const u32 w = sF[(fbit0 + (lo[e] >> kh.fpb) * (u32)FB) >> 5];
+                        const u32 w = sF[((lo[e] >> kh.fpb) * (u32)FB) >> 5];
                         const u32 m = filter_bit_mask(lo[e] & fmask);
                         pass |= (u32)((w & m) == m) << e;
                     }
@@ -1203,6 +1205,10 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     __shared__ unsigned short s_idx[CAPW];
     __shared__ u32 pos[WMAXB + 1];     // per-bucket counts, then (scanned in place) bucket starts
     __shared__ u32 s_w[DSB / 64];
+    // the window's filter words, built from the records as they are counted (up to FWW words: a
+    // window of 1024 buckets at up to 15 bits each; a larger one takes the per-word loop below)
+    constexpr u32 FWW = 512;
+    __shared__ u32 s_f[FWW];
     u32 *cnt = pos;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const u64 pw = blockIdx.x;
@@ -1214,7 +1220,15 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     const u32 n = wcount[pw] < (u32)CAPW ? wcount[pw] : (u32)CAPW;
     const u64 base = wbase[pw];
     const u64 gb = part * K + b_lo;                            // first bucket, global
+    // the window's filter words: from fw0 (its first bit fb0 bits into it), nfw of them
+    const u64 fw0 = FB > 0 ? part * filter_slice_words(K, FB) + ((b_lo * (u64)FB) >> 5) : 0;
+    const u32 fb0 = (u32)((b_lo * (u64)FB) & 31);
+    const u32 nfw = FB > 0 ? (fb0 + nb * (u32)FB + 31) >> 5 : 0;
+    const bool fl = FB > 0 && nfw <= FWW;
+    const u32 fmask = (1u << kh.fpb) - 1u;
     for (u32 i = threadIdx.x; i <= nb; i += DSB) cnt[i] = 0;
+    if (fl)
+        for (u32 i = threadIdx.x; i < nfw; i += DSB) s_f[i] = 0;
     for (u32 i = threadIdx.x; i < n; i += DSB) s_r[i] = wreg[pw * CAPW + i];
     lds_sync();
     constexpr int RPT = (CAPW + DSB - 1) / DSB;
@@ -1226,9 +1240,14 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
         if (i < n) {
             // a record's window is its bucket offset >> wb, so its bucket is in the window; a record
             // that is not (inconsistent input) is reported through kerr and the window left unsorted
-            const u32 bi = (s_r[i].x >> kh.fpb) - (u32)b_lo;
-            if (bi < nb) rk[k] = atomicAdd(&cnt[bi], 1u);
-            else bad = true;
+            const u32 x = s_r[i].x;
+            const u32 bi = (x >> kh.fpb) - (u32)b_lo;
+            if (bi < nb) {
+                rk[k] = atomicAdd(&cnt[bi], 1u);
+                if (fl) atomicOr(&s_f[(fb0 + bi * (u32)FB) >> 5], filter_bit_mask(x & fmask));
+            } else {
+                bad = true;
+            }
         }
     }
     if (__syncthreads_or(bad)) {
@@ -1284,7 +1303,6 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
                        (u64)(r.y >> DREC_EX_SHIFT) << DR_EX_SHIFT;
     }
     // table slots: one bucket per thread, 16 bytes each
-    const u32 fmask = (1u << kh.fpb) - 1u;
     for (u32 i = threadIdx.x; i < nb; i += DSB) {
         const u32 c = pos[i + 1] - pos[i];
         u32 sl[TSLOTS];
@@ -1298,17 +1316,28 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
     }
     // filter words of the window's buckets; a word shared with a neighbouring window is merged
     // atomically
-    if (FB > 0) {
-        const u64 bit0 = gb * (u64)FB, bit1 = (gb + nb) * (u64)FB;
-        for (u64 wd = (bit0 >> 5) + threadIdx.x; wd <= ((bit1 - 1) >> 5); wd += DSB) {
-            const u64 wb0 = filter_bucket_lo(wd, FB), wb1 = filter_bucket_lo(wd + 1, FB);
-            const u64 a = wb0 > gb ? wb0 : gb, z = wb1 < gb + nb ? wb1 : gb + nb;
+    if (fl) {
+        const bool open_end = ((fb0 + nb * (u32)FB) & 31u) != 0;   // the last word may hold the next window's bits
+        for (u32 j = threadIdx.x; j < nfw; j += DSB) {
+            const bool edge = (j == 0 && fb0 != 0) || (j + 1 == nfw && open_end);
+            if (edge) atomicOr(&F[fw0 + j], s_f[j]);
+            else F[fw0 + j] = s_f[j];
+        }
+    } else if (FB > 0) {
+        for (u32 j = threadIdx.x; j < nfw; j += DSB) {
+            const u64 wd = fw0 + j;
+            // the word's buckets as offsets from the window's first, in 32-bit arithmetic: its first
+            // bit is r0 bits past the window's first (-32 < r0 < nb * FB), so its buckets are
+            // [ceil(r0 / FB), ceil((r0 + 32) / FB)), cut to the window
+            const int r0 = (int)(j * 32) - (int)fb0;
+            const u32 a = r0 <= 0 ? 0u : ((u32)r0 + (u32)FB - 1u) / (u32)FB;
+            const u32 z0 = ((u32)(r0 + 32) + (u32)FB - 1u) / (u32)FB;
+            const u32 z = z0 < nb ? z0 : nb;
             u32 word = 0;
-            for (u64 gbk = a; gbk < z; ++gbk) {
-                const u32 i = (u32)(gbk - gb);
+            for (u32 i = a; i < z; ++i)
                 for (u32 e = pos[i]; e < pos[i + 1]; ++e) word |= filter_bit_mask(s_r[s_idx[e]].x & fmask);
-            }
-            const bool edge = wb0 < gb || wb1 > gb + nb;
+            // a word that may hold another window's buckets is merged atomically
+            const bool edge = r0 < 0 || z0 > nb;
             if (edge) atomicOr(&F[wd], word);
             else F[wd] = word;
         }
@@ -1602,20 +1631,20 @@ u64 cand_margin(const Tuning &tn) {
 }
 
 int filter_bits(const KeyHash &kh) {
-    // the widest filter whose partition slice (with up to 31 bits of misalignment) fits in LDS
+    // the widest filter whose partition slice fits in LDS
     const u64 K = kh_k(kh);
     for (int fb = 16; fb >= 4; --fb)
-        if (((31 + K * (u64)fb + 31) >> 5) <= (u64)FWMAX) return fb;
+        if (filter_slice_words(K, fb) <= (u64)FWMAX) return fb;
     return 0;
 }
 
-u64 filter_words(const KeyHash &kh, int FB) { return FB ? (kh_nb(kh) * (u64)FB + 31) / 32 : 0; }
+u64 filter_words(const KeyHash &kh, int FB) { return FB ? ((u64)1 << kh.lgP) * filter_slice_words(kh_k(kh), FB) : 0; }
 
 void part_filter(cf_ctx *ctx, const u64 *DR, const u64 *bstart, const KeyHash &kh, int FB, u32 *F) {
     const u64 nw = filter_words(kh, FB);
     if (!nw) return;
     CF_LAUNCH(ctx, "k_bucket_filter", k_bucket_filter, dim3(grid_for(nw, PB, 65536)), dim3(PB), 0, DR, bstart,
-              kh_nb(kh), FB, F, nw);
+              kh_k(kh), FB, F, nw);
 }
 
 void part_probe_pass1(cf_ctx *ctx, const KeyHash &kh, const SegLayout &S, const u64 *rec, const unsigned short *T,

@@ -585,14 +585,15 @@ def main():
     print(f"[bench] warm-up done: {keys:.4g} keys, logged-count mismatches batched={len(bad_b)} plain={len(bad_p)}, "
           f"batched == plain pair sets: {same}", file=sys.stderr, flush=True)
 
-    # ---- autotuner (untimed): slice per partition, D records per bucket, segment fill, probe grid
+    # ---- autotuner (untimed): slice per partition, D records per bucket, segment fill, probe grid;
+    # each candidate timed the way the step runs (graph replays unless --no-graph)
     tuning = {"setting": dict(zip(("slice_kib", "d_per_bucket", "seg_fill_pct", "probe_blocks"), pinned)),
               "pinned": True} if pinned else None
     if not args.no_tune and not pinned:
         from cfjoin import autotune
-        tr = autotune.tune(batch, reps=3)
+        tr = autotune.tune(batch, reps=5, graph=not args.no_graph)
         tuning = {"setting": tr["best"], "ms": round(tr["ms"], 3), "default_ms": round(tr["default_ms"], 3),
-                  "trials": len(tr["trials"])}
+                  "trials": len(tr["trials"]), "timed_as": "eager solve" if args.no_graph else "graph replay"}
         print(f"[bench] tuned: {tr['best']} {tr['ms']:.2f} ms (default {tr['default_ms']:.2f} ms)", file=sys.stderr,
               flush=True)
 

@@ -6,6 +6,10 @@ Search: coordinate descent from the defaults -- one knob at a time over its cand
 others held at their best so far, each candidate timed as the best of `reps` solves. Every candidate's
 result is compared with the default setting's (per-instance record counts and pair sets): a knob
 may only change speed, so a difference is an error, not a choice.
+
+With graph=True each candidate is timed as the captured solve the batch will run (cf_batch_capture
+on a copy of the batch, then replays): in a replay the radius groups run side by side and share the
+L2 slices, so the best eager setting is not always the best replayed one.
 """
 from __future__ import annotations
 
@@ -42,13 +46,35 @@ def _time(batch, setting: Dict[str, int], reps: int, ctx: L.Context):
     return best * 1e3, res
 
 
-def tune(batch, reps: int = 2, candidates: Optional[Dict[str, List[int]]] = None, log=None) -> Dict:
+def _time_graph(batch, setting: Dict[str, int], reps: int, ctx: L.Context):
+    from . import pipeline
+    b = pipeline.Batch(batch.insts, ctx)
+    try:
+        b.set_tuning(**setting)
+        b.capture()
+        res, _ = b.solve()
+        best = float("inf")
+        for _ in range(reps):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            b.solve()
+            ctx.synchronize()
+            best = min(best, time.perf_counter() - t0)
+    finally:
+        b.close()
+    return best * 1e3, res
+
+
+def tune(batch, reps: int = 2, candidates: Optional[Dict[str, List[int]]] = None, log=None,
+         graph: bool = False) -> Dict:
     """Tune a prepared, not yet captured pipeline.Batch in place; returns
-    {"best": setting, "ms": best time, "default_ms": ..., "trials": [(setting, ms), ...]}."""
+    {"best": setting, "ms": best time, "default_ms": ..., "trials": [(setting, ms), ...]}.
+    graph: time each candidate as a captured solve (replays) rather than as an eager one."""
     cands = candidates or CANDIDATES
     ctx = batch.ctx
+    timer = _time_graph if graph else _time
     best = dict(DEFAULTS)
-    best_ms, ref = _time(batch, best, reps, ctx)
+    best_ms, ref = timer(batch, best, reps, ctx)
     ref_sig = _signature(ref)
     default_ms = best_ms
     trials = [(dict(best), best_ms)]
@@ -57,7 +83,7 @@ def tune(batch, reps: int = 2, candidates: Optional[Dict[str, List[int]]] = None
             if v == best[knob]:
                 continue
             setting = dict(best, **{knob: v})
-            ms, res = _time(batch, setting, reps, ctx)
+            ms, res = timer(batch, setting, reps, ctx)
             if _signature(res) != ref_sig:
                 raise AssertionError(f"tuning {setting} changed the result of the solve")
             trials.append((setting, ms))

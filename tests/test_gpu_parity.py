@@ -459,6 +459,27 @@ def test_autotune_only_changes_speed(workloads):
     b.close()
 
 
+def test_autotune_graph_timing_only_changes_speed(workloads):
+    """The tuner timing captured replays (graph=True): each candidate is captured on a copy of the
+    batch and replayed, its result compared with the default setting's; the batch itself stays
+    uncaptured and takes the chosen setting."""
+    from cfjoin import autotune
+    from cfjoin import workloads as W
+    logged = W.logged_table(GOLDEN_LOGGED)
+    insts = W.batch_instances("lambdaN", radii=(4, 5), logged=logged, limit=12)
+    say("tune graph", len(insts))
+    eager, _ = pipeline.solve_batch(insts)
+    b = pipeline.Batch(insts)
+    tr = autotune.tune(b, reps=1, graph=True, candidates={"slice_kib": [512], "seg_fill_pct": [90]})
+    assert len(tr["trials"]) == 3
+    assert b.tuning == tr["best"]
+    b.capture()
+    res, _ = b.solve()
+    for ra, rb in zip(eager, res):
+        assert (ra.nD, ra.nI) == (rb.nD, rb.nI) and np.array_equal(ra.pair_array(), rb.pair_array())
+    b.close()
+
+
 def test_server_results_equal_alone(workloads):
     """Requests served together by the continuous-batching server (several workers, batches of
     whatever has arrived) get exactly the counts and pair sets they get solved alone."""

@@ -13,10 +13,10 @@
 // 2 MB).
 //
 // Probe table T: 8 slots of 16 bits per bucket (16 bytes), slot k holding 0x8000 | f of the
-// bucket's k-th D record (f cut to the fpb fingerprint bits an I record keeps); slot 7 also carries 0x4000 when the bucket holds more than 8 records
-// (rare at 2 per bucket: such a bucket makes every probe of it a candidate). Bucket filter F: a
-// few bits per bucket, one per class of fingerprint low bits present; the probe keeps its
-// partition's slice of F in LDS and reads T only for records whose class is present.
+// bucket's k-th D record (f cut to the fpb fingerprint bits an I record keeps, at most 15); slot 7
+// holds T_OVERFLOW instead when the bucket holds more than 8 records (rare at 2 per bucket: such a
+// bucket makes every probe of it a candidate). Bucket filter F: a few bits per bucket (below); the
+// probe keeps its partition's slice of F in LDS and reads T only for records the filter passes.
 //
 //  D side: records counted per bucket and scattered into bucket order (DR: the D leaf index; a
 //          record's exact key is recomputed from it when a candidate meets it); the scatter writes
@@ -42,11 +42,13 @@ constexpr int LT = LPB * LPT;      // 4096 leaves per tile
 constexpr int NXCD = 8;            // MI355X: 8 accelerator complex dies, blocks dealt round-robin
 constexpr int TSLOTS = CF_TABLE_SLOTS;   // table slots per bucket
 // A DR entry: the D record's leaf index (relative to its table's first leaf), above it the record's
-// fingerprint (the fpb bits its table slot holds) and 4 more bits of its hash (hash bits 12-15, used
-// by neither the bucket nor the fingerprint), so that a candidate recomputes only the keys of records
-// that agree with it on 18 hash bits.
+// fingerprint (the fpb bits its table slot holds, 15 bits of room) and 4 more bits of its hash (hash
+// bits 12-15, used by neither the bucket nor the fingerprint), so that a candidate recomputes only
+// the keys of records that agree with it on fpb + 4 hash bits.
 constexpr int DR_FP_SHIFT = 40;
-constexpr int DR_EX_SHIFT = 54;
+constexpr int DR_EX_SHIFT = 55;
+constexpr u32 DR_FP_MASK = 0x7FFFu;
+constexpr unsigned short T_OVERFLOW = 0x0001;   // slot 7 of a bucket of more than 8 records (no valid bit)
 constexpr u64 DR_LEAF_MASK = ((u64)1 << DR_FP_SHIFT) - 1;
 __device__ __forceinline__ u32 hash_ex(u64 h) { return (u32)(h >> 12) & 15u; }
 // A window-built D table holds fewer than 2^28 records, so its records' leaf indices leave their top
@@ -548,8 +550,8 @@ __global__ void __launch_bounds__(LPB) k_bucket_scatter(LeafSrc L, KeyHash kh, c
             // the leaf relative to the source's first, and its fingerprint
             DR[b0 + slot] = (tl.j0 - L.lo + i) | (u64)(hfp(h) & fmask) << DR_FP_SHIFT | (u64)hash_ex(h) << DR_EX_SHIFT;
             if (slot < (u32)TSLOTS) {
-                const u32 more = (slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS) ? 0x4000u : 0u;
-                T[b * TSLOTS + slot] = (unsigned short)(0x8000u | more | (hfp(h) & fmask));
+                const bool over = slot == TSLOTS - 1 && bstart[b + 1] - b0 > (u64)TSLOTS;
+                T[b * TSLOTS + slot] = over ? T_OVERFLOW : (unsigned short)(0x8000u | (hfp(h) & fmask));
             }
         }
         lds_sync();
@@ -846,9 +848,9 @@ __device__ __forceinline__ void leaf_exact(const LeafSrc &L, u64 leaf, u64 &key,
 }
 
 // Is the slot pattern fs2 (0x8000 | f in both halves of a word) among the 8 slots of a bucket? Slots
-// hold 0x8000 | f with f cut to the record's fingerprint bits (slot 7 of an overflowing bucket also
-// has 0x4000 set, so it matches nothing), and a slot matches when it XOR the pattern is zero, so
-// the 8 slots reduce to one packed 16-bit minimum.
+// hold 0x8000 | f with f cut to the record's fingerprint bits (slot 7 of an overflowing bucket holds
+// T_OVERFLOW, without the valid bit, so it matches nothing), and a slot matches when it XOR the
+// pattern is zero, so the 8 slots reduce to one packed 16-bit minimum.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ bool bucket_hit(const uint4 &a, u32 fs2) {
     const u16x2 x0 = __builtin_bit_cast(u16x2, a.x ^ fs2);
@@ -886,7 +888,7 @@ __global__ void __launch_bounds__(PB) k_bucket_filter(const u64 *__restrict__ DR
         u32 word = 0;
         for (u64 k = filter_bucket_lo(wl, FB); k < k1; ++k) {
             const u64 b = part * K + k;
-            for (u64 q = bstart[b]; q < bstart[b + 1]; ++q) word |= filter_bit_mask((u32)(DR[q] >> DR_FP_SHIFT) & 0x3FFFu);
+            for (u64 q = bstart[b]; q < bstart[b + 1]; ++q) word |= filter_bit_mask((u32)(DR[q] >> DR_FP_SHIFT) & DR_FP_MASK);
         }
         F[w] = word;
     }
@@ -1051,12 +1053,12 @@ __global__ void __launch_bounds__(PBP) k_part_probe(KeyHash kh, SegLayout S, con
                     ta[e] = make_uint4(x[0], x[1], x[2], x[3]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
-                // a fingerprint among the bucket's slots, or the overflow flag of slot 7, is a candidate
+                // a fingerprint among the bucket's slots, or the overflow mark in slot 7, is a candidate
                 u32 hmask = 0;
 #pragma unroll
                 for (int e = 0; e < NB2; ++e) {
                     const u32 fs2 = (0x8000u | (lo[e] & fmask)) * 0x10001u;
-                    hmask |= (u32)(bucket_hit(ta[e], fs2) | ((ta[e].w & 0x40000000u) != 0)) << e;
+                    hmask |= (u32)(bucket_hit(ta[e], fs2) | ((ta[e].w >> 16) == T_OVERFLOW)) << e;
                 }
                 hmask &= look;
                 if (__any(hmask != 0)) {
@@ -1309,7 +1311,7 @@ __global__ void __launch_bounds__(DSB) k_dwin_sort(KeyHash kh, DWin dw, const u3
 #pragma unroll
         for (int q = 0; q < TSLOTS; ++q)
             sl[q] = (u32)q < c ? (0x8000u | (s_r[s_idx[pos[i] + q]].x & fmask)) : 0u;
-        if (c > (u32)TSLOTS) sl[TSLOTS - 1] |= 0x4000u;
+        if (c > (u32)TSLOTS) sl[TSLOTS - 1] = T_OVERFLOW;
         static_assert(TSLOTS == 8, "a bucket is one 16-byte store");
         *(uint4 *)(T + (gb + i) * TSLOTS) =
             make_uint4(sl[0] | (sl[1] << 16), sl[2] | (sl[3] << 16), sl[4] | (sl[5] << 16), sl[6] | (sl[7] << 16));
@@ -1391,7 +1393,7 @@ __global__ void __launch_bounds__(PB) k_probe_candidates(LeafSrc L, LeafSrc LD0,
         const u64 lo = bstart[b], hi = bstart[b + 1];
         for (u64 q = lo; q < hi; ++q) {
             const u64 dr = DR[q];
-            if (((dr >> DR_FP_SHIFT) & 0x3FFFu) != fp) continue;
+            if (((dr >> DR_FP_SHIFT) & DR_FP_MASK) != fp) continue;
             if (!have) {
                 u64 h;
                 leaf_exact(L, r.y, key, iid, &h);

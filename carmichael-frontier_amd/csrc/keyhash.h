@@ -8,14 +8,14 @@
 // so range d owns exactly the buckets [d * KR, (d + 1) * KR) and partition p the buckets
 // [p * K, (p + 1) * K), K = KR << (lgR - lgP): a bucket never straddles a range or a partition,
 // and the D side can be counting-sorted one range at a time. NB = KR << lgR buckets in all.
-//   fingerprint f = bits 17..30 of h (14 bits, below every bit that places the record)
+//   fingerprint f = bits 17..31 of h (15 bits, below every bit that places the record)
 #pragma once
 #include "modarith.h"
 
 struct KeyHash {
     int lgP;    // 4 <= lgP <= CF_MAX_LGP
     int lgR;    // lgP <= lgR <= 24
-    int fpb;    // fingerprint bits kept in an I record (<= 14)
+    int fpb;    // fingerprint bits kept in an I record (<= 15)
     int pad;
     u64 KR;     // buckets per range
 };
@@ -36,7 +36,7 @@ CF_HD u64 hbucket(u64 h, const KeyHash &kh) {
 }
 // bucket offset within the record's partition, < K
 CF_HD u32 hoff(u64 h, const KeyHash &kh) { return (u32)(hbucket(h, kh) - (u64)hpart(h, kh.lgP) * kh_k(kh)); }
-CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x3FFFu; }
+CF_HD u32 hfp(u64 h) { return (u32)(h >> 17) & 0x7FFFu; }
 // The same offset from the high word of h in 32-bit arithmetic: the range within the partition
 // times KR, plus floor(r * KR / 2^(32 - lgR)) taken as the high word of (r << lgR) * KR (r has
 // 32 - lgR bits, so r << lgR fits 32 bits; KR < 2^32 and the offset < K <= 2^24).
@@ -51,7 +51,7 @@ CF_HD u32 hoff32(u32 hi, const KeyHash &kh) {
 // as keep one partition's slice of the 16-byte-per-bucket table within about 2 MB by default (an
 // XCD's L2 holds it while the partition is probed), 16 <= P <= 256; as few hash ranges as keep a
 // range within 8192 buckets (at most 2^12 ranges). An I record keeps the bucket offset within its
-// partition and 32 - that many bits of fingerprint, at most 14. Returns false if a partition would
+// partition and 32 - that many bits of fingerprint, at most 15. Returns false if a partition would
 // need more than 2^24 buckets.
 inline int kh_bitlen(u64 v) {
     int b = 0;
@@ -79,7 +79,7 @@ inline bool kh_choose_at(u64 nD, KeyHash &kh, u64 dpb, u64 slice_bytes) {
     kh.lgP = lgP;
     kh.lgR = lgR;
     kh.KR = KR;
-    kh.fpb = 32 - obits < 14 ? 32 - obits : 14;
+    kh.fpb = 32 - obits < 15 ? 32 - obits : 15;
     kh.pad = 0;
     return obits <= 24;
 }

@@ -470,8 +470,11 @@ def test_autotune_graph_timing_only_changes_speed(workloads):
     say("tune graph", len(insts))
     eager, _ = pipeline.solve_batch(insts)
     b = pipeline.Batch(insts)
-    tr = autotune.tune(b, reps=1, graph=True, candidates={"slice_kib": [512], "seg_fill_pct": [90]})
-    assert len(tr["trials"]) == 3
+    # one record per bucket doubles the window width (2048 buckets): at more than 8 filter bits per
+    # bucket a window's filter words then exceed the window sort's LDS array and take its per-word path
+    tr = autotune.tune(b, reps=1, graph=True, candidates={"slice_kib": [512], "d_per_bucket": [1],
+                                                          "seg_fill_pct": [90]})
+    assert len(tr["trials"]) == 4
     assert b.tuning == tr["best"]
     b.capture()
     res, _ = b.solve()

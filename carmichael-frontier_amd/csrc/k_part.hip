@@ -6,8 +6,11 @@
 // prime of the instance is below 2^32, so a leaf's residue state is (xm * 2^-32 mod m, x2 mod 2^a)
 // on both sides and equal residues still meet.
 //
-// Hash of a leaf: h = (xm + rot32(x2 mod 2^a) + koff) * 0x9E3779B97F4A7C15 -- no CRT step is needed
-// to place a record, only to compare one exactly. h gives the record's partition, hash range,
+// Hash of a leaf: v = (xm + rot32(x2 mod 2^a) + koff) * 0x9E3779B97F4A7C15, then h = (v ^ v >> 32) *
+// 0xD6E8FEB86659FD93 -- no CRT step is needed to place a record, only to compare one exactly. The
+// second round breaks up the structure a single multiplicative hash keeps from consecutive leaves'
+// keys (products sharing all but their last prime), which piled one block's records of a large D
+// side into a few partitions. h gives the record's partition, hash range,
 // bucket and fingerprint (keyhash.h); NB is chosen for about 2 D records per bucket, and a
 // partition's buckets are contiguous, so its slice of the probe table is NB * 16 / P bytes (about
 // 2 MB).
@@ -88,7 +91,8 @@ __device__ __forceinline__ void leaf_residue(const SideDesc &s, const LeafNode &
 }
 __device__ __forceinline__ u64 leaf_hash(u64 xm, u64 x2, u64 mask2, u64 koff) {
     const u64 x2m = x2 & mask2;
-    return (xm + ((x2m << 32) | (x2m >> 32)) + koff) * 0x9E3779B97F4A7C15ULL;
+    const u64 v = (xm + ((x2m << 32) | (x2m >> 32)) + koff) * 0x9E3779B97F4A7C15ULL;
+    return (v ^ (v >> 32)) * 0xD6E8FEB86659FD93ULL;
 }
 
 // ---- leaf tiles ------------------------------------------------------------------------------

@@ -343,9 +343,10 @@ def test_batch_equals_alone(workloads):
 
 @pytest.mark.parametrize("M", [64, 1024, 2 ** 20 + 7])
 def test_solve_heavy_collisions(M):
-    """Tiny moduli: keys repeat thousands of times, so buckets overflow their 16 table slots and the
-    probe finds more candidates than its first allowance (it is repeated with room for all); the
-    pair multiset must still equal the oracle join, with the D side bucketed either way."""
+    """Tiny moduli: keys repeat thousands of times, so buckets overflow their 8 table slots (slot 7
+    holds the overflow mark and every probe of such a bucket is a candidate) and the probe finds more
+    candidates than its first allowance (it is repeated with room for all); the pair multiset must
+    still equal the oracle join, with the D side bucketed either way."""
     P = [p for p in range(3, 2000) if all(p % q for q in range(2, int(p ** 0.5) + 1))]
     B0, INS = P[:20], P[20:80]
     inst = I.Instance(M=M, r=3, B0=B0, INS=INS, P0=prod(B0), Icap=prod(INS[-3:]) + 1, Dmin=1, split=None, hmax=3,
